@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: MUSCL-HLL finite-volume steps on one MI355X.
+
+Workload (the reference's 2-D configuration): shallow water on a 2048 x 2048
+quad mesh stored unstructured (4,194,304 cells, 8,392,704 faces), circular
+dam break (h = 2 in r < 0.25, else 1, u = v = 0) over 32 seeded Gaussian
+bumps (amplitude U[0, 0.3], width U[0.02, 0.1], seed 2048), LSQ-MUSCL with the
+minmod limiter, HLL flux with hydrostatic reconstruction, SSP-RK2, reflective
+walls, CFL 0.9 (0.45 effective in 2-D).  A step is one full time step:
+both RK stages, the dt reduction and the post_step aux refresh, all on the
+device.  Inputs are resident in HBM when the timed region starts.
+
+Prints ONE JSON line.  value = cell-updates/s in FP64 (the reference's
+precision); the FP32 path is reported beside it under "fp32".
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "zoomy-jax_amd"))
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM = 8.0e12           # MI355X HBM3E, bytes/s
+NX = NY = 2048
+
+
+def algorithmic_bytes(w):
+    """Algorithmic bytes per cell, quad mesh (k = 4 face neighbours, F/N = 2):
+    slope pass 28w + 16 B, flux stage 1 30w + 16 B, flux stage 2 33w + 16 B;
+    a cell-update (two stages) is 119w + 64 B."""
+    return {"slopes": 28 * w + 16, "flux_stage1": 30 * w + 16,
+            "flux_stage2": 33 * w + 16, "cell_update": 119 * w + 64}
+
+
+def build_state(mesh):
+    import numpy as np
+    rng = np.random.default_rng(2048)
+    c = rng.uniform(0.0, 1.0, (32, 2))
+    a = rng.uniform(0.0, 0.3, 32)
+    wd = rng.uniform(0.02, 0.1, 32)
+    xc = mesh.cell_centers()
+    b = np.zeros(mesh.n_cells)
+    for k in range(32):
+        r2 = ((xc - c[k]) ** 2).sum(1)
+        b += a[k] * np.exp(-r2 / (2.0 * wd[k] ** 2))
+    r = np.sqrt(((xc - 0.5) ** 2).sum(1))
+    Q = np.zeros((4, mesh.n_cells))
+    Q[0] = b
+    Q[1] = np.where(r < 0.25, 2.0, 1.0)
+    return Q
+
+
+def gpu_leg(mesh, Q0, dtype, steps, warmup):
+    import torch
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    model = ShallowWater(dimension=2, order=2, limiter="minmod",
+                         riemann="hll", boundary_conditions="wall")
+    s = HyperbolicSolver(time_end=float("inf"), cfl=0.9, dtype=dtype)
+    s.setup_simulation(mesh, model)
+    Q = torch.as_tensor(Q0, dtype=dtype, device="cuda").contiguous()
+    Qaux = torch.zeros((7, mesh.n_cells), dtype=dtype, device="cuda")
+    s.march_device(Q, warmup, Qaux=Qaux)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, t_dev, dt_dev = s.march_device(Q, steps, Qaux=Qaux)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    troubled = s.troubled_count()
+    sim_t = float(t_dev)
+    # per-kernel durations with HIP events on the launch stream
+    dt = float(dt_dev)
+    ms = s.time_kernels(Q, dt, reps=20)
+    mass = float(s.compute_mass(Q))
+    finite = bool(torch.isfinite(Q).all())
+    del s
+    return dict(elapsed=el, ms=ms, sim_t=sim_t, troubled=troubled, mass=mass,
+                finite=finite)
+
+
+def cpu_leg(budget_s=12.0):
+    """The FP64 C oracle (scalar, 1 thread) on a bounded sample: a 256 x 256
+    mesh of the same workload, as many steps as fit in ~budget_s."""
+    import numpy as np
+    from oracle.oracle import Oracle
+    from zoomy_hip.mesh import Mesh
+    m = Mesh.quad(256, 256).set_bc("wall")
+    Q = build_state(m)
+    o = Oracle(m, order=2, limiter="minmod", riemann="hll", cfl=0.9)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        dt = o.dt(Q)
+        Q, _ = o.step(dt, Q)
+        o.aux(Q)
+        k += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=k * m.n_cells / el, steps=k, cells=m.n_cells,
+                elapsed=el)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the multi-GPU halo exchange is "
+              "out of scope for this build (see DESIGN.md); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from zoomy_hip.mesh import Mesh
+
+    mesh = Mesh.quad(NX, NY).set_bc("wall")
+    Q0 = build_state(mesh)
+    N = mesh.n_cells
+    r64 = gpu_leg(mesh, Q0, torch.float64, args.steps, args.warmup)
+    r32 = None if args.no_fp32 else gpu_leg(mesh, Q0, torch.float32,
+                                             args.steps, args.warmup)
+    cpu = None if args.no_cpu else cpu_leg()
+
+    def summarize(r, w):
+        cups = N * args.steps / r["elapsed"]
+        B = algorithmic_bytes(w)
+        ms = r["ms"]
+        dom = max(("slopes", "flux_stage1", "flux_stage2"),
+                  key=lambda k: ms[k])
+        ach = B[dom] * N / (ms[dom] * 1e-3)
+        return cups, dict(
+            bound="hbm", kernel=dom, achieved=ach / 1e9, peak=PEAK_HBM / 1e9,
+            unit="GB/s", frac=ach / PEAK_HBM, traffic=None,
+            bytes_per_launch=B[dom] * N, launch_ms=ms[dom],
+            step_alg_bytes_per_cell=B["cell_update"],
+            step_frac=cups * B["cell_update"] / PEAK_HBM)
+
+    cups64, roof64 = summarize(r64, 8)
+    out = {
+        "metric": "cell-updates/s MUSCL-HLL",
+        "value": cups64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["elapsed"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2-D SWE circular dam break over 32 Gaussian "
+                               "bumps, 2048x2048 quads stored unstructured, "
+                               "LSQ-MUSCL minmod + HLL + SSP-RK2, walls",
+                   "cells": N, "faces": 8392704, "parallelism": "single"},
+        "roofline": roof64,
+        "kernel_ms": r64["ms"],
+        "checks": {"finite": r64["finite"], "troubled": r64["troubled"],
+                   "sim_time": r64["sim_t"], "mass": r64["mass"]},
+    }
+    if r32 is not None:
+        cups32, roof32 = summarize(r32, 4)
+        out["fp32"] = {"value": cups32,
+                       "ms_per_step": r32["elapsed"] * 1e3 / args.steps,
+                       "roofline": roof32, "kernel_ms": r32["ms"],
+                       "checks": {"finite": r32["finite"],
+                                  "troubled": r32["troubled"],
+                                  "mass": r32["mass"]}}
+    if cpu is not None:
+        out["cpu_baseline"] = {
+            "value": cpu["value"], "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"FP64 C oracle, same workload on a 256x256 mesh, "
+                      f"{cpu['steps']} steps in {cpu['elapsed']:.1f} s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

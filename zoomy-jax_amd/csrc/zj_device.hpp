@@ -1,0 +1,187 @@
+// Device-side numerics of the finite-volume step, shared by every kernel.
+//
+// Formulas (all per face, state rows b, h, hu, hv):
+//  * hinv = sqrt(2) h / sqrt(h^4 + max(h, eps)^4)        (KP desingularised
+//    1/h; zoomy_jax/transformation/jax_runtime.py:446-448)
+//  * face flux in the reference's (F, Dp, Dm) split, cell update
+//    dQ_A -= (F + Dm) |f| / |A|  (zoomy_jax/fvm/solver_jax.py:799-809)
+//  * Rusanov: centred flux + path-conservative B = g h db n integral and
+//    s_M I dissipation (zoomy_jax/fvm/nonconservative_flux.py:61-186); the
+//    positive variant dissipates h by the eta jump and never touches b
+//  * HLL with hydrostatic reconstruction (Audusse et al. 2004)
+//  * cell-edge nonconservative integral (solver_jax.py:811-825)
+//  * limiter candidates (zoomy_jax/fvm/reconstruction_jax.py:603-691)
+//
+// A gather kernel evaluates each interior face twice (once from each side),
+// with the face normal pointing out of the evaluating cell; every formula
+// here is antisymmetric under (L <-> R, n -> -n), so the two evaluations are
+// the reference's A-side and B-side updates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace zj {
+
+enum { LIM_NONE = 0, LIM_BJ = 1, LIM_MINMOD = 2, LIM_VK = 3, LIM_VANLEER = 4 };
+enum { RS_RUSANOV_NC = 0, RS_RUSANOV = 1, RS_HLL = 2 };
+enum { RECON_CONS = 0, RECON_ETA = 1 };
+enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2 };
+
+constexpr int MAXF = 4;      // faces per cell (quad)
+constexpr int BLOCK = 256;   // threads per block (4 wave64)
+
+template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T clip01(T x) {
+    return x < T(0) ? T(0) : (x > T(1) ? T(1) : x);
+}
+
+template <typename T> __device__ __forceinline__ T hinv_of(T h, T eps) {
+    T m = tmax(h, eps);
+    T m2 = m * m, h2 = h * h;
+    return T(1.4142135623730951) * h / sqrt(h2 * h2 + m2 * m2);
+}
+
+// Ghost state at a boundary face from the inner state q (Wall mirrors the
+// normal momentum, Extrapolation copies).
+template <typename T>
+__device__ __forceinline__ void bc_ghost(int kind, const T* q, T nx, T ny, T* o) {
+    o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[3];
+    if (kind == BC_WALL) {
+        T qn = q[2] * nx + q[3] * ny;
+        o[2] = q[2] - T(2) * qn * nx;
+        o[3] = q[3] - T(2) * qn * ny;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T lim_cand(int lim, T d, T dp, T dn, T ev2) {
+    const T eps = T(1e-30);
+    if (lim == LIM_NONE) return T(1);
+    if (lim == LIM_VK) {
+        if (d > eps) {
+            T num = dp * dp + ev2 + T(2) * d * dp;
+            T den = dp * dp + T(2) * d * d + d * dp + ev2;
+            return clip01(num / tmax(den, eps));
+        }
+        if (d < -eps) {
+            T num = dn * dn + ev2 + T(2) * d * dn;
+            T den = dn * dn + T(2) * d * d + d * dn + ev2;
+            return clip01(-num / tmin(den, -eps));
+        }
+        return T(1);
+    }
+    if (lim == LIM_VANLEER) {
+        T y;
+        if (d > eps) y = dp / d;
+        else if (d < -eps) y = dn / d;
+        else return T(1);
+        if (y <= T(0)) return T(0);
+        return y >= T(2) ? T(1) : y * (T(4) - y) * T(0.25);
+    }
+    // Barth-Jespersen; the reference's minmod is the same clipped ratio
+    if (d > eps) return clip01(dp / tmax(d, eps));
+    if (d < -eps) return clip01(dn / tmin(d, -eps));
+    return T(1);
+}
+
+// |u.n| + sqrt(g h) with u = q * hinv (the cell aux)
+template <typename T>
+__device__ __forceinline__ T max_abs_eig(const T* q, T hinv, T nx, T ny, T g) {
+    T un = (q[2] * nx + q[3] * ny) * hinv;
+    return fabs(un) + sqrt(g * tmax(q[1], T(0)));
+}
+
+// F + Dm for the evaluating cell (left state qL, neighbour qR, outward n).
+template <typename T>
+__device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
+                                               T hiL, T hiR, T nx, T ny, T g,
+                                               T* out) {
+    if (rs == RS_HLL) {
+        T bs = tmax(qL[0], qR[0]);
+        T hLs = tmax(T(0), qL[1] + qL[0] - bs);
+        T hRs = tmax(T(0), qR[1] + qR[0] - bs);
+        T uL = qL[2] * hiL, vL = qL[3] * hiL;
+        T uR = qR[2] * hiR, vR = qR[3] * hiR;
+        T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+        T cL = sqrt(g * hLs), cR = sqrt(g * hRs);
+        T sL = tmin(unL - cL, unR - cR);
+        T sR = tmax(unL + cL, unR + cR);
+        T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
+        T FL0 = hLs * unL, FL1 = hLs * uL * unL + pL * nx, FL2 = hLs * vL * unL + pL * ny;
+        T FR0 = hRs * unR, FR1 = hRs * uR * unR + pR * nx, FR2 = hRs * vR * unR + pR * ny;
+        T F0, F1, F2;
+        if (sL >= T(0)) {
+            F0 = FL0; F1 = FL1; F2 = FL2;
+        } else if (sR <= T(0)) {
+            F0 = FR0; F1 = FR1; F2 = FR2;
+        } else {
+            T inv = T(1) / (sR - sL);
+            T ss = sL * sR;
+            F0 = (sR * FL0 - sL * FR0 + ss * (hRs - hLs)) * inv;
+            F1 = (sR * FL1 - sL * FR1 + ss * (hRs * uR - hLs * uL)) * inv;
+            F2 = (sR * FL2 - sL * FR2 + ss * (hRs * vR - hLs * vL)) * inv;
+        }
+        T cA = T(0.5) * g * (qL[1] * qL[1] - hLs * hLs);
+        out[0] = T(0);
+        out[1] = F0;
+        out[2] = F1 + cA * nx;
+        out[3] = F2 + cA * ny;
+        return;
+    }
+    // Rusanov (centred flux + fluctuation with s_M dissipation)
+    T unL = (qL[2] * nx + qL[3] * ny) * hiL;
+    T unR = (qR[2] * nx + qR[3] * ny) * hiR;
+    T pL = T(0.5) * g * qL[1] * qL[1], pR = T(0.5) * g * qR[1] * qR[1];
+    T s = tmax(fabs(unL) + sqrt(g * tmax(qL[1], T(0))),
+               fabs(unR) + sqrt(g * tmax(qR[1], T(0))));
+    T d0 = qR[0] - qL[0], d1 = qR[1] - qL[1], d2 = qR[2] - qL[2], d3 = qR[3] - qL[3];
+    T Bd = g * (T(0.5) * (qL[1] + qR[1])) * d0;
+    T diss0, diss1;
+    if (rs == RS_RUSANOV) {
+        diss0 = T(0);
+        diss1 = s * (d1 + d0);
+    } else {
+        diss0 = s * d0;
+        diss1 = s * d1;
+    }
+    out[0] = T(0.5) * (T(0) + T(0)) + T(0.5) * (T(0) - diss0);
+    out[1] = T(0.5) * ((qL[2] * nx + qL[3] * ny) + (qR[2] * nx + qR[3] * ny)) + T(0.5) * (T(0) - diss1);
+    out[2] = T(0.5) * ((qL[2] * unL + pL * nx) + (qR[2] * unR + pR * nx)) + T(0.5) * (Bd * nx - s * d2);
+    out[3] = T(0.5) * ((qL[3] * unL + pL * ny) + (qR[3] * unR + pR * ny)) + T(0.5) * (Bd * ny - s * d3);
+}
+
+// Cell-edge nonconservative integral g (h_c + h_e)/2 (b_e - b_c) n (q rows).
+template <typename T>
+__device__ __forceinline__ void cell_edge(const T* qc, const T* qe, T nx, T ny, T g,
+                                          T* out) {
+    T k = g * (T(0.5) * (qc[1] + qe[1])) * (qe[0] - qc[0]);
+    out[2] = k * nx;
+    out[3] = k * ny;
+}
+
+// Block-wide reductions over BLOCK threads (wave64 shuffles + LDS).
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < BLOCK / 64; ++i) v = op(v, lds[i]);
+    }
+    return v;  // valid on thread 0
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of the chip; remap the
+// block index so that every XCD works on one contiguous range of cells (its
+// own L2 then holds the neighbours its cells gather).
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, k = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace zj

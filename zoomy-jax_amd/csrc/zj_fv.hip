@@ -1,0 +1,948 @@
+// MI355X finite-volume time step: kernels and the C ABI (include/zoomy_hip.h).
+//
+// One HIP thread owns one cell and gathers every face of that cell from the
+// per-cell slot tables (cell -> vertices, cell -> neighbour), so the update is
+// atomic-free and bit-reproducible: the face-scatter of the reference
+// (solver_jax.py:806-809, solver2/blocks.py:241-287) is turned into the
+// gather over a face-of-cell table that blocks.py:261-263 names.  Face
+// geometry (normal, length, centre) is recomputed from the vertex coordinates
+// instead of streamed from HBM.  An SSP-RK2 step is four launches:
+//
+//   slopes(Q0)      LSQ gradient + limiter bounds + limiter  -> S
+//   flux(Q0, S)     reconstruct, Riemann, cell-edge term      -> Q1
+//   slopes(Q1)                                                -> S
+//   flux(Q1, S, Q0) ... + (Q0 + Q2)/2 + source + troubled flag + dt of the
+//                   new state (block min -> one atomicMin)    -> Q
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/zoomy_hip.h"
+#include "zj_device.hpp"
+
+using namespace zj;
+
+namespace {
+
+struct DevMesh {
+    int dim, N, nverts, mf, mx, nbf;
+    const int* cv;        // (mf, N)
+    const int* nbr;       // (mf, N)
+    const int* extra;     // (mx, N)
+    const double2* verts; // (nverts)
+    const double* cen;    // (2, N)
+    const double* rin;    // (N)
+    const int* bf_kind;   // (nbf)
+    const int* bf_pcell;  // (nbf) periodic partner cell or -1
+};
+
+struct Opts {
+    double g, cfl, eps, dt_max, manning;
+    int order, lim, rs, recon;
+};
+
+struct Geo {
+    double cx, cy, vol;
+    double fx[MAXF], fy[MAXF], nx[MAXF], ny[MAXF], len[MAXF];
+};
+
+__device__ __forceinline__ void load_geo(const DevMesh& m, int c, Geo& G) {
+    const int N = m.N;
+    G.cx = m.cen[c];
+    G.cy = m.cen[N + c];
+    if (m.dim == 1) {
+        double2 a = m.verts[m.cv[c]];
+        double2 b = m.verts[m.cv[N + c]];
+        G.vol = b.x - a.x;
+        G.fx[0] = a.x; G.fy[0] = 0.0; G.nx[0] = -1.0; G.ny[0] = 0.0; G.len[0] = 1.0;
+        G.fx[1] = b.x; G.fy[1] = 0.0; G.nx[1] = 1.0; G.ny[1] = 0.0; G.len[1] = 1.0;
+        return;
+    }
+    double2 v[MAXF];
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k)
+        if (k < m.mf) v[k] = m.verts[m.cv[k * N + c]];
+    double area = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        if (k >= m.mf) break;
+        double2 a = v[k];
+        double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+        double ex = b.x - a.x, ey = b.y - a.y;
+        double L = sqrt(ex * ex + ey * ey);
+        G.len[k] = L;
+        G.nx[k] = ey / L;
+        G.ny[k] = -ex / L;
+        G.fx[k] = 0.5 * (a.x + b.x);
+        G.fy[k] = 0.5 * (a.y + b.y);
+        area += a.x * b.y - b.x * a.y;
+    }
+    G.vol = 0.5 * area;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_q(const T* __restrict__ Q, int N, int c, T* q) {
+    q[0] = Q[c];
+    q[1] = Q[N + c];
+    q[2] = Q[2 * N + c];
+    q[3] = Q[3 * N + c];
+}
+
+// Reconstruction variables: conservative (b, h, hu, hv) or (b, eta, u, v).
+template <typename T>
+__device__ __forceinline__ void to_w(int recon, const T* q, T hi, T* w) {
+    if (recon == RECON_ETA) {
+        w[0] = q[0];
+        w[1] = q[1] + q[0];
+        w[2] = q[2] * hi;
+        w[3] = q[3] * hi;
+    } else {
+        w[0] = q[0]; w[1] = q[1]; w[2] = q[2]; w[3] = q[3];
+    }
+}
+
+// Face value from a cell (state q, aux hi, limited slopes s[8]) at offset r.
+template <typename T>
+__device__ __forceinline__ void recon_face(int recon, const T* q, T hi, const T* s,
+                                           T rx, T ry, T* out) {
+    if (recon == RECON_ETA) {
+        T b = q[0] + s[0] * rx + s[1] * ry;
+        T eta = (q[1] + q[0]) + s[2] * rx + s[3] * ry;
+        T u = q[2] * hi + s[4] * rx + s[5] * ry;
+        T v = q[3] * hi + s[6] * rx + s[7] * ry;
+        eta = tmax(eta, b);  // hydrostatic face clip, reconstruction_jax.py:1269-1272
+        T h = eta - b;
+        out[0] = b; out[1] = h; out[2] = u * h; out[3] = v * h;
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] = q[v] + s[2 * v] * rx + s[2 * v + 1] * ry;
+    }
+}
+
+// ─────────────────────────────────────────────────────────────────────────
+// slopes: LSQ gradient with per-direction boundary ghosts, limiter bounds
+// over face neighbours and ghosts, limiter; writes S (N, 8) = phi * grad.
+// aux mode (Qaux != nullptr): conservative gradients with zero ghost deltas
+// -> derivative aux rows + hinv (post_step).
+// ─────────────────────────────────────────────────────────────────────────
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
+                                                  const T* __restrict__ Q,
+                                                  T* __restrict__ S,
+                                                  T* __restrict__ Qaux) {
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    if (c >= m.N) return;
+    const int N = m.N;
+    const bool aux = Qaux != nullptr;
+    const int recon = aux ? RECON_CONS : o.recon;
+    const T eps = T(o.eps);
+    T q[4];
+    load_q(Q, N, c, q);
+    const T hic = hinv_of(q[1], eps);
+    T w[4];
+    to_w(recon, q, hic, w);
+    Geo G;
+    load_geo(m, c, G);
+
+    double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
+    double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
+    T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
+    T gbx[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    T gby[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    T umax[4], umin[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
+
+    for (int k = 0; k < m.mf; ++k) {
+        const int nb = m.nbr[k * N + c];
+        if (nb >= 0) {
+            T qn[4], wn[4];
+            load_q(Q, N, nb, qn);
+            to_w(recon, qn, hinv_of(qn[1], eps), wn);
+            const double rx = m.cen[nb] - G.cx, ry = m.cen[N + nb] - G.cy;
+            Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T dv = wn[v] - w[v];
+                bx[v] += T(rx) * dv;
+                by[v] += T(ry) * dv;
+                umax[v] = tmax(umax[v], wn[v]);
+                umin[v] = tmin(umin[v], wn[v]);
+            }
+        } else {
+            const int bf = -1 - nb;
+            const int kind = m.bf_kind[bf];
+            T qg[4], wg[4];
+            if (kind == BC_PERIODIC) load_q(Q, N, m.bf_pcell[bf], qg);
+            else bc_ghost(kind, q, T(G.nx[k]), T(G.ny[k]), qg);
+            to_w(recon, qg, hinv_of(qg[1], eps), wg);
+            const double rx = 2.0 * (G.fx[k] - G.cx), ry = 2.0 * (G.fy[k] - G.cy);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T bv = (kind == BC_PERIODIC) ? wg[v] : T(2) * wg[v] - w[v];
+                umax[v] = tmax(umax[v], bv);
+                umin[v] = tmin(umin[v], bv);
+            }
+            const double an[2] = {fabs(G.nx[k]), fabs(G.ny[k])};
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                if (an[d] > 0.5) {
+                    Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry;
+                    if (!aux) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            T dv = wg[v] - w[v];
+                            gbx[d][v] += T(rx) * dv;
+                            gby[d][v] += T(ry) * dv;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int j = 0; j < m.mx; ++j) {
+        const int e = m.extra[j * N + c];
+        if (e < 0) break;
+        T qn[4], wn[4];
+        load_q(Q, N, e, qn);
+        to_w(recon, qn, hinv_of(qn[1], eps), wn);
+        const double rx = m.cen[e] - G.cx, ry = m.cen[N + e] - G.cy;
+        Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            T dv = wn[v] - w[v];
+            bx[v] += T(rx) * dv;
+            by[v] += T(ry) * dv;
+        }
+    }
+
+    T gx[4], gy[4];
+    if (m.dim == 1) {
+        const double inv = 1.0 / (Sxx + Gxx[0]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { gx[v] = T(inv) * (bx[v] + gbx[0][v]); gy[v] = T(0); }
+    } else {
+        // component x: normal matrix with the x-ghost rows only (and y alike)
+        const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
+        const double dinv0 = 1.0 / (axx * ayy - axy * axy);
+        const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
+        const double dinv1 = 1.0 / (bxx * byy - bxy * bxy);
+        const T c0 = T(ayy * dinv0), c1 = T(-axy * dinv0);
+        const T c2 = T(-bxy * dinv1), c3 = T(bxx * dinv1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            gx[v] = c0 * (bx[v] + gbx[0][v]) + c1 * (by[v] + gby[0][v]);
+            gy[v] = c2 * (bx[v] + gbx[1][v]) + c3 * (by[v] + gby[1][v]);
+        }
+    }
+
+    if (aux) {
+        // rows: per direction d [d q_d/dx_d, d h/dx_d], then d b/dx_d, hinv
+        int row = 0;
+        for (int d = 0; d < m.dim; ++d) {
+            Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[2] : gy[3];
+            Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[1] : gy[1];
+        }
+        for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[0] : gy[0];
+        Qaux[(size_t)row * N + c] = hic;
+        return;
+    }
+
+    T phi[4] = {1, 1, 1, 1};
+    if (o.lim != LIM_NONE) {
+        const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
+        for (int k = 0; k < m.mf; ++k) {
+            const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T d = gx[v] * rx + gy[v] * ry;
+                T cand = lim_cand(o.lim, d, umax[v] - w[v], umin[v] - w[v], ev2);
+                phi[v] = tmin(phi[v], cand);
+            }
+        }
+    }
+    if (recon == RECON_ETA) {
+        phi[0] = tmin(phi[0], phi[1]);
+        phi[2] = tmin(phi[2], phi[1]);
+        phi[3] = tmin(phi[3], phi[1]);
+    }
+    T* sp = S + (size_t)c * 8;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        T p = clip01(phi[v]);
+        sp[2 * v] = p * gx[v];
+        sp[2 * v + 1] = p * gy[v];
+    }
+}
+
+// ─────────────────────────────────────────────────────────────────────────
+// flux + update.  mode 0: Qout = Qin + dt L;  mode 1: Qout = (Q0 + Qin +
+// dt L)/2;  mode 2: Qout = L (flux operator).  final: explicit source with
+// the step-start aux, troubled count, dt reduction of the new state.
+// ─────────────────────────────────────────────────────────────────────────
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_flux(
+    DevMesh m, Opts o, const T* __restrict__ Qin, const T* __restrict__ S,
+    const T* Q0, T* Qout, const double* __restrict__ dtp,
+    int mode, int final_stage, int write_b, int* __restrict__ troubled,
+    unsigned long long* __restrict__ dt_red) {
+    __shared__ double lds_d[BLOCK / 64];
+    __shared__ int lds_i[BLOCK / 64];
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    const bool live = c < m.N;
+    const int N = m.N;
+    const T g = T(o.g), eps = T(o.eps);
+    int bad = 0;
+    double loc_dt = INFINITY;
+    if (live) {
+        T q[4];
+        load_q(Qin, N, c, q);
+        const T hic = hinv_of(q[1], eps);
+        T sc[8];
+        if (S) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sc[i] = S[(size_t)c * 8 + i];
+        }
+        Geo G;
+        load_geo(m, c, G);
+        T acc[4] = {0, 0, 0, 0};
+        for (int k = 0; k < m.mf; ++k) {
+            const T nx = T(G.nx[k]), ny = T(G.ny[k]);
+            T qL[4], qR[4];
+            if (S) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
+            else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
+            const int nb = m.nbr[k * N + c];
+            T hiR = hic;
+            if (nb >= 0) {
+                T qn[4];
+                load_q(Qin, N, nb, qn);
+                hiR = hinv_of(qn[1], eps);
+                if (S) {
+                    T sn[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) sn[i] = S[(size_t)nb * 8 + i];
+                    recon_face(o.recon, qn, hiR, sn, T(G.fx[k] - m.cen[nb]),
+                               T(G.fy[k] - m.cen[N + nb]), qR);
+                } else {
+                    qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
+                }
+            } else {
+                const int bf = -1 - nb;
+                const int kind = m.bf_kind[bf];
+                if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qR);
+                else bc_ghost(kind, qL, nx, ny, qR);
+            }
+            T f[4];
+            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, f);
+            const T L = T(G.len[k]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] += f[v] * L;
+            if (S) {
+                T ce[4];
+                cell_edge(q, qL, nx, ny, g, ce);
+                acc[2] += ce[2] * L;
+                acc[3] += ce[3] * L;
+            }
+        }
+        const T ivol = T(1.0 / G.vol);
+        T out[4];
+        if (mode == 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[v] = -acc[v] * ivol;
+        } else {
+            const T dt = T(*dtp);
+            if (mode == 1) {
+                T q0[4];
+                load_q(Q0, N, c, q0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) out[v] = T(0.5) * (q0[v] + (q[v] - dt * (acc[v] * ivol)));
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) out[v] = q[v] - dt * (acc[v] * ivol);
+            }
+            if (final_stage && o.manning > 0.0) {
+                // explicit source with the aux of the step-start state (ode.RK1
+                // after the hyperbolic step, solver_jax.py:1129)
+                const T h0 = mode == 1 ? Q0[N + c] : q[1];
+                const T hs = hinv_of(h0, eps);
+                const T u = out[2] * hs, vv = out[3] * hs;
+                const T sp = sqrt(u * u + vv * vv);
+                const T kk = T(o.g * o.manning * o.manning) * cbrt(hs);
+                out[2] += dt * (-kk * u * sp);
+                out[3] += dt * (-kk * vv * sp);
+            }
+        }
+        if (write_b) Qout[c] = out[0];
+        Qout[N + c] = out[1];
+        Qout[2 * N + c] = out[2];
+        Qout[3 * N + c] = out[3];
+        if (final_stage && mode != 2) {
+            bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
+                  !isfinite(out[2]) || !isfinite(out[3]);
+            if (dt_red) {
+                const T hn = hinv_of(out[1], eps);
+                const double rc = m.rin[c];
+                for (int k = 0; k < m.mf; ++k) {
+                    const int nb = m.nbr[k * N + c];
+                    const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
+                    const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
+                    double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
+                    if (l != l) l = 0.0;  // NaN speed: force the fatal dt guard
+                    loc_dt = fmin(loc_dt, l);
+                }
+            }
+        }
+    }
+    if (final_stage && mode != 2) {
+        if (troubled) {
+            int s = block_reduce(bad, [](int a, int b) { return a + b; }, lds_i);
+            if (threadIdx.x == 0 && s) atomicAdd(troubled, s);
+        }
+        if (dt_red) {
+            double s = block_reduce(loc_dt, [](double a, double b) { return fmin(a, b); }, lds_d);
+            if (threadIdx.x == 0)
+                atomicMin(dt_red, (unsigned long long)__double_as_longlong(s));
+        }
+    }
+}
+
+// dt of a state (compute_timestep): per-cell min over faces, block min,
+// atomicMin on the bit pattern (non-negative doubles order as integers).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __restrict__ Q,
+                                              unsigned long long* __restrict__ dt_red) {
+    __shared__ double lds[BLOCK / 64];
+    const int c = xcd_block() * BLOCK + threadIdx.x;
+    double loc = INFINITY;
+    if (c < m.N) {
+        const int N = m.N;
+        T q[4];
+        load_q(Q, N, c, q);
+        const T hn = hinv_of(q[1], T(o.eps));
+        Geo G;
+        load_geo(m, c, G);
+        const double rc = m.rin[c];
+        for (int k = 0; k < m.mf; ++k) {
+            const int nb = m.nbr[k * N + c];
+            const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
+            const double a = (double)max_abs_eig(q, hn, T(G.nx[k]), T(G.ny[k]), T(o.g));
+            double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
+            if (l != l) l = 0.0;
+            loc = fmin(loc, l);
+        }
+    }
+    double s = block_reduce(loc, [](double a, double b) { return fmin(a, b); }, lds);
+    if (threadIdx.x == 0) atomicMin(dt_red, (unsigned long long)__double_as_longlong(s));
+}
+
+__global__ void k_dt_init(unsigned long long* red, int* troubled) {
+    *red = (unsigned long long)__double_as_longlong(INFINITY);
+    if (troubled) *troubled = 0;
+}
+
+__global__ void k_dt_finalize(unsigned long long* red, double* dt, const double* t,
+                              double t_end, double dt_max) {
+    double d = __longlong_as_double((long long)*red);
+    if (dt_max > 0.0) d = fmin(d, dt_max);
+    if (t_end >= 0.0 && t) d = fmin(d, t_end - *t);
+    *dt = d;
+    *red = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+__global__ void k_t_advance(double* t, const double* dt) { *t += *dt; }
+
+// Compensated (Neumaier) mass: per-thread partials, per-block (sum, comp)
+// pairs, then a single ordered pass -- deterministic for a fixed grid.
+__device__ __forceinline__ void neumaier(double& s, double& comp, double x) {
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+    else comp += (x - t) + s;
+    s = t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_mass_partial(DevMesh m, const T* __restrict__ Q,
+                                                        double2* __restrict__ part) {
+    __shared__ double2 lds[BLOCK];
+    double s = 0.0, comp = 0.0;
+    for (int c = blockIdx.x * BLOCK + threadIdx.x; c < m.N; c += gridDim.x * BLOCK) {
+        Geo G;
+        load_geo(m, c, G);
+        neumaier(s, comp, (double)Q[m.N + c] * G.vol);
+    }
+    lds[threadIdx.x] = make_double2(s, comp);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = 0.0, C = 0.0;
+        for (int i = 0; i < BLOCK; ++i) {
+            neumaier(S, C, lds[i].x);
+            C += lds[i].y;
+        }
+        part[blockIdx.x] = make_double2(S, C);
+    }
+}
+
+__global__ void k_mass_final(const double2* part, int n, double* out) {
+    double S = 0.0, C = 0.0;
+    for (int i = 0; i < n; ++i) {
+        neumaier(S, C, part[i].x);
+        C += part[i].y;
+    }
+    *out = S + C;
+}
+
+// Local implicit Manning stage (solver_imex_jax.py:268-342).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_imex(DevMesh m, Opts o, const double* __restrict__ dtp,
+                                                int maxiter, const T* __restrict__ Qe,
+                                                T* __restrict__ Qo) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= m.N) return;
+    const int N = m.N;
+    T q[4];
+    load_q(Qe, N, c, q);
+    Qo[c] = q[0];
+    Qo[N + c] = q[1];
+    if (o.manning <= 0.0) {
+        Qo[2 * N + c] = q[2];
+        Qo[3 * N + c] = q[3];
+        return;
+    }
+    const T dt = T(*dtp);
+    const T hinv = hinv_of(q[1], T(o.eps));
+    const T k = T(o.g * o.manning * o.manning) * cbrt(hinv);
+    T qx = q[2], qy = q[3];
+    const T ex = qx, ey = qy;
+    for (int it = 0; it < maxiter; ++it) {
+        T u = qx * hinv, v = qy * hinv;
+        T sp = sqrt(u * u + v * v);
+        T Sx = -k * u * sp, Sy = -k * v * sp;
+        T jxx = 0, jxy = 0, jyy = 0;
+        if (sp > T(0)) {
+            jxx = -k * hinv * (sp + u * u / sp);
+            jxy = -k * hinv * (u * v / sp);
+            jyy = -k * hinv * (sp + v * v / sp);
+        }
+        T a11 = T(1) - dt * jxx, a12 = -dt * jxy, a21 = -dt * jxy, a22 = T(1) - dt * jyy;
+        T rx = qx - ex - dt * Sx, ry = qy - ey - dt * Sy;
+        T det = a11 * a22 - a12 * a21;
+        qx += (-rx * a22 + ry * a12) / det;
+        qy += (-ry * a11 + rx * a21) / det;
+    }
+    Qo[2 * N + c] = qx;
+    Qo[3 * N + c] = qy;
+}
+
+}  // namespace
+
+// ═════════════════════════════════════════════════════════════════════════
+// C ABI
+// ═════════════════════════════════════════════════════════════════════════
+struct zj_mesh {
+    DevMesh d;
+    std::vector<void*> allocs;
+};
+
+struct zj_solver {
+    zj_mesh* m;
+    zj_params p;
+    Opts o;
+    void* S;                    // (N, 8) limited slopes
+    void* Q1;                   // (4, N) stage state
+    unsigned long long* dt_red; // reduction scalar
+    int* troubled;              // scratch counter
+    double2* mass_part;         // per-block mass partials
+    double* dt_tmp;             // device dt
+    int mass_blocks;
+};
+
+#define HIPCHK(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) {                                       \
+            fprintf(stderr, "zoomy_hip: %s failed: %s\n", #x,         \
+                    hipGetErrorString(e_));                           \
+            return ZJ_EHIP;                                           \
+        }                                                             \
+    } while (0)
+
+static int dev_upload(zj_mesh* m, const void* src, size_t bytes, void** dst) {
+    if (bytes == 0) bytes = 8;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return ZJ_ENOMEM;
+    m->allocs.push_back(p);
+    if (src) HIPCHK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(p, 0, bytes));
+    *dst = p;
+    return ZJ_OK;
+}
+
+static inline int nblocks(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+extern "C" {
+
+const char* zj_version(void) { return "zoomy_hip 0.1 gfx950"; }
+
+int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
+    if (!d || !out) return ZJ_EINVAL;
+    if (d->dim < 1 || d->dim > 2 || d->n_cells <= 0 || d->max_faces < 2 ||
+        d->max_faces > MAXF || d->max_extra < 0 || d->n_bf < 0)
+        return ZJ_EINVAL;
+    if (d->dim == 1 && d->max_faces != 2) return ZJ_EINVAL;
+    const int N = d->n_cells, mf = d->max_faces, mx = d->max_extra;
+    // host-side index validation: every gather in the kernels stays in bounds
+    for (size_t i = 0; i < (size_t)mf * N; ++i) {
+        int v = d->cell_vertices[i];
+        if (v < 0 || v >= d->n_verts) return ZJ_EINVAL;
+        int nb = d->cell_nbr[i];
+        if (nb >= N || (nb < 0 && -1 - nb >= d->n_bf)) return ZJ_EINVAL;
+    }
+    for (size_t i = 0; i < (size_t)mx * N; ++i)
+        if (d->cell_extra[i] >= N) return ZJ_EINVAL;
+    std::vector<int> pcell(d->n_bf > 0 ? d->n_bf : 1, -1);
+    for (int b = 0; b < d->n_bf; ++b) {
+        int k = d->bface_kind[b];
+        if (k < 0 || k > 2) return ZJ_EINVAL;
+        if (k == ZJ_BC_PERIODIC) {
+            int pb = d->bface_partner ? d->bface_partner[b] : -1;
+            if (pb < 0 || pb >= d->n_bf) return ZJ_EINVAL;
+            pcell[b] = d->bface_cell[pb];
+            if (pcell[b] < 0 || pcell[b] >= N) return ZJ_EINVAL;
+        }
+    }
+    zj_mesh* m = new (std::nothrow) zj_mesh();
+    if (!m) return ZJ_ENOMEM;
+    m->d.dim = d->dim; m->d.N = N; m->d.nverts = d->n_verts;
+    m->d.mf = mf; m->d.mx = mx; m->d.nbf = d->n_bf;
+    void* p;
+    int rc = 0;
+    rc |= dev_upload(m, d->cell_vertices, sizeof(int) * mf * N, &p); m->d.cv = (const int*)p;
+    rc |= dev_upload(m, d->cell_nbr, sizeof(int) * mf * N, &p); m->d.nbr = (const int*)p;
+    rc |= dev_upload(m, mx ? d->cell_extra : nullptr, sizeof(int) * mx * N, &p); m->d.extra = (const int*)p;
+    rc |= dev_upload(m, d->verts, sizeof(double) * 2 * d->n_verts, &p); m->d.verts = (const double2*)p;
+    rc |= dev_upload(m, d->cell_center, sizeof(double) * 2 * N, &p); m->d.cen = (const double*)p;
+    rc |= dev_upload(m, d->cell_inradius, sizeof(double) * N, &p); m->d.rin = (const double*)p;
+    rc |= dev_upload(m, d->n_bf ? d->bface_kind : nullptr, sizeof(int) * d->n_bf, &p); m->d.bf_kind = (const int*)p;
+    rc |= dev_upload(m, pcell.data(), sizeof(int) * pcell.size(), &p); m->d.bf_pcell = (const int*)p;
+    if (rc) {
+        zj_mesh_destroy(m);
+        return ZJ_ENOMEM;
+    }
+    *out = m;
+    return ZJ_OK;
+}
+
+int zj_mesh_destroy(zj_mesh* m) {
+    if (!m) return ZJ_OK;
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+    return ZJ_OK;
+}
+
+int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
+    if (!m || !p || !out) return ZJ_EINVAL;
+    if (p->dtype != ZJ_F64 && p->dtype != ZJ_F32) return ZJ_EINVAL;
+    if (p->limiter < 0 || p->limiter > 4 || p->riemann < 0 || p->riemann > 2 ||
+        p->recon < 0 || p->recon > 1 || p->order < 1 || p->order > 2)
+        return ZJ_EINVAL;
+    zj_solver* s = new (std::nothrow) zj_solver();
+    if (!s) return ZJ_ENOMEM;
+    s->m = m;
+    s->p = *p;
+    s->o = Opts{p->g, p->cfl, p->eps_wet, p->dt_max, p->manning_n,
+                p->order, p->limiter, p->riemann, p->recon};
+    const size_t esz = p->dtype == ZJ_F64 ? 8 : 4;
+    const int N = m->d.N;
+    s->mass_blocks = std::min(1024, nblocks(N));
+    if (hipMalloc(&s->S, esz * 8 * N) != hipSuccess ||
+        hipMalloc(&s->Q1, esz * 4 * N) != hipSuccess ||
+        hipMalloc((void**)&s->dt_red, 8) != hipSuccess ||
+        hipMalloc((void**)&s->troubled, 8) != hipSuccess ||
+        hipMalloc((void**)&s->dt_tmp, 8) != hipSuccess ||
+        hipMalloc((void**)&s->mass_part, sizeof(double2) * s->mass_blocks) != hipSuccess) {
+        zj_solver_destroy(s);
+        return ZJ_ENOMEM;
+    }
+    *out = s;
+    return ZJ_OK;
+}
+
+int zj_solver_destroy(zj_solver* s) {
+    if (!s) return ZJ_OK;
+    (void)hipFree(s->S);
+    (void)hipFree(s->Q1);
+    (void)hipFree(s->dt_red);
+    (void)hipFree(s->troubled);
+    (void)hipFree(s->dt_tmp);
+    (void)hipFree(s->mass_part);
+    delete s;
+    return ZJ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled,
+              unsigned long long* dt_red, hipStream_t st) {
+    const DevMesh& m = s->m->d;
+    const int nb = nblocks(m.N);
+    T* S = (T*)s->S;
+    T* Q1 = (T*)s->Q1;
+    const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
+    if (s->o.order < 2) {
+        if (Qout == Q) {
+            if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)nullptr,
+                               (const T*)nullptr, Q1, dt, 0, 1, write_b, troubled, dt_red);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(Qout, Q1, sizeof(T) * 4 * m.N, hipMemcpyDeviceToDevice, st));
+            return ZJ_OK;
+        }
+        if (!write_b && Qout != Q)
+            HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)nullptr,
+                           (const T*)nullptr, Qout, dt, 0, 1, write_b, troubled, dt_red);
+        HIPCHK(hipGetLastError());
+        return ZJ_OK;
+    }
+    if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
+    if (!write_b && Qout != Q)
+        HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, (T*)nullptr);
+    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)S,
+                       (const T*)nullptr, Q1, dt, 0, 0, write_b, (int*)nullptr,
+                       (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const T*)Q1, S, (T*)nullptr);
+    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const T*)Q1, (const T*)S,
+                       Q, Qout, dt, 1, 1, write_b, troubled, dt_red);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+template <typename T>
+int flux_impl(zj_solver* s, const T* Q, T* dQ, hipStream_t st) {
+    const DevMesh& m = s->m->d;
+    const int nb = nblocks(m.N);
+    T* S = (T*)s->S;
+    if (s->o.order >= 2)
+        hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, (T*)nullptr);
+    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q,
+                       s->o.order >= 2 ? (const T*)S : (const T*)nullptr, (const T*)nullptr, dQ,
+                       (const double*)nullptr, 2, 0, 1, (int*)nullptr,
+                       (unsigned long long*)nullptr);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int zj_flux_operator(zj_solver* s, const void* Q, void* dQ, void* stream) {
+    if (!s || !Q || !dQ || Q == dQ) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    return s->p.dtype == ZJ_F64 ? flux_impl(s, (const double*)Q, (double*)dQ, st)
+                                : flux_impl(s, (const float*)Q, (float*)dQ, st);
+}
+
+int zj_compute_dt(zj_solver* s, const void* Q, double t_remaining, double* dt_out, void* stream) {
+    if (!s || !Q || !dt_out) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, (int*)nullptr);
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_dt<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const double*)Q, s->dt_red);
+    else
+        hipLaunchKernelGGL(k_dt<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const float*)Q, s->dt_red);
+    // t_remaining is applied through a host scalar: finalize with t = 0 and
+    // t_end = t_remaining.
+    if (t_remaining >= 0.0) {
+        HIPCHK(hipMemsetAsync(s->dt_tmp, 0, 8, st));
+        hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, s->dt_red, dt_out,
+                           (const double*)s->dt_tmp, t_remaining, s->p.dt_max);
+    } else {
+        hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, s->dt_red, dt_out,
+                           (const double*)nullptr, -1.0, s->p.dt_max);
+    }
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_step(zj_solver* s, const double* dt, const void* Q, void* Qout, int32_t* troubled_out,
+            void* stream) {
+    if (!s || !dt || !Q || !Qout) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (troubled_out)
+        hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, troubled_out);
+    return s->p.dtype == ZJ_F64
+               ? step_impl(s, dt, (const double*)Q, (double*)Qout, troubled_out,
+                           (unsigned long long*)nullptr, st)
+               : step_impl(s, dt, (const float*)Q, (float*)Qout, troubled_out,
+                           (unsigned long long*)nullptr, st);
+}
+
+int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {
+    if (!s || !Q || !Qaux) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_slopes<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const double*)Q, (double*)nullptr, (double*)Qaux);
+    else
+        hipLaunchKernelGGL(k_slopes<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const float*)Q, (float*)nullptr, (float*)Qaux);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, double* t_dev,
+             double* dt_dev, int32_t* troubled_out, void* stream) {
+    if (!s || !Q || !t_dev || !dt_dev || n_steps < 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, troubled_out);
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_dt<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const double*)Q, s->dt_red);
+    else
+        hipLaunchKernelGGL(k_dt<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const float*)Q, s->dt_red);
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < n_steps; ++k) {
+        hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev,
+                           (const double*)t_dev, t_end, s->p.dt_max);
+        int rc = s->p.dtype == ZJ_F64
+                     ? step_impl(s, dt_dev, (const double*)Q, (double*)Q, troubled_out, s->dt_red, st)
+                     : step_impl(s, dt_dev, (const float*)Q, (float*)Q, troubled_out, s->dt_red, st);
+        if (rc) return rc;
+        if (Qaux) {  // post_step aux refresh of the new state
+            rc = zj_update_aux(s, Q, Qaux, stream);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_t_advance, dim3(1), dim3(1), 0, st, t_dev, (const double*)dt_dev);
+    }
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+// Average duration (ms) of each kernel of a step, timed with HIP events on
+// the given stream over `reps` launches each: ms_out[0] slopes, [1] flux
+// stage 1, [2] flux stage 2 (final: update + source + troubled + dt), [3] dt,
+// [4] aux refresh.  Q is read only; the outputs go to solver scratch.
+int zj_time_kernels(zj_solver* s, const void* Q, const double* dt, int32_t reps,
+                    float* ms_out, void* stream) {
+    if (!s || !Q || !dt || !ms_out || reps <= 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    const int nb = nblocks(m.N);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+    void* out = nullptr;
+    void* aux = nullptr;
+    if (hipMalloc(&out, esz * 4 * m.N) != hipSuccess) return ZJ_ENOMEM;
+    if (hipMalloc(&aux, esz * (3 * m.dim + 1) * m.N) != hipSuccess) {
+        (void)hipFree(out);
+        return ZJ_ENOMEM;
+    }
+    const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
+    for (int which = 0; which < 5; ++which) {
+        for (int pass = 0; pass < 2; ++pass) {  // pass 0 warms up
+            const int n = pass ? reps : 2;
+            if (pass) HIPCHK(hipEventRecord(e0, st));
+            for (int r = 0; r < n; ++r) {
+                if (s->p.dtype == ZJ_F64) {
+                    const double* q = (const double*)Q;
+                    double* S = (double*)s->S;
+                    if (which == 0)
+                        hipLaunchKernelGGL(k_slopes<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, S, (double*)nullptr);
+                    else if (which == 1)
+                        hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const double*)S,
+                                           (const double*)nullptr, (double*)out, dt, 0, 0, write_b, (int*)nullptr,
+                                           (unsigned long long*)nullptr);
+                    else if (which == 2)
+                        hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const double*)S,
+                                           q, (double*)out, dt, 1, 1, write_b, s->troubled, s->dt_red);
+                    else if (which == 3)
+                        hipLaunchKernelGGL(k_dt<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
+                    else
+                        hipLaunchKernelGGL(k_slopes<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (double*)nullptr,
+                                           (double*)aux);
+                } else {
+                    const float* q = (const float*)Q;
+                    float* S = (float*)s->S;
+                    if (which == 0)
+                        hipLaunchKernelGGL(k_slopes<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, S, (float*)nullptr);
+                    else if (which == 1)
+                        hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const float*)S,
+                                           (const float*)nullptr, (float*)out, dt, 0, 0, write_b, (int*)nullptr,
+                                           (unsigned long long*)nullptr);
+                    else if (which == 2)
+                        hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const float*)S,
+                                           q, (float*)out, dt, 1, 1, write_b, s->troubled, s->dt_red);
+                    else if (which == 3)
+                        hipLaunchKernelGGL(k_dt<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
+                    else
+                        hipLaunchKernelGGL(k_slopes<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (float*)nullptr,
+                                           (float*)aux);
+                }
+            }
+            if (pass) {
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                ms_out[which] = ms / (float)reps;
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    (void)hipFree(out);
+    (void)hipFree(aux);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ZJ_OK;
+}
+
+int zj_mass(zj_solver* s, const void* Q, double* mass_out, void* stream) {
+    if (!s || !Q || !mass_out) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_mass_partial<double>, dim3(s->mass_blocks), dim3(BLOCK), 0, st, m,
+                           (const double*)Q, s->mass_part);
+    else
+        hipLaunchKernelGGL(k_mass_partial<float>, dim3(s->mass_blocks), dim3(BLOCK), 0, st, m,
+                           (const float*)Q, s->mass_part);
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1), 0, st, (const double2*)s->mass_part,
+                       s->mass_blocks, mass_out);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_imex_local(zj_solver* s, const double* dt, int32_t maxiter, const void* Qexp, void* Qout,
+                  void* stream) {
+    if (!s || !dt || !Qexp || !Qout || maxiter < 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = s->m->d;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_imex<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o, dt,
+                           maxiter, (const double*)Qexp, (double*)Qout);
+    else
+        hipLaunchKernelGGL(k_imex<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o, dt,
+                           maxiter, (const float*)Qexp, (float*)Qout);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+"""ctypes binding of ``libzoomy_hip.so`` (the C ABI in ``include/zoomy_hip.h``).
+
+The library is built in-tree by ``csrc/build.sh``; there is no fallback: if
+it is missing the import fails with an error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libzoomy_hip.so"
+
+ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP = 0, 1, 2, 3, 4
+LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
+            "venkatakrishnan": 3, "vanleer": 4}
+RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
+RECON = {"conservative": 0, "eta": 1}
+DTYPES = {"float64": 0, "float32": 1}
+
+
+class ZoomyHipError(RuntimeError):
+    pass
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("n_cells", ctypes.c_int32),
+                ("n_verts", ctypes.c_int32), ("max_faces", ctypes.c_int32),
+                ("max_extra", ctypes.c_int32), ("n_bf", ctypes.c_int32),
+                ("cell_vertices", ctypes.c_void_p),
+                ("cell_nbr", ctypes.c_void_p),
+                ("cell_extra", ctypes.c_void_p),
+                ("verts", ctypes.c_void_p),
+                ("cell_center", ctypes.c_void_p),
+                ("cell_inradius", ctypes.c_void_p),
+                ("bface_cell", ctypes.c_void_p),
+                ("bface_slot", ctypes.c_void_p),
+                ("bface_kind", ctypes.c_void_p),
+                ("bface_partner", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("eps_wet", ctypes.c_double), ("dt_max", ctypes.c_double),
+                ("manning_n", ctypes.c_double), ("order", ctypes.c_int32),
+                ("limiter", ctypes.c_int32), ("riemann", ctypes.c_int32),
+                ("recon", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+
+
+_lib = None
+
+_SIGS = {
+    "zj_version": (ctypes.c_char_p, []),
+    "zj_mesh_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc),
+                                      ctypes.POINTER(ctypes.c_void_p)]),
+    "zj_mesh_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zj_rcm": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_solver_create": (ctypes.c_int, [ctypes.c_void_p,
+                                        ctypes.POINTER(Params),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "zj_solver_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zj_flux_operator": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "zj_compute_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "zj_step": (ctypes.c_int, [ctypes.c_void_p] * 6),
+    "zj_update_aux": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "zj_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p,
+                                ctypes.c_int32, ctypes.c_double,
+                                ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_mass": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "zj_time_kernels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int32,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_imex_local": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+# the Poisson / Chorin / learned-preconditioner entry points
+_SIGS_POISSON = {}
+
+
+def exported_symbols():
+    return sorted(list(_SIGS) + list(_SIGS_POISSON))
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ZoomyHipError(
+                f"{LIB_PATH} not found: build it with "
+                f"`bash zoomy-jax_amd/csrc/build.sh` (or __graft_entry__.build())")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in {**_SIGS, **_SIGS_POISSON}.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def register(name, restype, argtypes):
+    _SIGS_POISSON[name] = (restype, argtypes)
+
+
+def check(rc, what=""):
+    if rc != ZJ_OK:
+        names = {ZJ_EINVAL: "invalid argument", ZJ_EHIP: "HIP runtime error",
+                 ZJ_ENOMEM: "out of device memory",
+                 ZJ_EUNSUP: "unsupported option"}
+        raise ZoomyHipError(f"{what}: {names.get(rc, rc)} (code {rc})")

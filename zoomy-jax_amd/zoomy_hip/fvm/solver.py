@@ -1,0 +1,392 @@
+"""Drop-in host API of the finite-volume time step on MI355X.
+
+Mirrors the reference's operator/solver surface (``zoomy_jax/fvm/
+solver_jax.py``) with the arithmetic done by the HIP kernels behind the C ABI:
+
+* ``HyperbolicSolver.setup_simulation(mesh, model) -> (Q, Qaux)``  (:901-988)
+* ``get_flux_operator(mesh, runtime) -> flux_operator(dt, time, Q, Qaux,
+  parameters, dQ, force_o1=None)``                                  (:563-897)
+* ``compute_timestep(Q, Qaux, p=None) -> dt``                        (:1222-1236)
+* ``step(dt, time, Q, Qaux, p=None) -> Q``                           (:1095-1131)
+* ``post_step(time, dt, Q, Qold, Qaux, p=None) -> (Q, Qaux)``        (:1133-1172)
+* ``run_simulation`` / ``solve``                                     (:1238-1368)
+* ``IMEXSourceSolver`` with the local Newton source stage
+  (``zoomy_jax/fvm/solver_imex_jax.py:268-342``, loop :589-646)
+
+``model`` is a :class:`ShallowWater` description: the reference's
+``NumericalSystemModel`` of ``SME(level=0)`` (state ``[b, h, q_0(, q_1)]``)
+with its ``ReconstructionSpec(order, limiter)`` and Riemann class.  Arrays are
+torch tensors on the GPU, ``(n_var, n_cells)`` float64 (or float32).  dt stays
+on the device; the only host synchronisation is the dt guard
+(``assert_dt_admissible``, ``zoomy_jax/fvm/solver2/blocks.py:98-124``), which
+raises ``FloatingPointError`` exactly like the reference.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from ..mesh import Mesh
+
+
+@dataclass
+class ShallowWater:
+    """SWE numerics description (the reference's NSM for ``SME(level=0)``).
+
+    ``riemann``: ``"rusanov_nc"`` (NonconservativeRusanov: s_M dissipation on
+    every row), ``"rusanov"`` (the positive/well-balanced Rusanov: eta jump on
+    h, b untouched), ``"hll"`` (HLL with hydrostatic reconstruction).
+    ``initial_conditions``: callable ``x (2,) -> state (n_var,)`` or an array.
+    """
+    dimension: int = 2
+    order: int = 1
+    limiter: str = "minmod"
+    riemann: str = "rusanov_nc"
+    g: float = 9.81
+    manning_n: float = 0.0
+    wet_dry_eps: float = 1e-8
+    dt_max: float = 0.0
+    initial_conditions: Optional[object] = None
+    boundary_conditions: object = "extrapolation"
+
+    @property
+    def n_state(self):
+        return 2 + self.dimension
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise L.ZoomyHipError("the HIP time step needs a GPU (torch.cuda)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class DeviceMesh:
+    """Mesh uploaded to the device (``zj_mesh_create``)."""
+
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+        lib = L.lib()
+        N = mesh.n_cells
+        ex = mesh.extra_stencil() if mesh.dim == 2 else \
+            np.full((N, 1), -1, np.int32)
+        self._keep = dict(
+            cv=np.ascontiguousarray(mesh.cell_vertices.T, np.int32),
+            nbr=np.ascontiguousarray(mesh.cell_nbr.T, np.int32),
+            ex=np.ascontiguousarray(ex.T, np.int32),
+            verts=np.ascontiguousarray(mesh.verts, np.float64),
+            cen=np.ascontiguousarray(mesh.cell_centers().T, np.float64),
+            rin=np.ascontiguousarray(mesh.cell_inradius(), np.float64),
+            bcell=np.ascontiguousarray(mesh.bface_cell, np.int32),
+            bslot=np.ascontiguousarray(mesh.bface_slot, np.int32),
+            bkind=np.ascontiguousarray(mesh.bface_kind(), np.int32),
+            bpart=np.ascontiguousarray(
+                mesh.periodic_partner if mesh.periodic_partner is not None
+                else -np.ones(mesh.n_bfaces), np.int32),
+        )
+        k = self._keep
+        d = L.MeshDesc(mesh.dim, N, len(mesh.verts), mesh.max_faces,
+                       k["ex"].shape[0], mesh.n_bfaces,
+                       *[k[n].ctypes.data for n in (
+                           "cv", "nbr", "ex", "verts", "cen", "rin", "bcell",
+                           "bslot", "bkind", "bpart")])
+        h = ctypes.c_void_p()
+        L.check(lib.zj_mesh_create(ctypes.byref(d), ctypes.byref(h)),
+                "zj_mesh_create")
+        self.handle = h
+        self.cell_volumes = mesh.cell_volumes()
+        self.cell_centers = mesh.cell_centers()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.zj_mesh_destroy(h)
+            self.handle = None
+
+
+class _Runtime:
+    """Solver context (``zj_solver_create``): mesh + numerics + scratch."""
+
+    def __init__(self, dmesh: DeviceMesh, model: ShallowWater, cfl: float,
+                 dtype: torch.dtype, recon: str):
+        lib = L.lib()
+        self.dmesh = dmesh
+        self.model = model
+        self.dtype = dtype
+        self.params = L.Params(
+            float(model.g), float(cfl), float(model.wet_dry_eps),
+            float(model.dt_max), float(model.manning_n), int(model.order),
+            L.LIMITERS[model.limiter], L.RIEMANN[model.riemann],
+            L.RECON[recon],
+            L.DTYPES["float64" if dtype == torch.float64 else "float32"])
+        h = ctypes.c_void_p()
+        L.check(lib.zj_solver_create(dmesh.handle, ctypes.byref(self.params),
+                                     ctypes.byref(h)), "zj_solver_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.zj_solver_destroy(h)
+            self.handle = None
+
+
+class HyperbolicSolver:
+    """Explicit FV march on the GPU (``zoomy_jax.fvm.solver_jax.
+    HyperbolicSolver``).  ``cfl`` is the reference's ``CFL`` of
+    ``timestepping.adaptive(CFL, dimension)`` (the 1/d factor is applied
+    inside, ``tests/conftest.py:80-93``)."""
+
+    def __init__(self, time_end: float = 1.0, cfl: float = 0.9,
+                 dtype: torch.dtype = torch.float64,
+                 reconstruction_variables: str = "conservative",
+                 compute_dt: Optional[Callable] = None):
+        if reconstruction_variables not in L.RECON:
+            raise ValueError(
+                f"reconstruction_variables must be one of {sorted(L.RECON)}")
+        self.time_end = float(time_end)
+        self.cfl = float(cfl)
+        self.dtype = dtype
+        self.reconstruction_variables = reconstruction_variables
+        self.compute_dt = compute_dt
+
+    # ── setup ─────────────────────────────────────────────────────────
+    def initialize(self, mesh: Mesh, model: ShallowWater):
+        N = mesh.n_cells
+        ic = model.initial_conditions
+        if ic is None:
+            Q = np.zeros((4, N))
+        elif callable(ic):
+            xc = mesh.cell_centers()
+            rows = np.array([np.asarray(ic(xc[i]), np.float64)
+                             for i in range(N)]).T
+            Q = np.zeros((4, N))
+            Q[:rows.shape[0]] = rows
+        else:
+            ic = np.asarray(ic, np.float64)
+            Q = np.zeros((4, N))
+            Q[:ic.shape[0]] = ic
+        return Q
+
+    def setup_simulation(self, mesh: Mesh, model: ShallowWater):
+        if model.dimension != mesh.dim:
+            raise ValueError(f"model dimension {model.dimension} does not "
+                             f"match mesh dimension {mesh.dim}")
+        if not mesh.bc:
+            mesh.set_bc(model.boundary_conditions)
+        self.mesh = mesh
+        self.model = model
+        self.n_state = model.n_state
+        self._dmesh = DeviceMesh(mesh)
+        self._rt = _Runtime(self._dmesh, model, self.cfl, self.dtype,
+                            self.reconstruction_variables)
+        dev = _device()
+        self.device = dev
+        self._dt = torch.zeros((), dtype=torch.float64, device=dev)
+        self._t = torch.zeros((), dtype=torch.float64, device=dev)
+        self._troubled = torch.zeros((2,), dtype=torch.int32, device=dev)
+        self._mass = torch.zeros((), dtype=torch.float64, device=dev)
+        Q = torch.as_tensor(self.initialize(mesh, model), dtype=self.dtype,
+                            device=dev).contiguous()
+        Qaux = torch.zeros((3 * mesh.dim + 1, mesh.n_cells),
+                           dtype=self.dtype, device=dev)
+        Qaux = self.update_qaux(Q, Qaux)
+        return self._view(Q), Qaux
+
+    @property
+    def runtime(self):
+        return self._rt
+
+    # ── state helpers ─────────────────────────────────────────────────
+    def _full(self, Q):
+        """(4, N) contiguous device tensor for the kernels."""
+        if Q.dtype != self.dtype or Q.device != self.device:
+            Q = Q.to(device=self.device, dtype=self.dtype)
+        if Q.shape[0] == 4 and Q.is_contiguous():
+            return Q
+        base = getattr(Q, "_base", None)
+        if Q.shape[0] == 3 and base is not None and base.shape[0] == 4 \
+                and base.is_contiguous() and Q.data_ptr() == base.data_ptr():
+            return base
+        out = torch.zeros((4, Q.shape[1]), dtype=self.dtype,
+                          device=self.device)
+        out[:Q.shape[0]] = Q
+        return out
+
+    def _view(self, Q4):
+        return Q4[:self.n_state] if self.n_state == 3 else Q4
+
+    def _dt_dev(self, dt):
+        if torch.is_tensor(dt) and dt.device == self.device and \
+                dt.dtype == torch.float64 and dt.numel() == 1:
+            return dt.reshape(())
+        self._dt.fill_(float(dt))
+        return self._dt
+
+    # ── operators ─────────────────────────────────────────────────────
+    def get_flux_operator(self, mesh=None, runtime=None):
+        lib = L.lib()
+
+        def flux_operator(dt, time, Q, Qaux, parameters=None, dQ=None,
+                          force_o1=None):
+            if force_o1 is not None:
+                raise NotImplementedError(
+                    "force_o1 (MOOD demotion) is not supported by this "
+                    "reconstruction")
+            Q4 = self._full(Q)
+            out = torch.empty_like(Q4)
+            L.check(lib.zj_flux_operator(self._rt.handle, _p(Q4), _p(out),
+                                         _stream()), "zj_flux_operator")
+            return self._view(out)
+
+        return flux_operator
+
+    def compute_timestep(self, Q, Qaux=None, parameters=None,
+                         t_remaining: Optional[float] = None):
+        Q4 = self._full(Q)
+        dt = torch.empty((), dtype=torch.float64, device=self.device)
+        L.check(L.lib().zj_compute_dt(
+            self._rt.handle, _p(Q4),
+            -1.0 if t_remaining is None else float(t_remaining), _p(dt),
+            _stream()), "zj_compute_dt")
+        return dt
+
+    def step(self, dt, time, Q, Qaux=None, parameters=None):
+        Q4 = self._full(Q)
+        out = torch.empty_like(Q4)
+        if self.n_state == 3:
+            out[3].zero_()
+        L.check(L.lib().zj_step(self._rt.handle, _p(self._dt_dev(dt)),
+                                _p(Q4), _p(out), _p(self._troubled),
+                                _stream()), "zj_step")
+        return self._view(out)
+
+    def troubled_count(self) -> int:
+        """Cells of the last step with h < 0 or a non-finite component
+        (``zoomy_jax/fvm/solver2/blocks.py:290-301``); synchronises."""
+        return int(self._troubled[0].item())
+
+    def update_qaux(self, Q, Qaux):
+        Q4 = self._full(Q)
+        if Qaux is None or Qaux.shape != (3 * self.mesh.dim + 1,
+                                          Q4.shape[1]):
+            Qaux = torch.zeros((3 * self.mesh.dim + 1, Q4.shape[1]),
+                               dtype=self.dtype, device=self.device)
+        L.check(L.lib().zj_update_aux(self._rt.handle, _p(Q4), _p(Qaux),
+                                      _stream()), "zj_update_aux")
+        return Qaux
+
+    def post_step(self, time, dt, Q, Qold, Qaux, parameters=None):
+        return Q, self.update_qaux(Q, Qaux)
+
+    def assert_dt_admissible(self, dt, time=0.0, iteration=0):
+        d = float(dt)
+        if np.isfinite(d) and d > 0.0:
+            return d
+        raise FloatingPointError(
+            f"reduce_dt returned dt = {d!r} at t = {float(time)} (iteration "
+            f"{int(iteration)}) -- FATAL: a non-positive or non-finite dt "
+            f"aborts the march")
+
+    def time_kernels(self, Q, dt, reps: int = 20):
+        """Average ms per launch of {slopes, flux stage 1, flux stage 2, dt,
+        aux} measured with HIP events on the current stream."""
+        Q4 = self._full(Q)
+        ms = (ctypes.c_float * 5)()
+        L.check(L.lib().zj_time_kernels(self._rt.handle, _p(Q4),
+                                        _p(self._dt_dev(dt)), int(reps),
+                                        ctypes.cast(ms, ctypes.c_void_p),
+                                        _stream()), "zj_time_kernels")
+        return dict(zip(("slopes", "flux_stage1", "flux_stage2", "dt",
+                         "aux"), list(ms)))
+
+    def compute_mass(self, Q):
+        """Compensated FP64 sum of h |cell| (device scalar)."""
+        Q4 = self._full(Q)
+        L.check(L.lib().zj_mass(self._rt.handle, _p(Q4), _p(self._mass),
+                                _stream()), "zj_mass")
+        return self._mass.clone()
+
+    # ── loops ─────────────────────────────────────────────────────────
+    def march_device(self, Q, n_steps: int, t0: float = 0.0,
+                     time_end: Optional[float] = None, Qaux=None):
+        """``n_steps`` adaptive steps entirely on the device, dt capped by
+        ``time_end - t`` (``run_simulation``'s loop body, with the post_step
+        aux refresh into ``Qaux`` when given).  Q is updated in place;
+        returns ``(Q, t_device, dt_device)``."""
+        Q4 = self._full(Q)
+        self._t.fill_(float(t0))
+        te = self.time_end if time_end is None else time_end
+        L.check(L.lib().zj_march(self._rt.handle, _p(Q4), _p(Qaux),
+                                 int(n_steps),
+                                 float(te) if np.isfinite(te) else -1.0,
+                                 _p(self._t), _p(self._dt),
+                                 _p(self._troubled), _stream()), "zj_march")
+        return self._view(Q4), self._t, self._dt
+
+    def run_simulation(self, Q, Qaux, write_output=False, parameters=None,
+                       max_steps: int = 10 ** 9):
+        """Adaptive march to ``time_end``; the dt guard reads dt each step
+        (the solver2 march's host guard)."""
+        t = 0.0
+        it = 0
+        while t < self.time_end and it < max_steps:
+            dt = self.compute_timestep(Q, Qaux,
+                                       t_remaining=self.time_end - t)
+            d = self.assert_dt_admissible(dt, t, it)
+            Qn = self.step(dt, t, Q, Qaux)
+            Q, Qaux = self.post_step(t + d, d, Qn, Q, Qaux)
+            t += d
+            it += 1
+        self.n_steps_taken = it
+        return Q, Qaux
+
+    def solve(self, mesh: Mesh, model: ShallowWater, write_output=False):
+        Q, Qaux = self.setup_simulation(mesh, model)
+        return self.run_simulation(Q, Qaux, write_output=write_output)
+
+
+class IMEXSourceSolver(HyperbolicSolver):
+    """Explicit hyperbolic step + local implicit source
+    (``zoomy_jax/fvm/solver_imex_jax.py``: ``implicit_maxiter = 6``, :142)."""
+
+    implicit_maxiter = 6
+
+    def step(self, dt, time, Q, Qaux=None, parameters=None):
+        Q4 = self._full(Q)
+        Qe = self._hyperbolic_only(dt, Q4)   # explicit stage, no source
+        out = torch.empty_like(Q4)
+        L.check(L.lib().zj_imex_local(self._rt.handle, _p(self._dt_dev(dt)),
+                                      int(self.implicit_maxiter), _p(Qe),
+                                      _p(out), _stream()), "zj_imex_local")
+        return self._view(out)
+
+    def setup_simulation(self, mesh, model):
+        out = super().setup_simulation(mesh, model)
+        # a second runtime without the explicit source for the hyperbolic
+        # stage (the implicit stage owns the friction)
+        import copy
+        m2 = copy.copy(model)
+        m2.manning_n = 0.0
+        self._rt_h = _Runtime(self._dmesh, m2, self.cfl, self.dtype,
+                              self.reconstruction_variables)
+        return out
+
+    def _hyperbolic_only(self, dt, Q4):
+        out = torch.empty_like(Q4)
+        L.check(L.lib().zj_step(self._rt_h.handle, _p(self._dt_dev(dt)),
+                                _p(Q4), _p(out), _p(self._troubled),
+                                _stream()), "zj_step")
+        return out
