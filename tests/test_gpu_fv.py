@@ -91,7 +91,7 @@ def test_config1_dam_break_vs_oracle():
     for r in (1, 2):
         assert rel_l2(Qg[r], Qo[r]) < 1e-12
     mass = float(s.compute_mass(Q))
-    assert abs(mass - 1.5) < 1e-12
+    assert abs(mass - o.mass(Qo)) < 1e-14
 
 
 @pytest.mark.parametrize("riemann", ["rusanov_nc", "rusanov", "hll"])

@@ -1,0 +1,263 @@
+"""The oracle pinned by what the mathematics fixes: closed forms, invariants
+and special cases, chosen so that a dropped term, a wrong sign or index or a
+transposed operand fails one of them."""
+import numpy as np
+import pytest
+
+from helpers import G, circular_dam_state, dam1d_ic, ic_rows
+from oracle.oracle import Oracle, face_flux
+from zoomy_hip.mesh import Mesh, delaunay_mesh, jittered_points
+
+
+def stoker_exact(x, t, hl, hr, x0, g=G):
+    """Exact wet dam break (Stoker): rarefaction + shock."""
+    cl, cr = np.sqrt(g * hl), np.sqrt(g * hr)
+    # middle state: u_m = 2(cl - cm), shock: hr, hm Rankine-Hugoniot
+    def f(cm):
+        hm = cm * cm / g
+        um = 2 * (cl - cm)
+        s = um * hm / (hm - hr)          # shock speed from mass
+        # momentum RH: s (hm um) = hm um^2 + g/2 (hm^2 - hr^2)
+        return s * hm * um - (hm * um * um + 0.5 * g * (hm * hm - hr * hr))
+    a, b = cr * 1.0000001, cl * 0.9999999
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if f(a) * f(m) <= 0:
+            b = m
+        else:
+            a = m
+    cm = 0.5 * (a + b)
+    hm, um = cm * cm / g, 2 * (cl - cm)
+    s = um * hm / (hm - hr)
+    xi = (np.asarray(x) - x0) / t
+    h = np.where(xi < -cl, hl, 0.0)
+    fan = (xi >= -cl) & (xi < um - cm)
+    h = np.where(fan, (2 * cl - xi) ** 2 / (9 * g), h)
+    h = np.where((xi >= um - cm) & (xi < s), hm, h)
+    h = np.where(xi >= s, hr, h)
+    return h
+
+
+def test_config1_mass_and_stoker_convergence():
+    """1-D dam break h=2|1 on [0,1], Rusanov o1, Euler, CFL 0.45, t=0.1:
+    the physical signal (4.43 m/s) does not reach the ends, so the mass moves
+    only by the numerical-diffusion tail reaching the transmissive ends (and
+    not at all between walls); the L1 error against the exact Stoker
+    solution decreases at first order."""
+    errs = []
+    for n in (250, 500, 1000):
+        m = Mesh.line(n, 0.0, 1.0).set_bc("extrapolation")
+        o = Oracle(m, order=1, riemann="rusanov_nc", cfl=0.45)
+        Q, t, k = o.march(ic_rows(m, dam1d_ic), t_end=0.1)
+        assert abs(t - 0.1) < 1e-14
+        assert abs(o.mass(Q) - 1.5) < 2e-4
+        mw = Mesh.line(n, 0.0, 1.0).set_bc("wall")
+        ow = Oracle(mw, order=1, riemann="rusanov_nc", cfl=0.45)
+        Qw, _, _ = ow.march(ic_rows(mw, dam1d_ic), t_end=0.1)
+        assert abs(ow.mass(Qw) - 1.5) < 1e-14
+        x = m.cell_centers()[:, 0]
+        he = stoker_exact(x, 0.1, 2.0, 1.0, 0.5)
+        errs.append(np.mean(np.abs(Q[1] - he)))
+    rate = np.log(errs[0] / errs[2]) / np.log(4)
+    assert errs[2] < 5e-3
+    assert 0.5 < rate < 1.2
+
+
+def test_equal_states_zero_fluctuation():
+    """Equal L/R states: Dp = Dm = 0 (Rusanov) and F = physical flux
+    (test_uniform_rank_malpasset_req84.py:68-81)."""
+    q = np.array([0.1, 1.3, 0.4, -0.2])
+    n = np.array([0.6, 0.8])
+    hi = 1 / q[1]
+    for rs in ("rusanov_nc", "rusanov"):
+        F, Dp, Dm = face_flux(q, q, hi, hi, n, riemann=rs)
+        assert np.abs(Dp).max() == 0 and np.abs(Dm).max() == 0
+        un = (q[2] * n[0] + q[3] * n[1]) * hi
+        phys = [0, q[2] * n[0] + q[3] * n[1],
+                q[2] * un + 0.5 * G * q[1] ** 2 * n[0],
+                q[3] * un + 0.5 * G * q[1] ** 2 * n[1]]
+        assert np.allclose(F, phys, rtol=1e-15, atol=1e-15)
+    F, Dp, Dm = face_flux(q, q, hi, hi, n, riemann="hll")
+    assert np.allclose(F, phys, rtol=1e-14, atol=1e-15)
+    assert np.abs(Dm).max() == 0 and np.abs(Dp).max() == 0
+
+
+@pytest.mark.parametrize("rs", ["rusanov_nc", "rusanov", "hll"])
+def test_face_numerics_antisymmetric(rs):
+    """Evaluating a face from the other side (swap states, flip n) gives the
+    other cell's update: F'+Dm' = -F+Dp.  This is what lets the gather
+    kernels evaluate each face from both cells."""
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        qL = np.array([rng.uniform(0, .3), rng.uniform(.1, 2),
+                       rng.normal(0, .5), rng.normal(0, .5)])
+        qR = np.array([rng.uniform(0, .3), rng.uniform(.1, 2),
+                       rng.normal(0, .5), rng.normal(0, .5)])
+        a = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(a), np.sin(a)])
+        hL, hR = 1 / qL[1], 1 / qR[1]
+        F, Dp, Dm = face_flux(qL, qR, hL, hR, n, riemann=rs)
+        F2, Dp2, Dm2 = face_flux(qR, qL, hR, hL, -n, riemann=rs)
+        assert np.allclose(F2 + Dm2, -F + Dp, rtol=1e-13, atol=1e-14)
+        assert np.allclose(-F2 + Dp2, F + Dm, rtol=1e-13, atol=1e-14)
+
+
+def test_hll_supersonic_upwinding():
+    """s_L >= 0: HLL returns the left physical flux."""
+    qL = np.array([0.0, 1.0, 5.0, 0.0])
+    qR = np.array([0.0, 1.2, 6.0, 0.0])
+    n = np.array([1.0, 0.0])
+    F, _, _ = face_flux(qL, qR, 1.0, 1 / 1.2, n, riemann="hll")
+    assert np.allclose(F, [0, 5.0, 25.0 + 0.5 * G, 0], rtol=1e-15)
+
+
+def _lake(mesh, seed=0):
+    xc = mesh.cell_centers()
+    rng = np.random.default_rng(seed)
+    b = 0.2 * np.exp(-((xc - rng.uniform(0.3, 0.7, 2)) ** 2).sum(1) / 0.02)
+    Q = np.zeros((4, mesh.n_cells))
+    Q[0] = b
+    Q[1] = 1.0 - b
+    return Q
+
+
+@pytest.mark.parametrize("order,recon", [(1, "conservative"), (2, "eta")])
+@pytest.mark.parametrize("kind", ["quad", "tri"])
+def test_lake_at_rest_hll(order, recon, kind):
+    if kind == "quad":
+        m = Mesh.quad(24, 20).set_bc("wall")
+    else:
+        pts = jittered_points(15, 13, (0.0, 1.0, 0.0, 1.0), 0.3, 5)
+        m = delaunay_mesh(pts, (0.0, 1.0, 0.0, 1.0)).set_bc("wall")
+    o = Oracle(m, order=order, limiter="minmod", riemann="hll", recon=recon)
+    Q0 = _lake(m)
+    Q, _, _ = o.march(Q0, n_steps=5)
+    assert np.abs(Q[0] + Q[1] - 1.0).max() < 1e-13
+    assert np.abs(Q[2:]).max() < 1e-13
+
+
+def test_conservation_walls_o2():
+    m = Mesh.quad(30, 30).set_bc("wall")
+    Q0 = circular_dam_state(m, seed=1)
+    for rs in ("rusanov", "hll"):
+        o = Oracle(m, order=2, limiter="minmod", riemann=rs)
+        Q, _, _ = o.march(Q0, n_steps=8)
+        assert abs(o.mass(Q) - o.mass(Q0)) < 1e-13 * o.mass(Q0)
+        assert np.array_equal(Q[0], Q0[0])     # bed untouched
+
+
+def test_lsq_linear_exact():
+    """LSQ gradients reproduce a linear field exactly, including boundary
+    and corner cells when the ghost carries the linear value
+    (zoomy_jax/mesh/mesh.py:42-45)."""
+    for m in (Mesh.quad(9, 7, (0, 2, 0, 1)),
+              delaunay_mesh(jittered_points(9, 8, (0, 2, 0, 1), 0.3, 2),
+                            (0, 2, 0, 1))):
+        m.set_bc("extrapolation")
+        o = Oracle(m)
+        xc = m.cell_centers()
+        u = 0.3 + 1.7 * xc[:, 0] - 0.9 * xc[:, 1]
+        rm = o.rm
+        ghost = np.zeros(m.n_bfaces)
+        for b in range(m.n_bfaces):
+            f = rm.bface_face[b]
+            c = m.bface_cell[b]
+            g = 2 * rm.face_centers[:, f] - xc[c]
+            ghost[b] = 0.3 + 1.7 * g[0] - 0.9 * g[1]
+        gr = o.grad(u, ghost)
+        assert np.abs(gr[0] - 1.7).max() < 1e-12
+        assert np.abs(gr[1] + 0.9).max() < 1e-12
+
+
+def test_limited_faces_within_bounds():
+    """Barth-Jespersen/minmod: phi in [0,1] and every reconstructed face
+    value inside the cell's neighbour bounds."""
+    m = Mesh.quad(20, 20).set_bc("extrapolation")
+    rng = np.random.default_rng(4)
+    Q = np.zeros((4, m.n_cells))
+    Q[1] = 1 + rng.uniform(0, 1, m.n_cells)
+    o = Oracle(m, order=2, limiter="minmod")
+    QL, QR, lg = o.reconstruct(Q)
+    fc = o.rm.face_cells
+    h = Q[1]
+    nb = [[] for _ in range(m.n_cells)]
+    for f in range(fc.shape[1]):
+        a, b = fc[0, f], fc[1, f]
+        if b >= 0:
+            nb[a].append(b)
+            nb[b].append(a)
+    for f in range(fc.shape[1]):
+        for side, c in ((0, fc[0, f]), (1, fc[1, f])):
+            if c < 0:
+                continue
+            v = (QL if side == 0 else QR)[1, f]
+            lo = min([h[c]] + [h[j] for j in nb[c]])
+            hi = max([h[c]] + [h[j] for j in nb[c]])
+            assert lo - 1e-14 <= v <= hi + 1e-14
+
+
+def test_vanleer_is_1d_van_leer_muscl():
+    """On a uniform 1-D mesh the van Leer cell limiter gives the harmonic
+    van Leer slope 2ab/(a+b) (a, b one-sided differences)."""
+    m = Mesh.line(40, 0.0, 1.0).set_bc("extrapolation")
+    rng = np.random.default_rng(2)
+    u = np.cumsum(rng.uniform(0.01, 1.0, 40)) * 0.01
+    Q = np.zeros((4, 40))
+    Q[1] = 1 + u
+    o = Oracle(m, order=2, limiter="vanleer")
+    _, _, lg = o.reconstruct(Q)
+    dx = 1 / 40
+    a = (Q[1, 1:-1] - Q[1, :-2]) / dx
+    b = (Q[1, 2:] - Q[1, 1:-1]) / dx
+    vl = 2 * a * b / (a + b)
+    assert np.allclose(lg[1, 0, 1:-1], vl, rtol=1e-12)
+    # Barth-Jespersen gives the minmod-like min(2a, 2b, (a+b)/2)
+    o2 = Oracle(m, order=2, limiter="minmod")
+    _, _, lg2 = o2.reconstruct(Q)
+    bj = np.minimum(np.minimum(2 * a, 2 * b), 0.5 * (a + b))
+    assert np.allclose(lg2[1, 0, 1:-1], bj, rtol=1e-12)
+
+
+def test_imex_manning_closed_form():
+    """1-D implicit Manning: q + a q|q| = q_e with a = dt g n^2 hinv^(7/3)
+    has the closed form q = sgn(q_e)(-1 + sqrt(1 + 4 a |q_e|)) / (2a),
+    evaluated without cancellation as 2 q_e / (1 + sqrt(1 + 4 a |q_e|))."""
+    m = Mesh.line(8, 0.0, 1.0).set_bc("wall")
+    Q = np.zeros((4, 8))
+    Q[1] = np.linspace(0.5, 2.0, 8)
+    Q[2] = np.linspace(-1.0, 1.5, 8)
+    dt, n = 0.05, 0.03
+    o = Oracle(m, manning_n=n)
+    out = o.imex_local(dt, Q, maxiter=6)
+    hinv = 1 / Q[1]
+    a = dt * G * n * n * hinv ** (7.0 / 3.0)
+    qe = Q[2]
+    exact = 2 * qe / (1 + np.sqrt(1 + 4 * a * np.abs(qe)))
+    assert np.allclose(out[2], exact, rtol=1e-13, atol=1e-15)
+    assert np.array_equal(out[:2], Q[:2])
+
+
+def test_wave_free_dt_max():
+    """All-dry domain: every |lambda| = 0, dt = dt_max
+    (test_dt_max_wave_free_jax_req190.py:86-98)."""
+    m = Mesh.quad(6, 6).set_bc("wall")
+    o = Oracle(m, dt_max=0.25)
+    assert o.dt(np.zeros((4, 36))) == 0.25
+
+
+def test_circular_dam_symmetry():
+    """Flat-bed circular dam on a square mesh: the march commutes with the
+    x<->y transpose of the grid (to round-off)."""
+    n = 24
+    m = Mesh.quad(n, n).set_bc("wall")
+    xc = m.cell_centers()
+    Q0 = np.zeros((4, m.n_cells))
+    Q0[1] = np.where(((xc - 0.5) ** 2).sum(1) < 0.0625, 2.0, 1.0)
+    o = Oracle(m, order=2, limiter="minmod", riemann="hll")
+    Q, _, _ = o.march(Q0, n_steps=10)
+    h = Q[1].reshape(n, n)
+    hu = Q[2].reshape(n, n)
+    hv = Q[3].reshape(n, n)
+    assert np.abs(h - h.T).max() < 1e-13
+    assert np.abs(hu - hv.T).max() < 1e-13
+    assert np.abs(h - h[::-1]).max() < 1e-13
