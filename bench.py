@@ -104,6 +104,20 @@ def cpu_leg(budget_s=12.0):
                 elapsed=el)
 
 
+def summarize_leg(r, w, N, steps):
+    cups = N * steps / r["elapsed"]
+    B = algorithmic_bytes(w)
+    ms = r["ms"]
+    dom = max(("slopes", "flux_stage1", "flux_stage2"), key=lambda k: ms[k])
+    ach = B[dom] * N / (ms[dom] * 1e-3)
+    return cups, dict(
+        bound="hbm", kernel=dom, achieved=ach / 1e9, peak=PEAK_HBM / 1e9,
+        unit="GB/s", frac=ach / PEAK_HBM, traffic=None,
+        bytes_per_launch=B[dom] * N, launch_ms=ms[dom],
+        step_alg_bytes_per_cell=B["cell_update"],
+        step_frac=cups * B["cell_update"] / PEAK_HBM)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +125,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 1 GiB (10 times): "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 8,
+                          "launches": 10}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the multi-GPU halo exchange is "
               "out of scope for this build (see DESIGN.md); use --gpus 1",
@@ -123,24 +151,19 @@ def main():
     mesh = Mesh.quad(NX, NY).set_bc("wall")
     Q0 = build_state(mesh)
     N = mesh.n_cells
-    r64 = gpu_leg(mesh, Q0, torch.float64, args.steps, args.warmup)
+    r64 = None if args.no_fp64 else gpu_leg(mesh, Q0, torch.float64,
+                                             args.steps, args.warmup)
     r32 = None if args.no_fp32 else gpu_leg(mesh, Q0, torch.float32,
                                              args.steps, args.warmup)
+    if r64 is None:
+        cups32, roof32 = summarize_leg(r32, 4, N, args.steps)
+        print(json.dumps({"fp32_only": True, "value": cups32,
+                          "roofline": roof32, "kernel_ms": r32["ms"]}))
+        return
     cpu = None if args.no_cpu else cpu_leg()
 
     def summarize(r, w):
-        cups = N * args.steps / r["elapsed"]
-        B = algorithmic_bytes(w)
-        ms = r["ms"]
-        dom = max(("slopes", "flux_stage1", "flux_stage2"),
-                  key=lambda k: ms[k])
-        ach = B[dom] * N / (ms[dom] * 1e-3)
-        return cups, dict(
-            bound="hbm", kernel=dom, achieved=ach / 1e9, peak=PEAK_HBM / 1e9,
-            unit="GB/s", frac=ach / PEAK_HBM, traffic=None,
-            bytes_per_launch=B[dom] * N, launch_ms=ms[dom],
-            step_alg_bytes_per_cell=B["cell_update"],
-            step_frac=cups * B["cell_update"] / PEAK_HBM)
+        return summarize_leg(r, w, N, args.steps)
 
     cups64, roof64 = summarize(r64, 8)
     out = {
