@@ -182,16 +182,25 @@ def test_triangles_eta_rcm_fp64():
     assert max_rel_rows(Qg.cpu().numpy(), Qo) < 1e-12
 
 
-def test_device_march_and_mass():
+@pytest.mark.parametrize("morton", [False, True])
+def test_device_march_and_mass(morton):
     from zoomy_hip.mesh import Mesh
-    mesh = Mesh.quad(40, 40).set_bc("wall")
+    mesh = Mesh.quad(40, 40)
+    if morton:
+        mesh = mesh.reorder(mesh.morton_permutation())
+    mesh.set_bc("wall")
     Q0 = circular_dam_state(mesh, seed=2)
     o = _oracle(mesh, order=2, riemann="hll")
     s = _solver(mesh, order=2, riemann="hll")
     Qo, to, ko = o.march(Q0, n_steps=6)
-    Qg, t_dev, dt_dev = s.march_device(_to_dev(Q0), 6)
+    Qaux = torch.zeros((7, mesh.n_cells), dtype=torch.float64, device="cuda")
+    Qg, t_dev, dt_dev = s.march_device(_to_dev(Q0), 6, Qaux=Qaux)
     assert abs(float(t_dev) - to) <= 1e-12 * to
     assert max_rel_rows(Qg.cpu().numpy(), Qo) < 1e-12
+    Ao = o.aux(Qo)
+    Ag = Qaux.cpu().numpy()
+    for r in range(7):
+        assert rel_l2(Ag[r], Ao[r]) < 1e-12
     m0 = o.mass(Q0)
     mg = float(s.compute_mass(Qg))
     assert abs(mg - o.mass(Qo)) <= 1e-14 * m0

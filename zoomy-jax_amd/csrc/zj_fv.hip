@@ -138,7 +138,11 @@ __global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= m.N) return;
     const int N = m.N;
-    const bool aux = Qaux != nullptr;
+    // aux only (S == nullptr), slopes only (Qaux == nullptr), or both from
+    // one gather (conservative reconstruction: the aux rows are the same
+    // LSQ fit with zero ghost deltas)
+    const bool aux = Qaux != nullptr && S == nullptr;
+    const bool both = Qaux != nullptr && S != nullptr;
     const int recon = aux ? RECON_CONS : o.recon;
     const T eps = T(o.eps);
     T q[4];
@@ -241,16 +245,37 @@ __global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
         }
     }
 
-    if (aux) {
+    if (aux || both) {
         // rows: per direction d [d q_d/dx_d, d h/dx_d], then d b/dx_d, hinv
+        T ax[4], ay[4];
+        if (both) {  // same normal matrices, inner deltas only
+            if (m.dim == 1) {
+                const T inv = T(1.0 / (Sxx + Gxx[0]));
+#pragma unroll
+                for (int v = 0; v < 4; ++v) { ax[v] = inv * bx[v]; ay[v] = T(0); }
+            } else {
+                const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
+                const double di0 = 1.0 / (axx * ayy - axy * axy);
+                const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
+                const double di1 = 1.0 / (bxx * byy - bxy * bxy);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    ax[v] = T(ayy * di0) * bx[v] + T(-axy * di0) * by[v];
+                    ay[v] = T(-bxy * di1) * bx[v] + T(bxx * di1) * by[v];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { ax[v] = gx[v]; ay[v] = gy[v]; }
+        }
         int row = 0;
         for (int d = 0; d < m.dim; ++d) {
-            Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[2] : gy[3];
-            Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[1] : gy[1];
+            Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[2] : ay[3];
+            Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[1] : ay[1];
         }
-        for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? gx[0] : gy[0];
+        for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[0] : ay[0];
         Qaux[(size_t)row * N + c] = hic;
-        return;
+        if (aux) return;
     }
 
     T phi[4] = {1, 1, 1, 1};
@@ -690,7 +715,7 @@ namespace {
 
 template <typename T>
 int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled,
-              unsigned long long* dt_red, hipStream_t st) {
+              unsigned long long* dt_red, hipStream_t st, T* aux_of_Q = nullptr) {
     const DevMesh& m = s->m->d;
     const int nb = nblocks(m.N);
     T* S = (T*)s->S;
@@ -715,7 +740,7 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
     if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
     if (!write_b && Qout != Q)
         HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, (T*)nullptr);
+    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, aux_of_Q);
     hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)S,
                        (const T*)nullptr, Q1, dt, 0, 0, write_b, (int*)nullptr,
                        (unsigned long long*)nullptr);
@@ -817,18 +842,28 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
         hipLaunchKernelGGL(k_dt<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
                            (const float*)Q, s->dt_red);
     HIPCHK(hipGetLastError());
+    // post_step aux refresh: fused into the first slope pass of the next
+    // step (same gather) when the reconstruction is conservative and the
+    // step is order 2; the state after the last step gets its own pass.
+    const bool fuse = Qaux && s->o.order >= 2 && s->o.recon == RECON_CONS;
     for (int k = 0; k < n_steps; ++k) {
         hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev,
                            (const double*)t_dev, t_end, s->p.dt_max);
         int rc = s->p.dtype == ZJ_F64
-                     ? step_impl(s, dt_dev, (const double*)Q, (double*)Q, troubled_out, s->dt_red, st)
-                     : step_impl(s, dt_dev, (const float*)Q, (float*)Q, troubled_out, s->dt_red, st);
+                     ? step_impl(s, dt_dev, (const double*)Q, (double*)Q, troubled_out, s->dt_red, st,
+                                 fuse && k > 0 ? (double*)Qaux : (double*)nullptr)
+                     : step_impl(s, dt_dev, (const float*)Q, (float*)Q, troubled_out, s->dt_red, st,
+                                 fuse && k > 0 ? (float*)Qaux : (float*)nullptr);
         if (rc) return rc;
-        if (Qaux) {  // post_step aux refresh of the new state
+        if (Qaux && !fuse) {
             rc = zj_update_aux(s, Q, Qaux, stream);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_t_advance, dim3(1), dim3(1), 0, st, t_dev, (const double*)dt_dev);
+    }
+    if (Qaux && fuse && n_steps > 0) {
+        int rc = zj_update_aux(s, Q, Qaux, stream);
+        if (rc) return rc;
     }
     HIPCHK(hipGetLastError());
     return ZJ_OK;

@@ -475,8 +475,28 @@ class Mesh:
         return self._extra
 
     # ── reordering ────────────────────────────────────────────────────
-    def reorder(self, perm: np.ndarray) -> "Mesh":
-        """New mesh with cells in order ``perm`` (new id i = old perm[i])."""
+    def morton_permutation(self, bits: int = 16) -> np.ndarray:
+        """Cell permutation along the Z-order (Morton) curve of the cell
+        centres (the reference orders points the same way in
+        ``gnn_blueprint/mesh_spectral_geometry.py:34-46``).  On a 2^k x 2^k
+        grid every run of 4^m consecutive cells is a 2^m x 2^m tile, so a
+        block of 256 cells gathers its neighbours from one 16 x 16 tile and
+        its one-cell ring."""
+        xc = self.cell_centers()
+        lo = self.verts.min(0)
+        span = np.maximum(self.verts.max(0) - lo, 1e-300)
+        q = np.floor((xc - lo) / span * (1 << bits)).astype(np.int64)
+        q = np.clip(q, 0, (1 << bits) - 1)
+        key = np.zeros(len(xc), np.int64)
+        for b in range(bits):
+            key |= ((q[:, 0] >> b) & 1) << (2 * b)
+            key |= ((q[:, 1] >> b) & 1) << (2 * b + 1)
+        return np.argsort(key, kind="stable")
+
+    def reorder(self, perm: np.ndarray, renumber_vertices: bool = True) -> "Mesh":
+        """New mesh with cells in order ``perm`` (new id i = old perm[i]);
+        vertices are renumbered in order of first use by the new cell order
+        so that vertex gathers follow the cells."""
         perm = np.asarray(perm, np.int64)
         N = self.n_cells
         inv = np.empty(N, np.int64)
@@ -484,8 +504,18 @@ class Mesh:
         nbr = self.cell_nbr[perm].astype(np.int64)
         nbr = np.where(nbr >= 0, inv[np.maximum(nbr, 0)], nbr)
         ex = getattr(self, "_extra", None)
-        m = Mesh(dim=self.dim, verts=self.verts,
-                 cell_vertices=self.cell_vertices[perm].copy(),
+        cv = self.cell_vertices[perm]
+        verts = self.verts
+        if renumber_vertices:
+            flat = cv.ravel()
+            _, first = np.unique(flat, return_index=True)
+            order = np.unique(flat)[np.argsort(first, kind="stable")]
+            vnew = np.full(len(self.verts), -1, np.int64)
+            vnew[order] = np.arange(len(order))
+            cv = vnew[cv]
+            verts = self.verts[order]
+        m = Mesh(dim=self.dim, verts=np.ascontiguousarray(verts),
+                 cell_vertices=np.ascontiguousarray(cv, np.int32),
                  cell_nbr=nbr.astype(np.int32),
                  bface_cell=inv[self.bface_cell].astype(np.int32),
                  bface_slot=self.bface_slot.copy(),
