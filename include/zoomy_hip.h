@@ -73,6 +73,13 @@ int zj_mesh_destroy(zj_mesh *m);
 int zj_rcm(int32_t n, const int32_t *indptr, const int32_t *indices,
            int32_t *perm_out);
 
+/* Host-only check of the LDS patch decomposition zj_mesh_create builds:
+ * patches of P consecutive cells with their face ring and stencil ring,
+ * local tables shared between patches of identical structure.  stats_out[6]
+ * = {P, npatch, ntemplates, max_load, max_slope, max_vert}.  No counterpart
+ * in the reference (it keeps no device mesh layout). */
+int zj_patch_check(const zj_mesh_desc *desc, int32_t P, int32_t *stats_out);
+
 /* Solver context: owns the per-stage scratch (limited slopes, stage state,
  * reduction scalars).  Replaces HyperbolicSolver.setup_simulation
  * (zoomy_jax/fvm/solver_jax.py:901-988). */
@@ -106,8 +113,9 @@ int zj_march(zj_solver *s, void *Q, void *Qaux, int32_t n_steps,
              int32_t *troubled_out, void *stream);
 
 /* Profiling: average duration (ms) of each kernel of a step over `reps`
- * launches, timed with HIP events on `stream`: ms_out[5] = {slopes, flux
- * stage 1, flux stage 2 (+source, troubled, dt), dt, aux refresh}.  No
+ * launches, timed with HIP events on `stream`: ms_out[5] = {stage 1, stage 2
+ * (+ RK average, source, troubled flag, dt of the new state), standalone dt,
+ * standalone aux refresh, stage 1 with the fused aux refresh}.  No
  * counterpart in the reference (it has no profiler integration); used by
  * bench.py for the per-kernel roofline. */
 int zj_time_kernels(zj_solver *s, const void *Q, const double *dt,

@@ -1,0 +1,102 @@
+"""Host-side mesh layer: geometry invariants, RCM against scipy fixtures,
+and the LDS patch decomposition (built and validated on the host)."""
+import glob
+import pathlib
+
+import numpy as np
+import pytest
+
+from zoomy_hip import _lib
+from zoomy_hip.fvm.solver import patch_stats
+from zoomy_hip.mesh import Mesh, delaunay_mesh, jittered_points
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def _meshes():
+    q = Mesh.quad(37, 23, (0.0, 2.0, -1.0, 0.5))
+    yield "quad", q
+    yield "quad_morton", q.reorder(q.morton_permutation())
+    t = delaunay_mesh(jittered_points(25, 14, (0.0, 10.0, 0.0, 5.0), 0.3, 3),
+                      (0.0, 10.0, 0.0, 5.0))
+    yield "tri", t
+    ip, ix = t.adjacency_csr()
+    perm = np.zeros(t.n_cells, np.int32)
+    assert _lib.lib().zj_rcm(t.n_cells, ip.ctypes.data, ix.ctypes.data,
+                             perm.ctypes.data) == 0
+    yield "tri_rcm", t.reorder(perm)
+    yield "line", Mesh.line(300, 0.0, 10.0)
+
+
+@pytest.mark.parametrize("name,mesh", list(_meshes()))
+def test_geometry_invariants(name, mesh):
+    mesh.set_bc("wall")
+    vol = mesh.cell_volumes()
+    assert (vol > 0).all()
+    x = mesh.verts
+    area = (x[:, 0].max() - x[:, 0].min()) * (
+        (x[:, 1].max() - x[:, 1].min()) if mesh.dim == 2 else 1.0)
+    assert abs(vol.sum() - area) < 1e-12 * area
+    n, L, fc = mesh.slot_geometry()
+    assert np.allclose((n ** 2).sum(-1), 1.0)
+    # closed cells: sum of n |f| vanishes
+    assert np.abs((n * L[..., None]).sum(1)).max() < 1e-12
+    # neighbour symmetry and opposite normals across every interior face
+    N, MF = mesh.cell_nbr.shape
+    for c in range(0, N, max(1, N // 200)):
+        for k in range(MF):
+            nb = mesh.cell_nbr[c, k]
+            if nb < 0:
+                continue
+            kk = list(mesh.cell_nbr[nb]).index(c)
+            assert np.allclose(n[c, k], -n[nb, kk])
+            assert np.allclose(fc[c, k], fc[nb, kk])
+
+
+@pytest.mark.parametrize("name,mesh", list(_meshes()))
+@pytest.mark.parametrize("P", [256, 64])
+def test_patch_decomposition_valid(name, mesh, P):
+    rc, st = patch_stats(mesh, P)
+    assert rc == 0, (rc, st)
+    assert st["npatch"] == -(-mesh.n_cells // P)
+    assert st["max_slope"] >= min(P, mesh.n_cells)
+
+
+def test_patch_templates_shared_on_regular_mesh():
+    m = Mesh.quad(256, 256)
+    m = m.reorder(m.morton_permutation())
+    rc, st = patch_stats(m, 256)
+    assert rc == 0
+    # interior 16x16 tiles share one template; only boundary tiles differ
+    assert st["ntemplates"] <= 16
+    assert st["max_load"] <= 448 and st["max_slope"] <= 320
+
+
+@pytest.mark.parametrize("f", sorted(glob.glob(str(GOLDEN / "rcm_*.npz"))))
+def test_rcm_matches_scipy(f):
+    d = np.load(f)
+    n = len(d["indptr"]) - 1
+    out = np.zeros(n, np.int32)
+    ip = np.ascontiguousarray(d["indptr"], np.int32)
+    ix = np.ascontiguousarray(d["indices"], np.int32)
+    assert _lib.lib().zj_rcm(n, ip.ctypes.data, ix.ctypes.data,
+                             out.ctypes.data) == 0
+    assert np.array_equal(out, d["perm"])
+
+
+def test_rcm_reduces_bandwidth():
+    t = delaunay_mesh(jittered_points(40, 30, (0.0, 10.0, 0.0, 5.0), 0.3, 1),
+                      (0.0, 10.0, 0.0, 5.0))
+    rng = np.random.default_rng(0)
+    t = t.reorder(rng.permutation(t.n_cells))
+    ip, ix = t.adjacency_csr()
+    perm = np.zeros(t.n_cells, np.int32)
+    _lib.lib().zj_rcm(t.n_cells, ip.ctypes.data, ix.ctypes.data,
+                      perm.ctypes.data)
+    assert sorted(perm) == list(range(t.n_cells))
+
+    def bw(m):
+        nb = m.cell_nbr
+        c = np.arange(m.n_cells)[:, None]
+        return np.abs(np.where(nb >= 0, nb - c, 0)).max()
+    assert bw(t.reorder(perm)) < bw(t) / 5

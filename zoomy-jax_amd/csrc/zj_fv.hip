@@ -24,6 +24,7 @@
 
 #include "../../include/zoomy_hip.h"
 #include "zj_device.hpp"
+#include "zj_patches.hpp"
 
 using namespace zj;
 
@@ -563,6 +564,377 @@ __global__ __launch_bounds__(BLOCK) void k_imex(DevMesh m, Opts o, const double*
     Qo[3 * N + c] = qy;
 }
 
+// ─────────────────────────────────────────────────────────────────────────
+// LDS-staged stage kernel: one block per patch of up to BLOCK own cells.
+// Phase 1 stages the state and centre of every cell the patch touches and
+// the vertices of its own and face-ring cells in LDS (coalesced: own cells
+// are contiguous ids); phase 2 computes the limited slopes of own and
+// face-ring cells from LDS; phase 3 computes the fluxes of the own cells.
+// No slope array round-trips through HBM and no gather chains global ->
+// global remain (boundary faces read their BC kind from global memory).
+// ─────────────────────────────────────────────────────────────────────────
+struct DevPatch {
+    int P, npatch, ML, MS, MV;
+    const int* tpl_id;
+    const int* n_load;
+    const int* n_vert;
+    const int* load_ids;
+    const int* vert_ids;
+    const uint16_t* tpl_nbr;
+    const uint16_t* tpl_extra;
+    const uint16_t* tpl_vert;
+    const int* tpl_nslope;
+    const int* tpl_nown;
+    const double2* cen2;
+};
+
+constexpr uint16_t LNONE = 0xFFFF;
+
+__device__ __forceinline__ void geo_from_lds(const DevMesh& m, const double2* Vs,
+                                             const uint16_t* tv, int MS, int ls,
+                                             double2 c, Geo& G) {
+    G.cx = c.x;
+    G.cy = c.y;
+    if (m.dim == 1) {
+        double2 a = Vs[tv[ls]];
+        double2 b = Vs[tv[MS + ls]];
+        G.vol = b.x - a.x;
+        G.fx[0] = a.x; G.fy[0] = 0.0; G.nx[0] = -1.0; G.ny[0] = 0.0; G.len[0] = 1.0;
+        G.fx[1] = b.x; G.fy[1] = 0.0; G.nx[1] = 1.0; G.ny[1] = 0.0; G.len[1] = 1.0;
+        return;
+    }
+    double2 v[MAXF];
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k)
+        if (k < m.mf) v[k] = Vs[tv[k * MS + ls]];
+    double area = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        if (k >= m.mf) break;
+        double2 a = v[k];
+        double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+        double ex = b.x - a.x, ey = b.y - a.y;
+        double L = sqrt(ex * ex + ey * ey);
+        double iL = 1.0 / L;
+        G.len[k] = L;
+        G.nx[k] = ey * iL;
+        G.ny[k] = -ex * iL;
+        G.fx[k] = 0.5 * (a.x + b.x);
+        G.fy[k] = 0.5 * (a.y + b.y);
+        area += a.x * b.y - b.x * a.y;
+    }
+    G.vol = 0.5 * area;
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_q(const T* Qs, int ML, int i, T* q) {
+    q[0] = Qs[i];
+    q[1] = Qs[ML + i];
+    q[2] = Qs[2 * ML + i];
+    q[3] = Qs[3 * ML + i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_stage(
+    DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
+    const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
+    int write_b, int use_slopes, int* __restrict__ troubled,
+    unsigned long long* __restrict__ dt_red) {
+    extern __shared__ double smem_d[];
+    __shared__ double lds_d[BLOCK / 64];
+    __shared__ int lds_i[BLOCK / 64];
+    const int p = xcd_block();
+    const int tid = threadIdx.x;
+    const int N = m.N;
+    const int ML = pt.ML, MS = pt.MS, MV = pt.MV;
+    const int tpl = pt.tpl_id[p];
+    const int nload = pt.n_load[p], nvert = pt.n_vert[p];
+    const int nslope = pt.tpl_nslope[tpl], nown = pt.tpl_nown[tpl];
+    double2* Cs = (double2*)smem_d;
+    double* Rs = (double*)(Cs + ML);
+    double2* Vs = (double2*)(Rs + ML);
+    T* Qs = (T*)(Vs + MV);
+    T* Ss = Qs + 4 * ML;
+    const T eps = T(o.eps), g = T(o.g);
+    const int* lids = pt.load_ids + (size_t)p * ML;
+    const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
+    const uint16_t* te = pt.tpl_extra + (size_t)tpl * m.mx * MS;
+    const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
+
+    // ── phase 1: stage cells and vertices ────────────────────────────
+    for (int i = tid; i < nload; i += BLOCK) {
+        const int gi = lids[i];
+        Qs[i] = Qin[gi];
+        Qs[ML + i] = Qin[N + gi];
+        Qs[2 * ML + i] = Qin[2 * N + gi];
+        Qs[3 * ML + i] = Qin[3 * N + gi];
+        Cs[i] = pt.cen2[gi];
+        if (final_stage) Rs[i] = m.rin[gi];
+    }
+    {
+        const int* vids = pt.vert_ids + (size_t)p * MV;
+        for (int i = tid; i < nvert; i += BLOCK) Vs[i] = m.verts[vids[i]];
+    }
+    __syncthreads();
+
+    // ── phase 2: limited slopes of own + face-ring cells ─────────────
+    if (use_slopes) {
+        for (int ls = tid; ls < nslope; ls += BLOCK) {
+            const int gi = lids[ls];
+            T q[4];
+            lds_q(Qs, ML, ls, q);
+            const T hic = hinv_of(q[1], eps);
+            T w[4];
+            to_w(o.recon, q, hic, w);
+            Geo G;
+            geo_from_lds(m, Vs, tv, MS, ls, Cs[ls], G);
+            double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
+            double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
+            T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
+            T gbx[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            T gby[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            T umax[4], umin[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
+            for (int k = 0; k < m.mf; ++k) {
+                const uint16_t nl = tn[k * MS + ls];
+                if (nl != LNONE) {
+                    T qn[4], wn[4];
+                    lds_q(Qs, ML, nl, qn);
+                    to_w(o.recon, qn, hinv_of(qn[1], eps), wn);
+                    const double2 cn = Cs[nl];
+                    const double rx = cn.x - G.cx, ry = cn.y - G.cy;
+                    Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        T dv = wn[v] - w[v];
+                        bx[v] += T(rx) * dv;
+                        by[v] += T(ry) * dv;
+                        umax[v] = tmax(umax[v], wn[v]);
+                        umin[v] = tmin(umin[v], wn[v]);
+                    }
+                } else {
+                    const int bf = -1 - m.nbr[k * N + gi];
+                    const int kind = m.bf_kind[bf];
+                    T qg[4], wg[4];
+                    if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qg);
+                    else bc_ghost(kind, q, T(G.nx[k]), T(G.ny[k]), qg);
+                    to_w(o.recon, qg, hinv_of(qg[1], eps), wg);
+                    const double rx = 2.0 * (G.fx[k] - G.cx), ry = 2.0 * (G.fy[k] - G.cy);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        T bv = (kind == BC_PERIODIC) ? wg[v] : T(2) * wg[v] - w[v];
+                        umax[v] = tmax(umax[v], bv);
+                        umin[v] = tmin(umin[v], bv);
+                    }
+                    const double an[2] = {fabs(G.nx[k]), fabs(G.ny[k])};
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        if (an[d] > 0.5) {
+                            Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry;
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                T dv = wg[v] - w[v];
+                                gbx[d][v] += T(rx) * dv;
+                                gby[d][v] += T(ry) * dv;
+                            }
+                        }
+                    }
+                }
+            }
+            for (int j = 0; j < m.mx; ++j) {
+                const uint16_t el = te[j * MS + ls];
+                if (el == LNONE) break;
+                T qn[4], wn[4];
+                lds_q(Qs, ML, el, qn);
+                to_w(o.recon, qn, hinv_of(qn[1], eps), wn);
+                const double2 cn = Cs[el];
+                const double rx = cn.x - G.cx, ry = cn.y - G.cy;
+                Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    T dv = wn[v] - w[v];
+                    bx[v] += T(rx) * dv;
+                    by[v] += T(ry) * dv;
+                }
+            }
+            T gx[4], gy[4], ax[4], ay[4];
+            if (m.dim == 1) {
+                const double inv = 1.0 / (Sxx + Gxx[0]);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    gx[v] = T(inv) * (bx[v] + gbx[0][v]); gy[v] = T(0);
+                    ax[v] = T(inv) * bx[v]; ay[v] = T(0);
+                }
+            } else {
+                const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
+                const double di0 = 1.0 / (axx * ayy - axy * axy);
+                const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
+                const double di1 = 1.0 / (bxx * byy - bxy * bxy);
+                const T c0 = T(ayy * di0), c1 = T(-axy * di0);
+                const T c2 = T(-bxy * di1), c3 = T(bxx * di1);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    gx[v] = c0 * (bx[v] + gbx[0][v]) + c1 * (by[v] + gby[0][v]);
+                    gy[v] = c2 * (bx[v] + gbx[1][v]) + c3 * (by[v] + gby[1][v]);
+                    ax[v] = c0 * bx[v] + c1 * by[v];
+                    ay[v] = c2 * bx[v] + c3 * by[v];
+                }
+            }
+            if (aux_out && ls < nown) {
+                // post_step aux rows of the stage input (conservative only)
+                int row = 0;
+                for (int d = 0; d < m.dim; ++d) {
+                    aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[2] : ay[3];
+                    aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[1] : ay[1];
+                }
+                for (int d = 0; d < m.dim; ++d)
+                    aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[0] : ay[0];
+                aux_out[(size_t)row * N + gi] = hic;
+            }
+            T phi[4] = {1, 1, 1, 1};
+            if (o.lim != LIM_NONE) {
+                const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
+                for (int k = 0; k < m.mf; ++k) {
+                    const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        T d = gx[v] * rx + gy[v] * ry;
+                        phi[v] = tmin(phi[v], lim_cand(o.lim, d, umax[v] - w[v], umin[v] - w[v], ev2));
+                    }
+                }
+            }
+            if (o.recon == RECON_ETA) {
+                phi[0] = tmin(phi[0], phi[1]);
+                phi[2] = tmin(phi[2], phi[1]);
+                phi[3] = tmin(phi[3], phi[1]);
+            }
+            T* sp = Ss + (size_t)ls * 8;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                T pv = clip01(phi[v]);
+                sp[2 * v] = pv * gx[v];
+                sp[2 * v + 1] = pv * gy[v];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ── phase 3: fluxes of the own cells ─────────────────────────────
+    int bad = 0;
+    double loc_dt = INFINITY;
+    const int ls = tid;
+    if (ls < nown) {
+        const int gi = lids[ls];
+        T q[4];
+        lds_q(Qs, ML, ls, q);
+        const T hic = hinv_of(q[1], eps);
+        T sc[8];
+        if (use_slopes) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sc[i] = Ss[(size_t)ls * 8 + i];
+        }
+        Geo G;
+        geo_from_lds(m, Vs, tv, MS, ls, Cs[ls], G);
+        T acc[4] = {0, 0, 0, 0};
+        for (int k = 0; k < m.mf; ++k) {
+            const T nx = T(G.nx[k]), ny = T(G.ny[k]);
+            T qL[4], qR[4];
+            if (use_slopes) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
+            else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
+            const uint16_t nl = tn[k * MS + ls];
+            T hiR = hic;
+            if (nl != LNONE) {
+                T qn[4];
+                lds_q(Qs, ML, nl, qn);
+                hiR = hinv_of(qn[1], eps);
+                if (use_slopes) {
+                    T sn[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) sn[i] = Ss[(size_t)nl * 8 + i];
+                    const double2 cn = Cs[nl];
+                    recon_face(o.recon, qn, hiR, sn, T(G.fx[k] - cn.x), T(G.fy[k] - cn.y), qR);
+                } else {
+                    qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
+                }
+            } else {
+                const int bf = -1 - m.nbr[k * N + gi];
+                const int kind = m.bf_kind[bf];
+                if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qR);
+                else bc_ghost(kind, qL, nx, ny, qR);
+            }
+            T f[4];
+            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, f);
+            const T L = T(G.len[k]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] += f[v] * L;
+            if (use_slopes) {
+                T ce[4];
+                cell_edge(q, qL, nx, ny, g, ce);
+                acc[2] += ce[2] * L;
+                acc[3] += ce[3] * L;
+            }
+        }
+        const T ivol = T(1.0 / G.vol);
+        T out[4];
+        if (mode == 2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[v] = -acc[v] * ivol;
+        } else {
+            const T dt = T(*dtp);
+            if (mode == 1) {
+                T q0[4];
+                load_q(Q0, N, gi, q0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) out[v] = T(0.5) * (q0[v] + (q[v] - dt * (acc[v] * ivol)));
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) out[v] = q[v] - dt * (acc[v] * ivol);
+            }
+            if (final_stage && o.manning > 0.0) {
+                const T h0 = mode == 1 ? Q0[N + gi] : q[1];
+                const T hs = hinv_of(h0, eps);
+                const T u = out[2] * hs, vv = out[3] * hs;
+                const T sp = sqrt(u * u + vv * vv);
+                const T kk = T(o.g * o.manning * o.manning) * cbrt(hs);
+                out[2] += dt * (-kk * u * sp);
+                out[3] += dt * (-kk * vv * sp);
+            }
+        }
+        if (write_b) Qout[gi] = out[0];
+        Qout[N + gi] = out[1];
+        Qout[2 * N + gi] = out[2];
+        Qout[3 * N + gi] = out[3];
+        if (final_stage && mode != 2) {
+            bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
+                  !isfinite(out[2]) || !isfinite(out[3]);
+            if (dt_red) {
+                const T hn = hinv_of(out[1], eps);
+                const double rc = Rs[ls];
+                for (int k = 0; k < m.mf; ++k) {
+                    const uint16_t nl = tn[k * MS + ls];
+                    const double rf = nl != LNONE ? fmin(rc, Rs[nl]) : rc;
+                    const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
+                    double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
+                    if (l != l) l = 0.0;
+                    loc_dt = fmin(loc_dt, l);
+                }
+            }
+        }
+    }
+    if (final_stage && mode != 2) {
+        if (troubled) {
+            int sb = block_reduce(bad, [](int a, int b) { return a + b; }, lds_i);
+            if (threadIdx.x == 0 && sb) atomicAdd(troubled, sb);
+        }
+        if (dt_red) {
+            double sd = block_reduce(loc_dt, [](double a, double b) { return fmin(a, b); }, lds_d);
+            if (threadIdx.x == 0)
+                atomicMin(dt_red, (unsigned long long)__double_as_longlong(sd));
+        }
+    }
+}
+
 }  // namespace
 
 // ═════════════════════════════════════════════════════════════════════════
@@ -570,8 +942,16 @@ __global__ __launch_bounds__(BLOCK) void k_imex(DevMesh m, Opts o, const double*
 // ═════════════════════════════════════════════════════════════════════════
 struct zj_mesh {
     DevMesh d;
+    DevPatch pt;
+    int n_templates = 0;
     std::vector<void*> allocs;
 };
+
+static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
+    return (size_t)pt.ML * (16 + 8) + (size_t)pt.MV * 16 + (size_t)pt.ML * 4 * esz +
+           (size_t)pt.MS * 8 * esz;
+}
+static constexpr size_t LDS_LIMIT = 64 * 1024;
 
 struct zj_solver {
     zj_mesh* m;
@@ -658,6 +1038,65 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         zj_mesh_destroy(m);
         return ZJ_ENOMEM;
     }
+    // patch decomposition for the LDS-staged stage kernel: the largest patch
+    // whose FP64 working set fits the LDS budget
+    {
+        std::vector<int32_t> ex_dummy(N, -1);
+        const int32_t* exh = mx ? d->cell_extra : ex_dummy.data();
+        zj::PatchHost ph;
+        int P = BLOCK;
+        for (;;) {
+            if (zj::build_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_vertices, d->cell_nbr,
+                                  exh, P, ph)) {
+                zj_mesh_destroy(m);
+                return ZJ_EINVAL;
+            }
+            DevPatch probe{};
+            probe.ML = ph.max_load; probe.MS = ph.max_slope; probe.MV = ph.max_vert;
+            if (stage_lds_bytes(probe, 8) <= LDS_LIMIT || P <= 32) break;
+            P /= 2;
+        }
+        if (zj::validate_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_nbr, exh,
+                                 d->cell_vertices, ph)) {
+            zj_mesh_destroy(m);
+            return ZJ_EINVAL;
+        }
+        DevPatch& pt = m->pt;
+        pt.P = ph.P; pt.npatch = ph.npatch; pt.ML = ph.max_load; pt.MS = ph.max_slope;
+        pt.MV = ph.max_vert;
+        if (stage_lds_bytes(pt, 8) > LDS_LIMIT) {
+            zj_mesh_destroy(m);
+            return ZJ_EUNSUP;
+        }
+        std::vector<double> cen2((size_t)2 * N);
+        for (int c = 0; c < N; ++c) {
+            cen2[2 * (size_t)c] = d->cell_center[c];
+            cen2[2 * (size_t)c + 1] = d->cell_center[N + c];
+        }
+        rc = 0;
+        rc |= dev_upload(m, ph.tpl_id.data(), sizeof(int) * ph.npatch, &p); pt.tpl_id = (const int*)p;
+        rc |= dev_upload(m, ph.n_load.data(), sizeof(int) * ph.npatch, &p); pt.n_load = (const int*)p;
+        rc |= dev_upload(m, ph.n_vert.data(), sizeof(int) * ph.npatch, &p); pt.n_vert = (const int*)p;
+        rc |= dev_upload(m, ph.load_ids.data(), sizeof(int) * ph.load_ids.size(), &p); pt.load_ids = (const int*)p;
+        rc |= dev_upload(m, ph.vert_ids.data(), sizeof(int) * ph.vert_ids.size(), &p); pt.vert_ids = (const int*)p;
+        rc |= dev_upload(m, ph.tpl_nbr.data(), 2 * ph.tpl_nbr.size(), &p); pt.tpl_nbr = (const uint16_t*)p;
+        rc |= dev_upload(m, ph.tpl_extra.data(), 2 * ph.tpl_extra.size(), &p); pt.tpl_extra = (const uint16_t*)p;
+        rc |= dev_upload(m, ph.tpl_vert.data(), 2 * ph.tpl_vert.size(), &p); pt.tpl_vert = (const uint16_t*)p;
+        rc |= dev_upload(m, ph.tpl_nslope.data(), sizeof(int) * ph.ntpl, &p); pt.tpl_nslope = (const int*)p;
+        rc |= dev_upload(m, ph.tpl_nown.data(), sizeof(int) * ph.ntpl, &p); pt.tpl_nown = (const int*)p;
+        rc |= dev_upload(m, cen2.data(), sizeof(double) * cen2.size(), &p); pt.cen2 = (const double2*)p;
+        // the kernels index m.mx stencil rows: keep at least one (all -1)
+        if (mx == 0) {
+            rc |= dev_upload(m, ex_dummy.data(), sizeof(int) * N, &p);
+            m->d.extra = (const int*)p;
+            m->d.mx = 1;
+        }
+        if (rc) {
+            zj_mesh_destroy(m);
+            return ZJ_ENOMEM;
+        }
+        m->n_templates = ph.ntpl;
+    }
     *out = m;
     return ZJ_OK;
 }
@@ -714,54 +1153,45 @@ int zj_solver_destroy(zj_solver* s) {
 namespace {
 
 template <typename T>
+void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double* dt, T* aux,
+                  int mode, int final_stage, int write_b, int use_slopes, int* troubled,
+                  unsigned long long* dt_red, hipStream_t st) {
+    const zj_mesh* M = s->m;
+    const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
+    hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(BLOCK), lds, st, M->d, M->pt, s->o,
+                       Qin, Q0, Qout, dt, aux, mode, final_stage, write_b, use_slopes, troubled,
+                       dt_red);
+}
+
+template <typename T>
 int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled,
               unsigned long long* dt_red, hipStream_t st, T* aux_of_Q = nullptr) {
     const DevMesh& m = s->m->d;
-    const int nb = nblocks(m.N);
-    T* S = (T*)s->S;
     T* Q1 = (T*)s->Q1;
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
-    if (s->o.order < 2) {
-        if (Qout == Q) {
-            if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
-            hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)nullptr,
-                               (const T*)nullptr, Q1, dt, 0, 1, write_b, troubled, dt_red);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(Qout, Q1, sizeof(T) * 4 * m.N, hipMemcpyDeviceToDevice, st));
-            return ZJ_OK;
-        }
-        if (!write_b && Qout != Q)
-            HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)nullptr,
-                           (const T*)nullptr, Qout, dt, 0, 1, write_b, troubled, dt_red);
-        HIPCHK(hipGetLastError());
-        return ZJ_OK;
-    }
     if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
     if (!write_b && Qout != Q)
         HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, aux_of_Q);
-    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, (const T*)S,
-                       (const T*)nullptr, Q1, dt, 0, 0, write_b, (int*)nullptr,
-                       (unsigned long long*)nullptr);
-    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const T*)Q1, S, (T*)nullptr);
-    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const T*)Q1, (const T*)S,
-                       Q, Qout, dt, 1, 1, write_b, troubled, dt_red);
+    if (s->o.order < 2) {
+        launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, (T*)nullptr, 0, 1, write_b, 0, troubled,
+                        dt_red, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(Qout, Q1, sizeof(T) * 4 * m.N, hipMemcpyDeviceToDevice, st));
+        return ZJ_OK;
+    }
+    const bool fuse_aux = aux_of_Q && s->o.recon == RECON_CONS;
+    launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, fuse_aux ? aux_of_Q : (T*)nullptr, 0, 0,
+                    write_b, 1, (int*)nullptr, (unsigned long long*)nullptr, st);
+    launch_stage<T>(s, (const T*)Q1, Q, Qout, dt, (T*)nullptr, 1, 1, write_b, 1, troubled, dt_red,
+                    st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
 
 template <typename T>
 int flux_impl(zj_solver* s, const T* Q, T* dQ, hipStream_t st) {
-    const DevMesh& m = s->m->d;
-    const int nb = nblocks(m.N);
-    T* S = (T*)s->S;
-    if (s->o.order >= 2)
-        hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, S, (T*)nullptr);
-    hipLaunchKernelGGL(k_flux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q,
-                       s->o.order >= 2 ? (const T*)S : (const T*)nullptr, (const T*)nullptr, dQ,
-                       (const double*)nullptr, 2, 0, 1, (int*)nullptr,
-                       (unsigned long long*)nullptr);
+    launch_stage<T>(s, Q, (const T*)nullptr, dQ, (const double*)nullptr, (T*)nullptr, 2, 0, 1,
+                    s->o.order >= 2 ? 1 : 0, (int*)nullptr, (unsigned long long*)nullptr, st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -870,67 +1300,49 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
 }
 
 // Average duration (ms) of each kernel of a step, timed with HIP events on
-// the given stream over `reps` launches each: ms_out[0] slopes, [1] flux
-// stage 1, [2] flux stage 2 (final: update + source + troubled + dt), [3] dt,
-// [4] aux refresh.  Q is read only; the outputs go to solver scratch.
-int zj_time_kernels(zj_solver* s, const void* Q, const double* dt, int32_t reps,
-                    float* ms_out, void* stream) {
-    if (!s || !Q || !dt || !ms_out || reps <= 0) return ZJ_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
+// the given stream over `reps` launches each: ms_out[0] stage 1 (slopes +
+// fluxes -> stage state), [1] stage 2 (slopes + fluxes + RK average + source
+// + troubled + dt of the new state), [2] standalone dt, [3] standalone aux
+// refresh, [4] stage 1 with the fused aux refresh.  Q is read only; the
+// outputs go to solver scratch.
+}  // extern "C"
+
+template <typename T>
+static int time_kernels_impl(zj_solver* s, const T* q, const double* dt, int reps, float* ms_out,
+                             hipStream_t st) {
     const DevMesh& m = s->m->d;
     const int nb = nblocks(m.N);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
-    void* out = nullptr;
-    void* aux = nullptr;
-    if (hipMalloc(&out, esz * 4 * m.N) != hipSuccess) return ZJ_ENOMEM;
-    if (hipMalloc(&aux, esz * (3 * m.dim + 1) * m.N) != hipSuccess) {
+    T* out = nullptr;
+    T* aux = nullptr;
+    if (hipMalloc((void**)&out, sizeof(T) * 4 * m.N) != hipSuccess) return ZJ_ENOMEM;
+    if (hipMalloc((void**)&aux, sizeof(T) * (3 * m.dim + 1) * m.N) != hipSuccess) {
         (void)hipFree(out);
         return ZJ_ENOMEM;
     }
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
+    const int slopes = s->o.order >= 2 ? 1 : 0;
     for (int which = 0; which < 5; ++which) {
         for (int pass = 0; pass < 2; ++pass) {  // pass 0 warms up
             const int n = pass ? reps : 2;
             if (pass) HIPCHK(hipEventRecord(e0, st));
             for (int r = 0; r < n; ++r) {
-                if (s->p.dtype == ZJ_F64) {
-                    const double* q = (const double*)Q;
-                    double* S = (double*)s->S;
-                    if (which == 0)
-                        hipLaunchKernelGGL(k_slopes<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, S, (double*)nullptr);
-                    else if (which == 1)
-                        hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const double*)S,
-                                           (const double*)nullptr, (double*)out, dt, 0, 0, write_b, (int*)nullptr,
-                                           (unsigned long long*)nullptr);
-                    else if (which == 2)
-                        hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const double*)S,
-                                           q, (double*)out, dt, 1, 1, write_b, s->troubled, s->dt_red);
-                    else if (which == 3)
-                        hipLaunchKernelGGL(k_dt<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
-                    else
-                        hipLaunchKernelGGL(k_slopes<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (double*)nullptr,
-                                           (double*)aux);
-                } else {
-                    const float* q = (const float*)Q;
-                    float* S = (float*)s->S;
-                    if (which == 0)
-                        hipLaunchKernelGGL(k_slopes<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, S, (float*)nullptr);
-                    else if (which == 1)
-                        hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const float*)S,
-                                           (const float*)nullptr, (float*)out, dt, 0, 0, write_b, (int*)nullptr,
-                                           (unsigned long long*)nullptr);
-                    else if (which == 2)
-                        hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (const float*)S,
-                                           q, (float*)out, dt, 1, 1, write_b, s->troubled, s->dt_red);
-                    else if (which == 3)
-                        hipLaunchKernelGGL(k_dt<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
-                    else
-                        hipLaunchKernelGGL(k_slopes<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, (float*)nullptr,
-                                           (float*)aux);
-                }
+                if (which == 0)
+                    launch_stage<T>(s, q, (const T*)nullptr, out, dt, (T*)nullptr, 0, 0, write_b,
+                                    slopes, (int*)nullptr, (unsigned long long*)nullptr, st);
+                else if (which == 1)
+                    launch_stage<T>(s, q, q, out, dt, (T*)nullptr, 1, 1, write_b, slopes,
+                                    s->troubled, s->dt_red, st);
+                else if (which == 2)
+                    hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
+                else if (which == 3)
+                    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q,
+                                       (T*)nullptr, aux);
+                else
+                    launch_stage<T>(s, q, (const T*)nullptr, out, dt, aux, 0, 0, write_b, slopes,
+                                    (int*)nullptr, (unsigned long long*)nullptr, st);
             }
             if (pass) {
                 HIPCHK(hipEventRecord(e1, st));
@@ -948,6 +1360,17 @@ int zj_time_kernels(zj_solver* s, const void* Q, const double* dt, int32_t reps,
     (void)hipEventDestroy(e1);
     return ZJ_OK;
 }
+
+extern "C" int zj_time_kernels(zj_solver* s, const void* Q, const double* dt, int32_t reps,
+                               float* ms_out, void* stream) {
+    if (!s || !Q || !dt || !ms_out || reps <= 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    return s->p.dtype == ZJ_F64
+               ? time_kernels_impl(s, (const double*)Q, dt, reps, ms_out, st)
+               : time_kernels_impl(s, (const float*)Q, dt, reps, ms_out, st);
+}
+
+extern "C" {
 
 int zj_mass(zj_solver* s, const void* Q, double* mass_out, void* stream) {
     if (!s || !Q || !mass_out) return ZJ_EINVAL;

@@ -73,6 +73,43 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def mesh_desc(mesh: Mesh):
+    """``zj_mesh_desc`` of a host mesh; returns (desc, arrays kept alive)."""
+    N = mesh.n_cells
+    ex = mesh.extra_stencil() if mesh.dim == 2 else \
+        np.full((N, 1), -1, np.int32)
+    keep = dict(
+        cv=np.ascontiguousarray(mesh.cell_vertices.T, np.int32),
+        nbr=np.ascontiguousarray(mesh.cell_nbr.T, np.int32),
+        ex=np.ascontiguousarray(ex.T, np.int32),
+        verts=np.ascontiguousarray(mesh.verts, np.float64),
+        cen=np.ascontiguousarray(mesh.cell_centers().T, np.float64),
+        rin=np.ascontiguousarray(mesh.cell_inradius(), np.float64),
+        bcell=np.ascontiguousarray(mesh.bface_cell, np.int32),
+        bslot=np.ascontiguousarray(mesh.bface_slot, np.int32),
+        bkind=np.ascontiguousarray(mesh.bface_kind(), np.int32),
+        bpart=np.ascontiguousarray(
+            mesh.periodic_partner if mesh.periodic_partner is not None
+            else -np.ones(mesh.n_bfaces), np.int32),
+    )
+    d = L.MeshDesc(mesh.dim, N, len(mesh.verts), mesh.max_faces,
+                   keep["ex"].shape[0], mesh.n_bfaces,
+                   *[keep[n].ctypes.data for n in (
+                       "cv", "nbr", "ex", "verts", "cen", "rin", "bcell",
+                       "bslot", "bkind", "bpart")])
+    return d, keep
+
+
+def patch_stats(mesh: Mesh, P: int = 256):
+    """Host-only build + validation of the LDS patch decomposition."""
+    d, keep = mesh_desc(mesh)
+    st = (ctypes.c_int32 * 6)()
+    rc = L.lib().zj_patch_check(ctypes.byref(d), int(P),
+                                ctypes.cast(st, ctypes.c_void_p))
+    return rc, dict(zip(("P", "npatch", "ntemplates", "max_load",
+                         "max_slope", "max_vert"), list(st)))
+
+
 class DeviceMesh:
     """Mesh uploaded to the device (``zj_mesh_create``)."""
 
@@ -301,16 +338,16 @@ class HyperbolicSolver:
             f"aborts the march")
 
     def time_kernels(self, Q, dt, reps: int = 20):
-        """Average ms per launch of {slopes, flux stage 1, flux stage 2, dt,
-        aux} measured with HIP events on the current stream."""
+        """Average ms per launch of {stage1, stage2, dt, aux, stage1_aux}
+        measured with HIP events on the current stream."""
         Q4 = self._full(Q)
         ms = (ctypes.c_float * 5)()
         L.check(L.lib().zj_time_kernels(self._rt.handle, _p(Q4),
                                         _p(self._dt_dev(dt)), int(reps),
                                         ctypes.cast(ms, ctypes.c_void_p),
                                         _stream()), "zj_time_kernels")
-        return dict(zip(("slopes", "flux_stage1", "flux_stage2", "dt",
-                         "aux"), list(ms)))
+        return dict(zip(("stage1", "stage2", "dt", "aux", "stage1_aux"),
+                        list(ms)))
 
     def compute_mass(self, Q):
         """Compensated FP64 sum of h |cell| (device scalar)."""
