@@ -30,6 +30,7 @@ enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2 };
 
 constexpr int MAXF = 4;      // faces per cell (quad)
 constexpr int BLOCK = 256;   // threads per block (4 wave64)
+constexpr int SBLOCK = 320;  // stage kernel: 5 wave64, one per slope cell of a quad patch
 
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
@@ -162,16 +163,29 @@ __device__ __forceinline__ void cell_edge(const T* qc, const T* qe, T nx, T ny, 
 }
 
 // Block-wide reductions over BLOCK threads (wave64 shuffles + LDS).
-template <typename T, typename Op>
+template <typename T, typename Op, int NT = BLOCK>
 __device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
     for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) lds[w] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int i = 1; i < BLOCK / 64; ++i) v = op(v, lds[i]);
+        for (int i = 1; i < NT / 64; ++i) v = op(v, lds[i]);
     }
     return v;  // valid on thread 0
+}
+
+// Barth-Jespersen / minmod limiter of one variable over all faces with two
+// divisions: min_f (dp / d_f) over d_f > eps equals dp / max_f d_f (and
+// dn / min_f d_f over d_f < -eps), exactly, because correctly rounded
+// division is monotone in the divisor.
+template <typename T>
+__device__ __forceinline__ T bj_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
+    const T eps = T(1e-30);
+    T phi = T(1);
+    if (dmax_pos > eps) phi = tmin(phi, clip01(dp / dmax_pos));
+    if (dmin_neg < -eps) phi = tmin(phi, clip01(dn / dmin_neg));
+    return phi;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs of the chip; remap the

@@ -635,14 +635,14 @@ __device__ __forceinline__ void lds_q(const T* Qs, int ML, int i, T* q) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_stage(
+__global__ __launch_bounds__(SBLOCK) void k_stage(
     DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled,
     unsigned long long* __restrict__ dt_red) {
     extern __shared__ double smem_d[];
-    __shared__ double lds_d[BLOCK / 64];
-    __shared__ int lds_i[BLOCK / 64];
+    __shared__ double lds_d[SBLOCK / 64];
+    __shared__ int lds_i[SBLOCK / 64];
     const int p = xcd_block();
     const int tid = threadIdx.x;
     const int N = m.N;
@@ -655,6 +655,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
     double2* Vs = (double2*)(Rs + ML);
     T* Qs = (T*)(Vs + MV);
     T* Ss = Qs + 4 * ML;
+    T* Hs = Ss + 8 * MS;  // hinv of every staged cell
     const T eps = T(o.eps), g = T(o.g);
     const int* lids = pt.load_ids + (size_t)p * ML;
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
@@ -662,28 +663,33 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
     const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
 
     // ── phase 1: stage cells and vertices ────────────────────────────
-    for (int i = tid; i < nload; i += BLOCK) {
+    for (int i = tid; i < nload; i += SBLOCK) {
         const int gi = lids[i];
         Qs[i] = Qin[gi];
-        Qs[ML + i] = Qin[N + gi];
+        const T h = Qin[N + gi];
+        Qs[ML + i] = h;
         Qs[2 * ML + i] = Qin[2 * N + gi];
         Qs[3 * ML + i] = Qin[3 * N + gi];
+        Hs[i] = hinv_of(h, eps);
         Cs[i] = pt.cen2[gi];
         if (final_stage) Rs[i] = m.rin[gi];
     }
     {
         const int* vids = pt.vert_ids + (size_t)p * MV;
-        for (int i = tid; i < nvert; i += BLOCK) Vs[i] = m.verts[vids[i]];
+        for (int i = tid; i < nvert; i += SBLOCK) Vs[i] = m.verts[vids[i]];
     }
     __syncthreads();
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
     if (use_slopes) {
-        for (int ls = tid; ls < nslope; ls += BLOCK) {
+        for (int ls = tid; ls < nslope; ls += SBLOCK) {
             const int gi = lids[ls];
+            uint16_t nls[MAXF];
+#pragma unroll
+            for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
             T q[4];
             lds_q(Qs, ML, ls, q);
-            const T hic = hinv_of(q[1], eps);
+            const T hic = Hs[ls];
             T w[4];
             to_w(o.recon, q, hic, w);
             Geo G;
@@ -697,11 +703,11 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
 #pragma unroll
             for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
             for (int k = 0; k < m.mf; ++k) {
-                const uint16_t nl = tn[k * MS + ls];
+                const uint16_t nl = nls[k];
                 if (nl != LNONE) {
                     T qn[4], wn[4];
                     lds_q(Qs, ML, nl, qn);
-                    to_w(o.recon, qn, hinv_of(qn[1], eps), wn);
+                    to_w(o.recon, qn, Hs[nl], wn);
                     const double2 cn = Cs[nl];
                     const double rx = cn.x - G.cx, ry = cn.y - G.cy;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
@@ -747,7 +753,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
                 if (el == LNONE) break;
                 T qn[4], wn[4];
                 lds_q(Qs, ML, el, qn);
-                to_w(o.recon, qn, hinv_of(qn[1], eps), wn);
+                to_w(o.recon, qn, Hs[el], wn);
                 const double2 cn = Cs[el];
                 const double rx = cn.x - G.cx, ry = cn.y - G.cy;
                 Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
@@ -793,7 +799,21 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
                 aux_out[(size_t)row * N + gi] = hic;
             }
             T phi[4] = {1, 1, 1, 1};
-            if (o.lim != LIM_NONE) {
+            if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
+                T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
+                for (int k = 0; k < m.mf; ++k) {
+                    const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        T d = gx[v] * rx + gy[v] * ry;
+                        dpos[v] = tmax(dpos[v], d);
+                        dneg[v] = tmin(dneg[v], d);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    phi[v] = bj_phi(dpos[v], dneg[v], umax[v] - w[v], umin[v] - w[v]);
+            } else if (o.lim != LIM_NONE) {
                 const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
                 for (int k = 0; k < m.mf; ++k) {
                     const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
@@ -826,9 +846,12 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
     const int ls = tid;
     if (ls < nown) {
         const int gi = lids[ls];
+        uint16_t nls[MAXF];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
         T q[4];
         lds_q(Qs, ML, ls, q);
-        const T hic = hinv_of(q[1], eps);
+        const T hic = Hs[ls];
         T sc[8];
         if (use_slopes) {
 #pragma unroll
@@ -842,12 +865,12 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
             T qL[4], qR[4];
             if (use_slopes) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
             else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
-            const uint16_t nl = tn[k * MS + ls];
+            const uint16_t nl = nls[k];
             T hiR = hic;
             if (nl != LNONE) {
                 T qn[4];
                 lds_q(Qs, ML, nl, qn);
-                hiR = hinv_of(qn[1], eps);
+                hiR = Hs[nl];
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
@@ -912,7 +935,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
                 const T hn = hinv_of(out[1], eps);
                 const double rc = Rs[ls];
                 for (int k = 0; k < m.mf; ++k) {
-                    const uint16_t nl = tn[k * MS + ls];
+                    const uint16_t nl = nls[k];
                     const double rf = nl != LNONE ? fmin(rc, Rs[nl]) : rc;
                     const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
                     double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
@@ -924,11 +947,13 @@ __global__ __launch_bounds__(BLOCK) void k_stage(
     }
     if (final_stage && mode != 2) {
         if (troubled) {
-            int sb = block_reduce(bad, [](int a, int b) { return a + b; }, lds_i);
+            auto add = [](int a, int b) { return a + b; };
+            int sb = block_reduce<int, decltype(add), SBLOCK>(bad, add, lds_i);
             if (threadIdx.x == 0 && sb) atomicAdd(troubled, sb);
         }
         if (dt_red) {
-            double sd = block_reduce(loc_dt, [](double a, double b) { return fmin(a, b); }, lds_d);
+            auto mn = [](double a, double b) { return fmin(a, b); };
+            double sd = block_reduce<double, decltype(mn), SBLOCK>(loc_dt, mn, lds_d);
             if (threadIdx.x == 0)
                 atomicMin(dt_red, (unsigned long long)__double_as_longlong(sd));
         }
@@ -948,7 +973,7 @@ struct zj_mesh {
 };
 
 static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
-    return (size_t)pt.ML * (16 + 8) + (size_t)pt.MV * 16 + (size_t)pt.ML * 4 * esz +
+    return (size_t)pt.ML * (16 + 8) + (size_t)pt.MV * 16 + (size_t)pt.ML * 5 * esz +
            (size_t)pt.MS * 8 * esz;
 }
 static constexpr size_t LDS_LIMIT = 64 * 1024;
@@ -1158,7 +1183,7 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   unsigned long long* dt_red, hipStream_t st) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
-    hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(BLOCK), lds, st, M->d, M->pt, s->o,
+    hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(SBLOCK), lds, st, M->d, M->pt, s->o,
                        Qin, Q0, Qout, dt, aux, mode, final_stage, write_b, use_slopes, troubled,
                        dt_red);
 }
