@@ -590,48 +590,70 @@ struct DevPatch {
 
 constexpr uint16_t LNONE = 0xFFFF;
 
-__device__ __forceinline__ void geo_from_lds(const DevMesh& m, const double2* Vs,
+// Face centres of a cell's slots from the staged vertices (cheap: no sqrt).
+__device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs,
                                              const uint16_t* tv, int MS, int ls,
-                                             double2 c, Geo& G) {
-    G.cx = c.x;
-    G.cy = c.y;
+                                             double2* v, double* fx, double* fy) {
     if (m.dim == 1) {
-        double2 a = Vs[tv[ls]];
-        double2 b = Vs[tv[MS + ls]];
-        G.vol = b.x - a.x;
-        G.fx[0] = a.x; G.fy[0] = 0.0; G.nx[0] = -1.0; G.ny[0] = 0.0; G.len[0] = 1.0;
-        G.fx[1] = b.x; G.fy[1] = 0.0; G.nx[1] = 1.0; G.ny[1] = 0.0; G.len[1] = 1.0;
+        v[0] = Vs[tv[ls]];
+        v[1] = Vs[tv[MS + ls]];
+        fx[0] = v[0].x; fy[0] = 0.0;
+        fx[1] = v[1].x; fy[1] = 0.0;
         return;
     }
-    double2 v[MAXF];
 #pragma unroll
     for (int k = 0; k < MAXF; ++k)
         if (k < m.mf) v[k] = Vs[tv[k * MS + ls]];
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) {
+        if (k >= m.mf) break;
+        const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+        fx[k] = 0.5 * (v[k].x + b.x);
+        fy[k] = 0.5 * (v[k].y + b.y);
+    }
+}
+
+// Outward unit normal and length of slot k.
+__device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, int k,
+                                            double& nx, double& ny, double& len) {
+    if (m.dim == 1) {
+        nx = k == 0 ? -1.0 : 1.0;
+        ny = 0.0;
+        len = 1.0;
+        return;
+    }
+    const double2 a = v[k];
+    const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+    const double ex = b.x - a.x, ey = b.y - a.y;
+    const double l2 = ex * ex + ey * ey;
+    const double il = rsqrt(l2);
+    len = l2 * il;
+    nx = ey * il;
+    ny = -ex * il;
+}
+
+__device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v) {
+    if (m.dim == 1) return v[1].x - v[0].x;
     double area = 0.0;
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
         if (k >= m.mf) break;
-        double2 a = v[k];
-        double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
-        double ex = b.x - a.x, ey = b.y - a.y;
-        double L = sqrt(ex * ex + ey * ey);
-        double iL = 1.0 / L;
-        G.len[k] = L;
-        G.nx[k] = ey * iL;
-        G.ny[k] = -ex * iL;
-        G.fx[k] = 0.5 * (a.x + b.x);
-        G.fy[k] = 0.5 * (a.y + b.y);
+        const double2 a = v[k];
+        const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
         area += a.x * b.y - b.x * a.y;
     }
-    G.vol = 0.5 * area;
+    return 0.5 * area;
 }
 
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { using type = double4; };
+template <> struct Vec4<float> { using type = float4; };
+
 template <typename T>
-__device__ __forceinline__ void lds_q(const T* Qs, int ML, int i, T* q) {
-    q[0] = Qs[i];
-    q[1] = Qs[ML + i];
-    q[2] = Qs[2 * ML + i];
-    q[3] = Qs[3 * ML + i];
+__device__ __forceinline__ void lds_q4(const T* Qs, int i, T* q) {
+    using V = typename Vec4<T>::type;
+    const V x = *reinterpret_cast<const V*>(Qs + 4 * i);
+    q[0] = x.x; q[1] = x.y; q[2] = x.z; q[3] = x.w;
 }
 
 template <typename T>
@@ -640,9 +662,9 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled,
     unsigned long long* __restrict__ dt_red) {
-    extern __shared__ double smem_d[];
-    __shared__ double lds_d[SBLOCK / 64];
-    __shared__ int lds_i[SBLOCK / 64];
+    // dynamic LDS, 32-byte aligned (the AoS state is read as double4)
+    extern __shared__ double4 smem_v4[];
+    double* smem_d = reinterpret_cast<double*>(smem_v4);
     const int p = xcd_block();
     const int tid = threadIdx.x;
     const int N = m.N;
@@ -650,12 +672,14 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
     const int tpl = pt.tpl_id[p];
     const int nload = pt.n_load[p], nvert = pt.n_vert[p];
     const int nslope = pt.tpl_nslope[tpl], nown = pt.tpl_nown[tpl];
-    double2* Cs = (double2*)smem_d;
-    double* Rs = (double*)(Cs + ML);
-    double2* Vs = (double2*)(Rs + ML);
-    T* Qs = (T*)(Vs + MV);
-    T* Ss = Qs + 4 * ML;
-    T* Hs = Ss + 8 * MS;  // hinv of every staged cell
+    double2* Cs = (double2*)smem_d;         // ML centres
+    double2* Vs = Cs + ML;                  // MV vertices
+    double* Rs = (double*)(Vs + MV);        // MS inradius (final stage)
+    T* Qs = (T*)(Rs + MS);                  // ML x 4 states (AoS)
+    T* Hs = Qs + 4 * ML;                    // ML hinv
+    T* Ss = Hs + ML;                        // MS x 8 limited slopes
+    double* lds_d = reinterpret_cast<double*>(Ss + 8 * MS);  // block reductions
+    int* lds_i = reinterpret_cast<int*>(lds_d + SBLOCK / 64);
     const T eps = T(o.eps), g = T(o.g);
     const int* lids = pt.load_ids + (size_t)p * ML;
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
@@ -665,14 +689,16 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
     // ── phase 1: stage cells and vertices ────────────────────────────
     for (int i = tid; i < nload; i += SBLOCK) {
         const int gi = lids[i];
-        Qs[i] = Qin[gi];
-        const T h = Qin[N + gi];
-        Qs[ML + i] = h;
-        Qs[2 * ML + i] = Qin[2 * N + gi];
-        Qs[3 * ML + i] = Qin[3 * N + gi];
-        Hs[i] = hinv_of(h, eps);
+        using V = typename Vec4<T>::type;
+        V x;
+        x.x = Qin[gi];
+        x.y = Qin[N + gi];
+        x.z = Qin[2 * N + gi];
+        x.w = Qin[3 * N + gi];
+        *reinterpret_cast<V*>(Qs + 4 * i) = x;
+        Hs[i] = hinv_of(x.y, eps);
         Cs[i] = pt.cen2[gi];
-        if (final_stage) Rs[i] = m.rin[gi];
+        if (final_stage && i < nslope) Rs[i] = m.rin[gi];
     }
     {
         const int* vids = pt.vert_ids + (size_t)p * MV;
@@ -688,12 +714,14 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
             T q[4];
-            lds_q(Qs, ML, ls, q);
+            lds_q4(Qs, ls, q);
             const T hic = Hs[ls];
             T w[4];
             to_w(o.recon, q, hic, w);
-            Geo G;
-            geo_from_lds(m, Vs, tv, MS, ls, Cs[ls], G);
+            double2 v[MAXF];
+            double fx[MAXF], fy[MAXF];
+            face_centres(m, Vs, tv, MS, ls, v, fx, fy);
+            const double2 cc = Cs[ls];
             double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
             double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
             T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
@@ -701,48 +729,50 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             T gby[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
             T umax[4], umin[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
+            for (int vv = 0; vv < 4; ++vv) { umax[vv] = w[vv]; umin[vv] = w[vv]; }
             for (int k = 0; k < m.mf; ++k) {
                 const uint16_t nl = nls[k];
                 if (nl != LNONE) {
                     T qn[4], wn[4];
-                    lds_q(Qs, ML, nl, qn);
+                    lds_q4(Qs, nl, qn);
                     to_w(o.recon, qn, Hs[nl], wn);
                     const double2 cn = Cs[nl];
-                    const double rx = cn.x - G.cx, ry = cn.y - G.cy;
+                    const double rx = cn.x - cc.x, ry = cn.y - cc.y;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        T dv = wn[v] - w[v];
-                        bx[v] += T(rx) * dv;
-                        by[v] += T(ry) * dv;
-                        umax[v] = tmax(umax[v], wn[v]);
-                        umin[v] = tmin(umin[v], wn[v]);
+                    for (int vv = 0; vv < 4; ++vv) {
+                        T dv = wn[vv] - w[vv];
+                        bx[vv] += T(rx) * dv;
+                        by[vv] += T(ry) * dv;
+                        umax[vv] = tmax(umax[vv], wn[vv]);
+                        umin[vv] = tmin(umin[vv], wn[vv]);
                     }
                 } else {
+                    double nx, ny, len;
+                    face_normal(m, v, k, nx, ny, len);
                     const int bf = -1 - m.nbr[k * N + gi];
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
                     if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qg);
-                    else bc_ghost(kind, q, T(G.nx[k]), T(G.ny[k]), qg);
+                    else bc_ghost(kind, q, T(nx), T(ny), qg);
                     to_w(o.recon, qg, hinv_of(qg[1], eps), wg);
-                    const double rx = 2.0 * (G.fx[k] - G.cx), ry = 2.0 * (G.fy[k] - G.cy);
+                    const double rx = 2.0 * (fx[k] - cc.x), ry = 2.0 * (fy[k] - cc.y);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        T bv = (kind == BC_PERIODIC) ? wg[v] : T(2) * wg[v] - w[v];
-                        umax[v] = tmax(umax[v], bv);
-                        umin[v] = tmin(umin[v], bv);
+                    for (int vv = 0; vv < 4; ++vv) {
+                        T bv = (kind == BC_PERIODIC) ? wg[vv] : T(2) * wg[vv] - w[vv];
+                        umax[vv] = tmax(umax[vv], bv);
+                        umin[vv] = tmin(umin[vv], bv);
                     }
-                    const double an[2] = {fabs(G.nx[k]), fabs(G.ny[k])};
+                    const double an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
                     for (int d = 0; d < 2; ++d) {
                         if (an[d] > 0.5) {
                             Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry;
 #pragma unroll
-                            for (int v = 0; v < 4; ++v) {
-                                T dv = wg[v] - w[v];
-                                gbx[d][v] += T(rx) * dv;
-                                gby[d][v] += T(ry) * dv;
+                            for (int vv = 0; vv < 4; ++vv) {
+                                T dv = wg[vv] - w[vv];
+                                gbx[d][vv] += T(rx) * dv;
+                                gby[d][vv] += T(ry) * dv;
                             }
                         }
                     }
@@ -752,25 +782,25 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
                 const uint16_t el = te[j * MS + ls];
                 if (el == LNONE) break;
                 T qn[4], wn[4];
-                lds_q(Qs, ML, el, qn);
+                lds_q4(Qs, el, qn);
                 to_w(o.recon, qn, Hs[el], wn);
                 const double2 cn = Cs[el];
-                const double rx = cn.x - G.cx, ry = cn.y - G.cy;
+                const double rx = cn.x - cc.x, ry = cn.y - cc.y;
                 Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    T dv = wn[v] - w[v];
-                    bx[v] += T(rx) * dv;
-                    by[v] += T(ry) * dv;
+                for (int vv = 0; vv < 4; ++vv) {
+                    T dv = wn[vv] - w[vv];
+                    bx[vv] += T(rx) * dv;
+                    by[vv] += T(ry) * dv;
                 }
             }
             T gx[4], gy[4], ax[4], ay[4];
             if (m.dim == 1) {
                 const double inv = 1.0 / (Sxx + Gxx[0]);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    gx[v] = T(inv) * (bx[v] + gbx[0][v]); gy[v] = T(0);
-                    ax[v] = T(inv) * bx[v]; ay[v] = T(0);
+                for (int vv = 0; vv < 4; ++vv) {
+                    gx[vv] = T(inv) * (bx[vv] + gbx[0][vv]); gy[vv] = T(0);
+                    ax[vv] = T(inv) * bx[vv]; ay[vv] = T(0);
                 }
             } else {
                 const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
@@ -780,11 +810,11 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
                 const T c0 = T(ayy * di0), c1 = T(-axy * di0);
                 const T c2 = T(-bxy * di1), c3 = T(bxx * di1);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    gx[v] = c0 * (bx[v] + gbx[0][v]) + c1 * (by[v] + gby[0][v]);
-                    gy[v] = c2 * (bx[v] + gbx[1][v]) + c3 * (by[v] + gby[1][v]);
-                    ax[v] = c0 * bx[v] + c1 * by[v];
-                    ay[v] = c2 * bx[v] + c3 * by[v];
+                for (int vv = 0; vv < 4; ++vv) {
+                    gx[vv] = c0 * (bx[vv] + gbx[0][vv]) + c1 * (by[vv] + gby[0][vv]);
+                    gy[vv] = c2 * (bx[vv] + gbx[1][vv]) + c3 * (by[vv] + gby[1][vv]);
+                    ax[vv] = c0 * bx[vv] + c1 * by[vv];
+                    ay[vv] = c2 * bx[vv] + c3 * by[vv];
                 }
             }
             if (aux_out && ls < nown) {
@@ -802,25 +832,26 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
                 for (int k = 0; k < m.mf; ++k) {
-                    const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
+                    const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        T d = gx[v] * rx + gy[v] * ry;
-                        dpos[v] = tmax(dpos[v], d);
-                        dneg[v] = tmin(dneg[v], d);
+                    for (int vv = 0; vv < 4; ++vv) {
+                        T d = gx[vv] * rx + gy[vv] * ry;
+                        dpos[vv] = tmax(dpos[vv], d);
+                        dneg[vv] = tmin(dneg[vv], d);
                     }
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    phi[v] = bj_phi(dpos[v], dneg[v], umax[v] - w[v], umin[v] - w[v]);
+                for (int vv = 0; vv < 4; ++vv)
+                    phi[vv] = bj_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv]);
             } else if (o.lim != LIM_NONE) {
-                const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
+                const double vol = cell_volume(m, v);
+                const T ev2 = T(m.dim == 1 ? vol * vol : fabs(vol));
                 for (int k = 0; k < m.mf; ++k) {
-                    const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
+                    const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        T d = gx[v] * rx + gy[v] * ry;
-                        phi[v] = tmin(phi[v], lim_cand(o.lim, d, umax[v] - w[v], umin[v] - w[v], ev2));
+                    for (int vv = 0; vv < 4; ++vv) {
+                        T d = gx[vv] * rx + gy[vv] * ry;
+                        phi[vv] = tmin(phi[vv], lim_cand(o.lim, d, umax[vv] - w[vv], umin[vv] - w[vv], ev2));
                     }
                 }
             }
@@ -831,10 +862,10 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             }
             T* sp = Ss + (size_t)ls * 8;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                T pv = clip01(phi[v]);
-                sp[2 * v] = pv * gx[v];
-                sp[2 * v + 1] = pv * gy[v];
+            for (int vv = 0; vv < 4; ++vv) {
+                T pv = clip01(phi[vv]);
+                sp[2 * vv] = pv * gx[vv];
+                sp[2 * vv + 1] = pv * gy[vv];
             }
         }
         __syncthreads();
@@ -842,7 +873,7 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
 
     // ── phase 3: fluxes of the own cells ─────────────────────────────
     int bad = 0;
-    double loc_dt = INFINITY;
+    double best_r = INFINITY, best_a = 1.0;  // min over faces of r / a
     const int ls = tid;
     if (ls < nown) {
         const int gi = lids[ls];
@@ -850,33 +881,40 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
         T q[4];
-        lds_q(Qs, ML, ls, q);
+        lds_q4(Qs, ls, q);
         const T hic = Hs[ls];
         T sc[8];
         if (use_slopes) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) sc[i] = Ss[(size_t)ls * 8 + i];
         }
-        Geo G;
-        geo_from_lds(m, Vs, tv, MS, ls, Cs[ls], G);
+        double2 v[MAXF];
+        double fx[MAXF], fy[MAXF];
+        face_centres(m, Vs, tv, MS, ls, v, fx, fy);
+        const double2 cc = Cs[ls];
+        double nxs[MAXF], nys[MAXF];
         T acc[4] = {0, 0, 0, 0};
         for (int k = 0; k < m.mf; ++k) {
-            const T nx = T(G.nx[k]), ny = T(G.ny[k]);
+            double nxd, nyd, len;
+            face_normal(m, v, k, nxd, nyd, len);
+            nxs[k] = nxd;
+            nys[k] = nyd;
+            const T nx = T(nxd), ny = T(nyd);
             T qL[4], qR[4];
-            if (use_slopes) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
+            if (use_slopes) recon_face(o.recon, q, hic, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qL);
             else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
             const uint16_t nl = nls[k];
             T hiR = hic;
             if (nl != LNONE) {
                 T qn[4];
-                lds_q(Qs, ML, nl, qn);
+                lds_q4(Qs, nl, qn);
                 hiR = Hs[nl];
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
                     for (int i = 0; i < 8; ++i) sn[i] = Ss[(size_t)nl * 8 + i];
                     const double2 cn = Cs[nl];
-                    recon_face(o.recon, qn, hiR, sn, T(G.fx[k] - cn.x), T(G.fy[k] - cn.y), qR);
+                    recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
                 } else {
                     qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
                 }
@@ -888,9 +926,9 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             }
             T f[4];
             face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, f);
-            const T L = T(G.len[k]);
+            const T L = T(len);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] += f[v] * L;
+            for (int vv = 0; vv < 4; ++vv) acc[vv] += f[vv] * L;
             if (use_slopes) {
                 T ce[4];
                 cell_edge(q, qL, nx, ny, g, ce);
@@ -898,30 +936,30 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
                 acc[3] += ce[3] * L;
             }
         }
-        const T ivol = T(1.0 / G.vol);
+        const T ivol = T(1.0 / cell_volume(m, v));
         T out[4];
         if (mode == 2) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = -acc[v] * ivol;
+            for (int vv = 0; vv < 4; ++vv) out[vv] = -acc[vv] * ivol;
         } else {
             const T dt = T(*dtp);
             if (mode == 1) {
                 T q0[4];
                 load_q(Q0, N, gi, q0);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) out[v] = T(0.5) * (q0[v] + (q[v] - dt * (acc[v] * ivol)));
+                for (int vv = 0; vv < 4; ++vv) out[vv] = T(0.5) * (q0[vv] + (q[vv] - dt * (acc[vv] * ivol)));
             } else {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) out[v] = q[v] - dt * (acc[v] * ivol);
+                for (int vv = 0; vv < 4; ++vv) out[vv] = q[vv] - dt * (acc[vv] * ivol);
             }
             if (final_stage && o.manning > 0.0) {
                 const T h0 = mode == 1 ? Q0[N + gi] : q[1];
                 const T hs = hinv_of(h0, eps);
-                const T u = out[2] * hs, vv = out[3] * hs;
-                const T sp = sqrt(u * u + vv * vv);
+                const T u = out[2] * hs, vv2 = out[3] * hs;
+                const T sp = sqrt(u * u + vv2 * vv2);
                 const T kk = T(o.g * o.manning * o.manning) * cbrt(hs);
                 out[2] += dt * (-kk * u * sp);
-                out[3] += dt * (-kk * vv * sp);
+                out[3] += dt * (-kk * vv2 * sp);
             }
         }
         if (write_b) Qout[gi] = out[0];
@@ -932,15 +970,17 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
                   !isfinite(out[2]) || !isfinite(out[3]);
             if (dt_red) {
+                // min over faces of r_f / a_f without a division per face
                 const T hn = hinv_of(out[1], eps);
+                const double c = sqrt(o.g * fmax((double)out[1], 0.0));
                 const double rc = Rs[ls];
                 for (int k = 0; k < m.mf; ++k) {
                     const uint16_t nl = nls[k];
                     const double rf = nl != LNONE ? fmin(rc, Rs[nl]) : rc;
-                    const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
-                    double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
-                    if (l != l) l = 0.0;
-                    loc_dt = fmin(loc_dt, l);
+                    const double un = (double)((out[2] * T(nxs[k]) + out[3] * T(nys[k])) * hn);
+                    const double a = fabs(un) + c;
+                    if (!(a == a)) { best_r = 0.0; best_a = 1.0; }
+                    else if (rf * best_a < best_r * a) { best_r = rf; best_a = a; }
                 }
             }
         }
@@ -952,6 +992,9 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             if (threadIdx.x == 0 && sb) atomicAdd(troubled, sb);
         }
         if (dt_red) {
+            double loc_dt = best_r == INFINITY ? INFINITY
+                                               : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
+            if (loc_dt != loc_dt) loc_dt = 0.0;
             auto mn = [](double a, double b) { return fmin(a, b); };
             double sd = block_reduce<double, decltype(mn), SBLOCK>(loc_dt, mn, lds_d);
             if (threadIdx.x == 0)
@@ -973,8 +1016,8 @@ struct zj_mesh {
 };
 
 static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
-    return (size_t)pt.ML * (16 + 8) + (size_t)pt.MV * 16 + (size_t)pt.ML * 5 * esz +
-           (size_t)pt.MS * 8 * esz;
+    return (size_t)pt.ML * 16 + (size_t)pt.MV * 16 + (size_t)pt.MS * 8 +
+           (size_t)pt.ML * 5 * esz + (size_t)pt.MS * 8 * esz + (SBLOCK / 64) * 12;
 }
 static constexpr size_t LDS_LIMIT = 64 * 1024;
 
