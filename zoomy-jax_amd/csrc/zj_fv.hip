@@ -28,6 +28,11 @@
 
 using namespace zj;
 
+// Per-face loops are unrolled over MAXF with an early exit so that per-face
+// private arrays stay in registers (a runtime trip count would put them in
+// scratch memory).
+#define ZJ_FOR_FACES(k) _Pragma("unroll") for (int k = 0; k < MAXF; ++k) if (k < m.mf)
+
 namespace {
 
 struct DevMesh {
@@ -53,37 +58,41 @@ struct Geo {
 };
 
 __device__ __forceinline__ void load_geo(const DevMesh& m, int c, Geo& G) {
+    // every field is written on both branches (keeps G in registers)
     const int N = m.N;
     G.cx = m.cen[c];
     G.cy = m.cen[N + c];
-    if (m.dim == 1) {
-        double2 a = m.verts[m.cv[c]];
-        double2 b = m.verts[m.cv[N + c]];
-        G.vol = b.x - a.x;
-        G.fx[0] = a.x; G.fy[0] = 0.0; G.nx[0] = -1.0; G.ny[0] = 0.0; G.len[0] = 1.0;
-        G.fx[1] = b.x; G.fy[1] = 0.0; G.nx[1] = 1.0; G.ny[1] = 0.0; G.len[1] = 1.0;
-        return;
-    }
     double2 v[MAXF];
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k)
-        if (k < m.mf) v[k] = m.verts[m.cv[k * N + c]];
-    double area = 0.0;
+    for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? m.verts[m.cv[k * N + c]] : make_double2(0.0, 0.0);
+    if (m.dim == 1) {
+        G.vol = v[1].x - v[0].x;
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k) {
-        if (k >= m.mf) break;
-        double2 a = v[k];
-        double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
-        double ex = b.x - a.x, ey = b.y - a.y;
-        double L = sqrt(ex * ex + ey * ey);
-        G.len[k] = L;
-        G.nx[k] = ey / L;
-        G.ny[k] = -ex / L;
-        G.fx[k] = 0.5 * (a.x + b.x);
-        G.fy[k] = 0.5 * (a.y + b.y);
-        area += a.x * b.y - b.x * a.y;
+        for (int k = 0; k < MAXF; ++k) {
+            G.fx[k] = v[k].x;
+            G.fy[k] = 0.0;
+            G.nx[k] = k == 0 ? -1.0 : 1.0;
+            G.ny[k] = 0.0;
+            G.len[k] = 1.0;
+        }
+    } else {
+        double area = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            const double2 a = v[k];
+            const double2 b = (k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0];
+            const double ex = b.x - a.x, ey = b.y - a.y;
+            const double L = sqrt(ex * ex + ey * ey);
+            const bool on = k < m.mf;
+            G.len[k] = on ? L : 0.0;
+            G.nx[k] = on ? ey / L : 0.0;
+            G.ny[k] = on ? -ex / L : 0.0;
+            G.fx[k] = 0.5 * (a.x + b.x);
+            G.fy[k] = 0.5 * (a.y + b.y);
+            area += on ? a.x * b.y - b.x * a.y : 0.0;
+        }
+        G.vol = 0.5 * area;
     }
-    G.vol = 0.5 * area;
 }
 
 template <typename T>
@@ -163,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
 #pragma unroll
     for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
 
-    for (int k = 0; k < m.mf; ++k) {
+    ZJ_FOR_FACES(k) {
         const int nb = m.nbr[k * N + c];
         if (nb >= 0) {
             T qn[4], wn[4];
@@ -282,7 +291,7 @@ __global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
     T phi[4] = {1, 1, 1, 1};
     if (o.lim != LIM_NONE) {
         const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
-        for (int k = 0; k < m.mf; ++k) {
+        ZJ_FOR_FACES(k) {
             const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -337,7 +346,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(
         Geo G;
         load_geo(m, c, G);
         T acc[4] = {0, 0, 0, 0};
-        for (int k = 0; k < m.mf; ++k) {
+        ZJ_FOR_FACES(k) {
             const T nx = T(G.nx[k]), ny = T(G.ny[k]);
             T qL[4], qR[4];
             if (S) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
@@ -413,7 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(
             if (dt_red) {
                 const T hn = hinv_of(out[1], eps);
                 const double rc = m.rin[c];
-                for (int k = 0; k < m.mf; ++k) {
+                ZJ_FOR_FACES(k) {
                     const int nb = m.nbr[k * N + c];
                     const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
                     const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
@@ -453,7 +462,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __rest
         Geo G;
         load_geo(m, c, G);
         const double rc = m.rin[c];
-        for (int k = 0; k < m.mf; ++k) {
+        ZJ_FOR_FACES(k) {
             const int nb = m.nbr[k * N + c];
             const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
             const double a = (double)max_abs_eig(q, hn, T(G.nx[k]), T(G.ny[k]), T(o.g));
@@ -481,6 +490,17 @@ __global__ void k_dt_finalize(unsigned long long* red, double* dt, const double*
 }
 
 __global__ void k_t_advance(double* t, const double* dt) { *t += *dt; }
+
+// Folds the per-wave dt minima of a stage launch into the reduction scalar.
+__global__ __launch_bounds__(1024) void k_reduce_part(const double* __restrict__ part, int n,
+                                                      unsigned long long* __restrict__ dt_red) {
+    __shared__ double lds[1024 / 64];
+    double v = INFINITY;
+    for (int i = threadIdx.x; i < n; i += 1024) v = fmin(v, part[i]);
+    auto mn = [](double a, double b) { return fmin(a, b); };
+    v = block_reduce<double, decltype(mn), 1024>(v, mn, lds);
+    if (threadIdx.x == 0) atomicMin(dt_red, (unsigned long long)__double_as_longlong(v));
+}
 
 // Compensated (Neumaier) mass: per-thread partials, per-block (sum, comp)
 // pairs, then a single ordered pass -- deterministic for a fixed grid.
@@ -594,22 +614,21 @@ constexpr uint16_t LNONE = 0xFFFF;
 __device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs,
                                              const uint16_t* tv, int MS, int ls,
                                              double2* v, double* fx, double* fy) {
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? Vs[tv[k * MS + ls]] : make_double2(0.0, 0.0);
     if (m.dim == 1) {
-        v[0] = Vs[tv[ls]];
-        v[1] = Vs[tv[MS + ls]];
-        fx[0] = v[0].x; fy[0] = 0.0;
-        fx[1] = v[1].x; fy[1] = 0.0;
-        return;
-    }
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k)
-        if (k < m.mf) v[k] = Vs[tv[k * MS + ls]];
+        for (int k = 0; k < MAXF; ++k) {
+            fx[k] = v[k].x;
+            fy[k] = 0.0;
+        }
+    } else {
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k) {
-        if (k >= m.mf) break;
-        const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
-        fx[k] = 0.5 * (v[k].x + b.x);
-        fy[k] = 0.5 * (v[k].y + b.y);
+        for (int k = 0; k < MAXF; ++k) {
+            const double2 b = (k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0];
+            fx[k] = 0.5 * (v[k].x + b.x);
+            fy[k] = 0.5 * (v[k].y + b.y);
+        }
     }
 }
 
@@ -623,7 +642,7 @@ __device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, 
         return;
     }
     const double2 a = v[k];
-    const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+    const double2 b = ((k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0]);
     const double ex = b.x - a.x, ey = b.y - a.y;
     const double l2 = ex * ex + ey * ey;
     const double il = rsqrt(l2);
@@ -637,9 +656,9 @@ __device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v
     double area = 0.0;
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) {
-        if (k >= m.mf) break;
+        if (k >= m.mf) continue;
         const double2 a = v[k];
-        const double2 b = v[(k + 1 == m.mf) ? 0 : k + 1];
+        const double2 b = ((k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0]);
         area += a.x * b.y - b.x * a.y;
     }
     return 0.5 * area;
@@ -657,11 +676,10 @@ __device__ __forceinline__ void lds_q4(const T* Qs, int i, T* q) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(SBLOCK) void k_stage(
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_stage(
     DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
-    int write_b, int use_slopes, int* __restrict__ troubled,
-    unsigned long long* __restrict__ dt_red) {
+    int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part) {
     // dynamic LDS, 32-byte aligned (the AoS state is read as double4)
     extern __shared__ double4 smem_v4[];
     double* smem_d = reinterpret_cast<double*>(smem_v4);
@@ -678,8 +696,6 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
     T* Qs = (T*)(Rs + MS);                  // ML x 4 states (AoS)
     T* Hs = Qs + 4 * ML;                    // ML hinv
     T* Ss = Hs + ML;                        // MS x 8 limited slopes
-    double* lds_d = reinterpret_cast<double*>(Ss + 8 * MS);  // block reductions
-    int* lds_i = reinterpret_cast<int*>(lds_d + SBLOCK / 64);
     const T eps = T(o.eps), g = T(o.g);
     const int* lids = pt.load_ids + (size_t)p * ML;
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
@@ -730,7 +746,7 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             T umax[4], umin[4];
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) { umax[vv] = w[vv]; umin[vv] = w[vv]; }
-            for (int k = 0; k < m.mf; ++k) {
+            ZJ_FOR_FACES(k) {
                 const uint16_t nl = nls[k];
                 if (nl != LNONE) {
                     T qn[4], wn[4];
@@ -831,7 +847,7 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             T phi[4] = {1, 1, 1, 1};
             if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
-                for (int k = 0; k < m.mf; ++k) {
+                ZJ_FOR_FACES(k) {
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
@@ -846,7 +862,7 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
             } else if (o.lim != LIM_NONE) {
                 const double vol = cell_volume(m, v);
                 const T ev2 = T(m.dim == 1 ? vol * vol : fabs(vol));
-                for (int k = 0; k < m.mf; ++k) {
+                ZJ_FOR_FACES(k) {
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
@@ -874,6 +890,8 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
     // ── phase 3: fluxes of the own cells ─────────────────────────────
     int bad = 0;
     double best_r = INFINITY, best_a = 1.0;  // min over faces of r / a
+    T out_keep[4] = {0, 0, 0, 0};
+    int gi_keep = -1;
     const int ls = tid;
     if (ls < nown) {
         const int gi = lids[ls];
@@ -894,7 +912,7 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
         const double2 cc = Cs[ls];
         double nxs[MAXF], nys[MAXF];
         T acc[4] = {0, 0, 0, 0};
-        for (int k = 0; k < m.mf; ++k) {
+        ZJ_FOR_FACES(k) {
             double nxd, nyd, len;
             face_normal(m, v, k, nxd, nyd, len);
             nxs[k] = nxd;
@@ -962,19 +980,15 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
                 out[3] += dt * (-kk * vv2 * sp);
             }
         }
-        if (write_b) Qout[gi] = out[0];
-        Qout[N + gi] = out[1];
-        Qout[2 * N + gi] = out[2];
-        Qout[3 * N + gi] = out[3];
         if (final_stage && mode != 2) {
             bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
                   !isfinite(out[2]) || !isfinite(out[3]);
-            if (dt_red) {
+            if (dt_part) {
                 // min over faces of r_f / a_f without a division per face
                 const T hn = hinv_of(out[1], eps);
                 const double c = sqrt(o.g * fmax((double)out[1], 0.0));
                 const double rc = Rs[ls];
-                for (int k = 0; k < m.mf; ++k) {
+                ZJ_FOR_FACES(k) {
                     const uint16_t nl = nls[k];
                     const double rf = nl != LNONE ? fmin(rc, Rs[nl]) : rc;
                     const double un = (double)((out[2] * T(nxs[k]) + out[3] * T(nys[k])) * hn);
@@ -984,22 +998,30 @@ __global__ __launch_bounds__(SBLOCK) void k_stage(
                 }
             }
         }
+        out_keep[0] = out[0]; out_keep[1] = out[1]; out_keep[2] = out[2]; out_keep[3] = out[3];
+        gi_keep = gi;
     }
-    if (final_stage && mode != 2) {
-        if (troubled) {
-            auto add = [](int a, int b) { return a + b; };
-            int sb = block_reduce<int, decltype(add), SBLOCK>(bad, add, lds_i);
-            if (threadIdx.x == 0 && sb) atomicAdd(troubled, sb);
+    // Wave-level reductions, no block barrier: the troubled count is added
+    // only by waves that found one, the dt minimum of every wave goes to a
+    // per-wave slot that k_reduce_part folds (deterministic, no contention).
+    if (final_stage && mode != 2 && (troubled || dt_part)) {
+        double loc_dt = best_r == INFINITY ? INFINITY
+                                           : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
+        if (loc_dt != loc_dt) loc_dt = 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            bad += __shfl_xor(bad, off, 64);
+            loc_dt = fmin(loc_dt, __shfl_xor(loc_dt, off, 64));
         }
-        if (dt_red) {
-            double loc_dt = best_r == INFINITY ? INFINITY
-                                               : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
-            if (loc_dt != loc_dt) loc_dt = 0.0;
-            auto mn = [](double a, double b) { return fmin(a, b); };
-            double sd = block_reduce<double, decltype(mn), SBLOCK>(loc_dt, mn, lds_d);
-            if (threadIdx.x == 0)
-                atomicMin(dt_red, (unsigned long long)__double_as_longlong(sd));
+        if ((tid & 63) == 0) {
+            if (troubled && bad) atomicAdd(troubled, bad);
+            if (dt_part) dt_part[(size_t)p * (SBLOCK / 64) + (tid >> 6)] = loc_dt;
         }
+    }
+    if (gi_keep >= 0) {
+        if (write_b) Qout[gi_keep] = out_keep[0];
+        Qout[N + gi_keep] = out_keep[1];
+        Qout[2 * N + gi_keep] = out_keep[2];
+        Qout[3 * N + gi_keep] = out_keep[3];
     }
 }
 
@@ -1017,7 +1039,7 @@ struct zj_mesh {
 
 static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
     return (size_t)pt.ML * 16 + (size_t)pt.MV * 16 + (size_t)pt.MS * 8 +
-           (size_t)pt.ML * 5 * esz + (size_t)pt.MS * 8 * esz + (SBLOCK / 64) * 12;
+           (size_t)pt.ML * 5 * esz + (size_t)pt.MS * 8 * esz;
 }
 static constexpr size_t LDS_LIMIT = 64 * 1024;
 
@@ -1028,6 +1050,7 @@ struct zj_solver {
     void* S;                    // (N, 8) limited slopes
     void* Q1;                   // (4, N) stage state
     unsigned long long* dt_red; // reduction scalar
+    double* dt_part;            // per-wave dt minima of a stage launch
     int* troubled;              // scratch counter
     double2* mass_part;         // per-block mass partials
     double* dt_tmp;             // device dt
@@ -1194,6 +1217,8 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
     if (hipMalloc(&s->S, esz * 8 * N) != hipSuccess ||
         hipMalloc(&s->Q1, esz * 4 * N) != hipSuccess ||
         hipMalloc((void**)&s->dt_red, 8) != hipSuccess ||
+        hipMalloc((void**)&s->dt_part, sizeof(double) * (size_t)m->pt.npatch * (SBLOCK / 64)) !=
+            hipSuccess ||
         hipMalloc((void**)&s->troubled, 8) != hipSuccess ||
         hipMalloc((void**)&s->dt_tmp, 8) != hipSuccess ||
         hipMalloc((void**)&s->mass_part, sizeof(double2) * s->mass_blocks) != hipSuccess) {
@@ -1209,6 +1234,7 @@ int zj_solver_destroy(zj_solver* s) {
     (void)hipFree(s->S);
     (void)hipFree(s->Q1);
     (void)hipFree(s->dt_red);
+    (void)hipFree(s->dt_part);
     (void)hipFree(s->troubled);
     (void)hipFree(s->dt_tmp);
     (void)hipFree(s->mass_part);
@@ -1228,7 +1254,10 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
     hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(SBLOCK), lds, st, M->d, M->pt, s->o,
                        Qin, Q0, Qout, dt, aux, mode, final_stage, write_b, use_slopes, troubled,
-                       dt_red);
+                       dt_red ? s->dt_part : (double*)nullptr);
+    if (dt_red)
+        hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(1024), 0, st, (const double*)s->dt_part,
+                           M->pt.npatch * (SBLOCK / 64), dt_red);
 }
 
 template <typename T>
