@@ -107,6 +107,20 @@ def cpu_leg(budget_s=12.0):
                 elapsed=el)
 
 
+def measured_traffic(w, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated with a plain
+    1 GiB copy), or None when no summary exists for it."""
+    p = ROOT / "profiles" / "fv_stage_traffic.json"
+    try:
+        t = json.loads(p.read_text())
+        e = t["f64" if w == 8 else "f32"][kernel]
+        return (e["fetch_kib"] * t["fetch_scale"] +
+                e["write_kib"] * t["write_scale"]) * 1024.0
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def summarize_leg(r, w, N, steps):
     cups = N * steps / r["elapsed"]
     B = algorithmic_bytes(w)
@@ -115,7 +129,7 @@ def summarize_leg(r, w, N, steps):
     ach = B[dom] * N / (ms[dom] * 1e-3)
     return cups, dict(
         bound="hbm", kernel=dom, achieved=ach / 1e9, peak=PEAK_HBM / 1e9,
-        unit="GB/s", frac=ach / PEAK_HBM, traffic=None,
+        unit="GB/s", frac=ach / PEAK_HBM, traffic=measured_traffic(w, dom),
         bytes_per_launch=B[dom] * N, launch_ms=ms[dom],
         step_alg_bytes_per_cell=B["cell_update"],
         step_frac=cups * B["cell_update"] / PEAK_HBM)
