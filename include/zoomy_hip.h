@@ -27,6 +27,7 @@ extern "C" {
 #define ZJ_EHIP 2       /* a HIP runtime call failed                   */
 #define ZJ_ENOMEM 3     /* device allocation failed                    */
 #define ZJ_EUNSUP 4     /* unsupported option combination              */
+#define ZJ_ESOLVE 5     /* pressure solve missed its accept gate       */
 
 /* boundary kinds, limiters, Riemann solvers, reconstruction variables */
 enum { ZJ_BC_EXTRAPOLATION = 0, ZJ_BC_WALL = 1, ZJ_BC_PERIODIC = 2 };
@@ -128,6 +129,68 @@ int zj_mass(zj_solver *s, const void *Q, double *mass_out, void *stream);
  * iterations per cell on Q - Qexp - dt S(Q) = 0. */
 int zj_imex_local(zj_solver *s, const double *dt, int32_t maxiter,
                   const void *Qexp, void *Qout, void *stream);
+
+/* ───────────────────────── Chorin projection ─────────────────────────────
+ * Non-hydrostatic projection on an nx x ny wall-bounded quad grid whose
+ * cells are numbered i*ny + j (Mesh.quad order).  State (5, N) rows
+ * (b, h, hu, hv, hw).  The FP64 pressure operator S is symmetric positive
+ * definite (5-point; a_c = h_c + b_c - b_e, c_f = |f|/d_f hinv(h_f),
+ * S_cc = sum_f c_f a_c^2 + 4|A_c| hinv(h_c), S_ce = -c_f a_c a_e).
+ * Pressure solves are host-synchronous once per chunk of CG iterations.
+ * Replaces ChorinSplitVAMSolverJax (zoomy_jax/fvm/solver_chorin_vam_jax.py). */
+enum { ZJ_PRECOND_MG = 0, ZJ_PRECOND_LEARNED = 1 };
+typedef struct zj_chorin zj_chorin;
+typedef struct zj_mpnn zj_mpnn;     /* learned preconditioner (zj_mpnn_*) */
+typedef struct {
+    double tol;          /* relative residual target (1e-8)                  */
+    double inner_tol;    /* FP32 inner solve target of the refinement (1e-4) */
+    double eps_wet;      /* hinv desingularisation (1e-8)                    */
+    int32_t maxiter;     /* CG iteration cap per solve                       */
+    int32_t nu;          /* smoothing sweeps per level and direction         */
+    int32_t coarse_sweeps; /* symmetric sweep pairs on the coarsest level    */
+    int32_t min_coarse;  /* stop coarsening before a side drops below this   */
+    int32_t precond;     /* ZJ_PRECOND_*                                     */
+    int32_t time_order;  /* 1: one stage; 2: two stages + average            */
+    int32_t max_refine;  /* outer FP64 refinement sweeps (learned)           */
+} zj_chorin_params;
+typedef struct {
+    int32_t solves, iterations, fallbacks, learned_inner, learned_outer;
+    double relres, learned_relres; /* worst relative residual of the call */
+} zj_chorin_stats;
+/* fv: an FP64 zj_solver of the same nx x ny mesh (hyperbolic predictor). */
+int zj_chorin_create(zj_solver *fv, int32_t nx, int32_t ny, double dx, double dy,
+                     const zj_chorin_params *prm, zj_chorin **out);
+int zj_chorin_destroy(zj_chorin *c);
+/* Select the preconditioner; net must be non-NULL for ZJ_PRECOND_LEARNED. */
+int zj_chorin_set_precond(zj_chorin *c, int32_t precond, zj_mpnn *net);
+/* chorin_cycle (solver_chorin_vam_jax.py:574-615): one step of time_order
+ * stages; returns ZJ_ESOLVE when a pressure solve misses the accept gate
+ * max(1e-6, 10 tol) (:72-91, EllipticSolveError). */
+int zj_chorin_step(zj_chorin *c, double dt, const double *Q, double *Qout, void *stream);
+/* chorin_stage (:524-572): hyperbolic Euler -> pressure -> corrector; with
+ * Q0 != NULL the result is averaged with Q0 (second SSP-RK2 stage). */
+int zj_chorin_stage(zj_chorin *c, double dt, const double *Q, double *Qout,
+                    const double *Q0, void *stream);
+/* _hyperbolic_stage (:496-522): Qs = Q + dt L(Q), hw advected. */
+int zj_chorin_predict(zj_chorin *c, double dt, const double *Q, double *Qs, void *stream);
+/* Right-hand side of the pressure equation of a predicted state:
+ * rhs = -D(Qs)/dt (_step_pressure_pure's residual, :728-743). */
+int zj_chorin_rhs(zj_chorin *c, double dt, const double *Qs, double *rhs, void *stream);
+/* _step_corrector_pure (:885-907). */
+int zj_chorin_correct(zj_chorin *c, double dt, const double *Qs, const double *p,
+                      const double *Q0, double *Qout, void *stream);
+/* Build the operator hierarchy of S(h, b) (device rows b, h). */
+int zj_chorin_setup_operator(zj_chorin *c, const double *b, const double *h, void *stream);
+/* y = S x on the current operator (the JVP matvec, :759-761). */
+int zj_chorin_apply(zj_chorin *c, const double *x, double *y, void *stream);
+/* z = V-cycle(r) on the current operator, FP64 or FP32 hierarchy. */
+int zj_chorin_vcycle(zj_chorin *c, const void *r, void *z, int32_t dtype, void *stream);
+/* Solve S p = rhs with the selected preconditioner (GMRES of :800-805). */
+int zj_chorin_solve(zj_chorin *c, const double *rhs, double *p, void *stream);
+/* Number of multigrid levels and their (nx, ny). */
+int zj_chorin_levels(zj_chorin *c, int32_t *n_levels, int32_t *dims_out, int32_t max_levels);
+/* Statistics of the last step / stage / solve call. */
+int zj_chorin_get_stats(zj_chorin *c, zj_chorin_stats *out);
 
 /* Library identification string. */
 const char *zj_version(void);

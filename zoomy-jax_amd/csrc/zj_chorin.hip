@@ -1,0 +1,1023 @@
+// Chorin projection on a structured nx x ny quad grid (cell id i*ny + j,
+// walls): predictor, pressure operator, geometric multigrid, preconditioned
+// CG and corrector, behind the zj_chorin_* C ABI (include/zoomy_hip.h).
+//
+// Pressure operator (5-point, SPD; a_c = h_c + b_c - b_e, a_e = h_e + b_e -
+// b_c, c_f = |f|/d_f hinv(h_f) on the face between cells c and e):
+//     S_cc += c_f a_c^2,  S_ce = -c_f a_c a_e,  S_cc += 4 |A_c| hinv(h_c)
+// stored per cell as (diag, cE = S[c, c+ny], cN = S[c, c+1]).
+//
+// Multigrid: coarse levels are re-discretised from 4-cell averages of h and
+// b; red-black Gauss-Seidel smoothing (pre: red, black; post: black, red);
+// cell-centred bilinear prolongation and its transpose as restriction; the
+// coarsest level is smoothed to convergence by one workgroup.  The V-cycle is
+// symmetric, so plain CG uses it as preconditioner.
+//
+// Every CG kernel takes the device flag `done`; once the residual test has
+// passed, later kernels of an already launched chunk return at once, so the
+// host enqueues iterations in chunks and synchronises once per chunk.
+// Reductions are two-pass (per-block partials in a fixed order, then one
+// workgroup), hence bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/zoomy_hip.h"
+#include "zj_chorin.hpp"
+#include "zj_device.hpp"
+
+using namespace zj;
+
+namespace {
+
+constexpr int CB = 256;      // threads per block of the grid kernels
+constexpr int RED_MAX = 1024;
+
+__device__ __forceinline__ bool is_done(const int* done) { return done && *done; }
+
+// ───────────────────────── hierarchy: averages and stencils ─────────────────
+__global__ __launch_bounds__(CB) void k_hb_avg(int nxc, int nyc, const double* __restrict__ hf,
+                                               const double* __restrict__ bf,
+                                               double* __restrict__ hc, double* __restrict__ bc) {
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= nxc * nyc) return;
+    int I = c / nyc, J = c - I * nyc;
+    int nyf = 2 * nyc;
+    int f00 = (2 * I) * nyf + 2 * J;
+    int f10 = f00 + nyf;
+    hc[c] = 0.25 * (((hf[f00] + hf[f10]) + hf[f00 + 1]) + hf[f10 + 1]);
+    bc[c] = 0.25 * (((bf[f00] + bf[f10]) + bf[f00 + 1]) + bf[f10 + 1]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_coef(int nx, int ny, double dx, double dy, double eps,
+                                             const double* __restrict__ h,
+                                             const double* __restrict__ b, T* __restrict__ diag,
+                                             T* __restrict__ cE, T* __restrict__ cN) {
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= nx * ny) return;
+    int i = c / ny, j = c - i * ny;
+    double hc = h[c], bc = b[c];
+    double d = 4.0 * dx * dy * hinv_of(hc, eps);
+    double e = 0.0, n = 0.0;
+    const double rx = dy / dx, ry = dx / dy;
+    // faces to the four neighbours: (offset, ratio |f|/d_f, is the E/N face)
+    if (i + 1 < nx) {
+        int o = c + ny;
+        double cf = rx * hinv_of(0.5 * (hc + h[o]), eps);
+        double ac = hc + bc - b[o], ae = h[o] + b[o] - bc;
+        d += cf * ac * ac;
+        e = -cf * ac * ae;
+    }
+    if (i > 0) {
+        int o = c - ny;
+        double cf = rx * hinv_of(0.5 * (h[o] + hc), eps);
+        double ac = hc + bc - b[o];
+        d += cf * ac * ac;
+    }
+    if (j + 1 < ny) {
+        int o = c + 1;
+        double cf = ry * hinv_of(0.5 * (hc + h[o]), eps);
+        double ac = hc + bc - b[o], ae = h[o] + b[o] - bc;
+        d += cf * ac * ac;
+        n = -cf * ac * ae;
+    }
+    if (j > 0) {
+        int o = c - 1;
+        double cf = ry * hinv_of(0.5 * (h[o] + hc), eps);
+        double ac = hc + bc - b[o];
+        d += cf * ac * ac;
+    }
+    diag[c] = (T)d;
+    cE[c] = (T)e;
+    cN[c] = (T)n;
+}
+
+// sum_{e != c} S_ce x_e
+template <typename T>
+__device__ __forceinline__ T offdiag(int c, int i, int j, int nx, int ny, const T* __restrict__ cE,
+                                     const T* __restrict__ cN, const T* __restrict__ x) {
+    T s = T(0);
+    if (i + 1 < nx) s += cE[c] * x[c + ny];
+    if (i > 0) s += cE[c - ny] * x[c - ny];
+    if (j + 1 < ny) s += cN[c] * x[c + 1];
+    if (j > 0) s += cN[c - 1] * x[c - 1];
+    return s;
+}
+
+// ───────────────────────── multigrid kernels ───────────────────────────────
+// One thread per cell of the given colour ((i + j) & 1 == colour).  With
+// first != 0 the iterate is zero on entry (colour 0 only).
+template <typename T>
+__global__ __launch_bounds__(CB) void k_rbgs(int nx, int ny, const T* __restrict__ diag,
+                                             const T* __restrict__ cE, const T* __restrict__ cN,
+                                             T* __restrict__ x, const T* __restrict__ f, int colour,
+                                             int first, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const int half = (ny + 1) >> 1;
+    int t = blockIdx.x * CB + threadIdx.x;
+    if (t >= nx * half) return;
+    int i = t / half, k = t - i * half;
+    int j = 2 * k + ((i + colour) & 1);
+    if (j >= ny) return;
+    int c = i * ny + j;
+    if (first) {
+        x[c] = f[c] / diag[c];
+        return;
+    }
+    x[c] = (f[c] - offdiag(c, i, j, nx, ny, cE, cN, x)) / diag[c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_resid(int nx, int ny, const T* __restrict__ diag,
+                                              const T* __restrict__ cE, const T* __restrict__ cN,
+                                              const T* __restrict__ x, const T* __restrict__ f,
+                                              T* __restrict__ r, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= nx * ny) return;
+    int i = c / ny, j = c - i * ny;
+    r[c] = f[c] - (diag[c] * x[c] + offdiag(c, i, j, nx, ny, cE, cN, x));
+}
+
+// 1-D weight of fine cell 2I + d (d in -1..2) in coarse cell I of nc.
+__device__ __forceinline__ double rw(int I, int d, int nc) {
+    if (d == -1) return I >= 1 ? 0.25 : 0.0;
+    if (d == 0) return I == 0 ? 1.0 : 0.75;
+    if (d == 1) return I == nc - 1 ? 1.0 : 0.75;
+    return I + 1 < nc ? 0.25 : 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_restrict(int nxc, int nyc, const T* __restrict__ r,
+                                                 T* __restrict__ fc, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= nxc * nyc) return;
+    int I = c / nyc, J = c - I * nyc;
+    const int nyf = 2 * nyc;
+    double s = 0.0;
+#pragma unroll
+    for (int di = -1; di <= 2; ++di) {
+        double wx = rw(I, di, nxc);
+        if (wx == 0.0) continue;
+        const T* row = r + (size_t)(2 * I + di) * nyf;
+        double sr = 0.0;
+#pragma unroll
+        for (int dj = -1; dj <= 2; ++dj) {
+            double wy = rw(J, dj, nyc);
+            if (wy != 0.0) sr += wy * (double)row[2 * J + dj];
+        }
+        s += wx * sr;
+    }
+    fc[c] = (T)s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_prolong(int nx, int ny, const T* __restrict__ xc,
+                                                T* __restrict__ x, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= nx * ny) return;
+    int i = c / ny, j = c - i * ny;
+    const int nxc = nx >> 1, nyc = ny >> 1;
+    int I = i >> 1, Io = I + ((i & 1) ? 1 : -1);
+    int J = j >> 1, Jo = J + ((j & 1) ? 1 : -1);
+    double wI = 1.0, wIo = 0.0, wJ = 1.0, wJo = 0.0;
+    if (Io >= 0 && Io < nxc) { wI = 0.75; wIo = 0.25; } else Io = I;
+    if (Jo >= 0 && Jo < nyc) { wJ = 0.75; wJo = 0.25; } else Jo = J;
+    double v = wI * (wJ * (double)xc[I * nyc + J] + wJo * (double)xc[I * nyc + Jo]) +
+               wIo * (wJ * (double)xc[Io * nyc + J] + wJo * (double)xc[Io * nyc + Jo]);
+    x[c] = (T)((double)x[c] + v);
+}
+
+// Coarsest level: sweeps x (red, black, black, red) from x = 0, one block.
+template <typename T>
+__global__ __launch_bounds__(CB) void k_coarse(int nx, int ny, const T* __restrict__ diag,
+                                               const T* __restrict__ cE, const T* __restrict__ cN,
+                                               T* __restrict__ x, const T* __restrict__ f,
+                                               int sweeps, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const int n = nx * ny;
+    for (int c = threadIdx.x; c < n; c += CB) x[c] = T(0);
+    __syncthreads();
+    for (int s = 0; s < sweeps; ++s) {
+        for (int q = 0; q < 4; ++q) {
+            const int colour = (q == 0 || q == 3) ? 0 : 1;
+            for (int c = threadIdx.x; c < n; c += CB) {
+                int i = c / ny, j = c - i * ny;
+                if (((i + j) & 1) != colour) continue;
+                x[c] = (f[c] - offdiag(c, i, j, nx, ny, cE, cN, x)) / diag[c];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ───────────────────────── CG kernels ──────────────────────────────────────
+__device__ __forceinline__ void block_sum_store(double v, double* part) {
+    __shared__ double lds[CB / 64];
+    auto add = [](double a, double b) { return a + b; };
+    v = block_reduce<double, decltype(add), CB>(v, add, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_spmv(int nx, int ny, const T* __restrict__ diag,
+                                             const T* __restrict__ cE, const T* __restrict__ cN,
+                                             const T* __restrict__ x, T* __restrict__ y,
+                                             double* __restrict__ part,
+                                             const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const int n = nx * ny;
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        int i = c / ny, j = c - i * ny;
+        T v = diag[c] * x[c] + offdiag(c, i, j, nx, ny, cE, cN, x);
+        y[c] = v;
+        acc += (double)x[c] * (double)v;
+    }
+    if (part) block_sum_store(acc, part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                            double* __restrict__ part,
+                                            const int* __restrict__ done) {
+    if (is_done(done)) return;
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB)
+        acc += (double)a[c] * (double)b[c];
+    block_sum_store(acc, part);
+}
+
+// x += alpha d, r -= alpha q (r_old keeps r before the update when given);
+// partial sums of r.r
+template <typename T>
+__global__ __launch_bounds__(CB) void k_axpy2(int n, T* __restrict__ x, T* __restrict__ r,
+                                              T* __restrict__ r_old, const T* __restrict__ d,
+                                              const T* __restrict__ q,
+                                              const double* __restrict__ sc,
+                                              double* __restrict__ part,
+                                              const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const T alpha = (T)sc[CG_ALPHA];
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        x[c] += alpha * d[c];
+        T rv = r[c];
+        if (r_old) r_old[c] = rv;
+        rv -= alpha * q[c];
+        r[c] = rv;
+        acc += (double)rv * (double)rv;
+    }
+    block_sum_store(acc, part);
+}
+
+// d = z + beta d
+template <typename T>
+__global__ __launch_bounds__(CB) void k_xpby(int n, const T* __restrict__ z, T* __restrict__ d,
+                                             const double* __restrict__ sc,
+                                             const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const T beta = (T)sc[CG_BETA];
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB)
+        d[c] = z[c] + beta * d[c];
+}
+
+// r = rhs - S x (x may be NULL: zero initial guess, then x is zeroed);
+// partial sums of rhs.rhs
+template <typename T>
+__global__ __launch_bounds__(CB) void k_cg_init(int nx, int ny, const T* __restrict__ diag,
+                                                const T* __restrict__ cE,
+                                                const T* __restrict__ cN, T* __restrict__ x,
+                                                int zero_x, const T* __restrict__ rhs,
+                                                T* __restrict__ r, double* __restrict__ part) {
+    const int n = nx * ny;
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        T f = rhs[c];
+        if (zero_x) {
+            x[c] = T(0);
+            r[c] = f;
+        } else {
+            int i = c / ny, j = c - i * ny;
+            r[c] = f - (diag[c] * x[c] + offdiag(c, i, j, nx, ny, cE, cN, x));
+        }
+        acc += (double)f * (double)f;
+    }
+    block_sum_store(acc, part);
+}
+
+// Two sums (a.b, a.c) in one pass (flexible CG: z.r and z.r_old).
+template <typename T>
+__global__ __launch_bounds__(CB) void k_dot2(int n, const T* __restrict__ a,
+                                             const T* __restrict__ b, const T* __restrict__ c2,
+                                             double* __restrict__ part,
+                                             const int* __restrict__ done) {
+    if (is_done(done)) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        double av = (double)a[c];
+        s1 += av * (double)b[c];
+        s2 += av * (double)c2[c];
+    }
+    block_sum_store(s1, part);
+    __syncthreads();
+    block_sum_store(s2, part + gridDim.x);
+}
+
+// Scalar step of CG; sums the partials of the preceding reduction in a fixed
+// order.  sc: CG_* slots, fl: {done, iterations}.
+__global__ __launch_bounds__(RED_MAX) void k_cg_fin(const double* __restrict__ part, int np,
+                                                    int op, double tol2, int maxit,
+                                                    double* __restrict__ sc, int* __restrict__ fl) {
+    if (op != FIN_B2 && fl[0]) return;
+    __shared__ double lds[RED_MAX / 64];
+    auto add = [](double a, double b) { return a + b; };
+    double v = 0.0, v2 = 0.0;
+    for (int i = threadIdx.x; i < np; i += RED_MAX) v += part[i];
+    v = block_reduce<double, decltype(add), RED_MAX>(v, add, lds);
+    if (op == FIN_RZ_FLEX) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < np; i += RED_MAX) v2 += part[np + i];
+        v2 = block_reduce<double, decltype(add), RED_MAX>(v2, add, lds);
+    }
+    if (threadIdx.x) return;
+    switch (op) {
+        case FIN_B2:
+            sc[CG_B2] = v;
+            sc[CG_RR] = v;
+            sc[CG_BETA] = 0.0;
+            fl[1] = 0;
+            fl[0] = !(v > 0.0);
+            break;
+        case FIN_RR0:  // residual of a non-zero initial guess
+            sc[CG_RR] = v;
+            if (v <= tol2 * sc[CG_B2]) fl[0] = 1;
+            break;
+        case FIN_RZ0:
+            sc[CG_RZ] = v;
+            sc[CG_BETA] = 0.0;
+            break;
+        case FIN_PQ:
+            sc[CG_ALPHA] = sc[CG_RZ] / v;
+            if (!(v > 0.0)) fl[0] = 2;  // breakdown: not positive definite
+            break;
+        case FIN_RR:
+            sc[CG_RR] = v;
+            fl[1] += 1;
+            if (v <= tol2 * sc[CG_B2] || fl[1] >= maxit) fl[0] = 1;
+            break;
+        case FIN_RZ:
+            sc[CG_BETA] = v / sc[CG_RZ];
+            sc[CG_RZ] = v;
+            break;
+        case FIN_RZ_FLEX:  // Polak-Ribiere: beta = z.(r - r_old) / rz_old
+            sc[CG_BETA] = (v - v2) / sc[CG_RZ];
+            sc[CG_RZ] = v;
+            break;
+    }
+}
+
+// ───────────────────────── Chorin stage kernels ────────────────────────────
+// Qs = Q + dt dQ (rows b, h, hu, hv) and the upwind advection of hw with the
+// central face mass flux of Q.
+__global__ __launch_bounds__(CB) void k_predict(int nx, int ny, double dx, double dy, double dt,
+                                                double eps, const double* __restrict__ Q,
+                                                const double* __restrict__ dQ,
+                                                double* __restrict__ Qs) {
+    const int n = nx * ny;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= n) return;
+    int i = c / ny, j = c - i * ny;
+    const double *h = Q + n, *hu = Q + 2 * n, *hv = Q + 3 * n, *hw = Q + 4 * n;
+    Qs[c] = Q[c] + dt * dQ[c];
+    Qs[n + c] = h[c] + dt * dQ[n + c];
+    Qs[2 * n + c] = hu[c] + dt * dQ[2 * n + c];
+    Qs[3 * n + c] = hv[c] + dt * dQ[3 * n + c];
+    const double wc = hw[c] * hinv_of(h[c], eps);
+    double acc = 0.0;
+    if (i + 1 < nx) {
+        int o = c + ny;
+        double m = 0.5 * (hu[c] + hu[o]) * dy;
+        acc += m * (m > 0.0 ? wc : hw[o] * hinv_of(h[o], eps));
+    }
+    if (i > 0) {
+        int o = c - ny;
+        double m = 0.5 * (hu[o] + hu[c]) * dy;
+        acc -= m * (m > 0.0 ? hw[o] * hinv_of(h[o], eps) : wc);
+    }
+    if (j + 1 < ny) {
+        int o = c + 1;
+        double m = 0.5 * (hv[c] + hv[o]) * dx;
+        acc += m * (m > 0.0 ? wc : hw[o] * hinv_of(h[o], eps));
+    }
+    if (j > 0) {
+        int o = c - 1;
+        double m = 0.5 * (hv[o] + hv[c]) * dx;
+        acc -= m * (m > 0.0 ? hw[o] * hinv_of(h[o], eps) : wc);
+    }
+    Qs[4 * n + c] = hw[c] - dt / (dx * dy) * acc;
+}
+
+// rhs = -D(Qs)/dt, D_c = sum_f |f| v_f a_c + 2 |A_c| w_c, v_f the average of
+// the two cells' outward normal velocities
+template <typename T>
+__global__ __launch_bounds__(CB) void k_rhs(int nx, int ny, double dx, double dy, double dt,
+                                            double eps, const double* __restrict__ Qs,
+                                            T* __restrict__ rhs) {
+    const int n = nx * ny;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= n) return;
+    int i = c / ny, j = c - i * ny;
+    const double *b = Qs, *h = Qs + n, *hu = Qs + 2 * n, *hv = Qs + 3 * n, *hw = Qs + 4 * n;
+    const double hc = h[c], bc = b[c], hic = hinv_of(hc, eps);
+    const double uc = hu[c] * hic, vc = hv[c] * hic;
+    double D = 2.0 * dx * dy * hw[c] * hic;
+    if (i + 1 < nx) {
+        int o = c + ny;
+        D += dy * 0.5 * (uc + hu[o] * hinv_of(h[o], eps)) * (hc + bc - b[o]);
+    }
+    if (i > 0) {
+        int o = c - ny;
+        D -= dy * 0.5 * (hu[o] * hinv_of(h[o], eps) + uc) * (hc + bc - b[o]);
+    }
+    if (j + 1 < ny) {
+        int o = c + 1;
+        D += dx * 0.5 * (vc + hv[o] * hinv_of(h[o], eps)) * (hc + bc - b[o]);
+    }
+    if (j > 0) {
+        int o = c - 1;
+        D -= dx * 0.5 * (hv[o] * hinv_of(h[o], eps) + vc) * (hc + bc - b[o]);
+    }
+    rhs[c] = (T)(-D / dt);
+}
+
+// hu -= dt * (average of the two x-face B p), hv likewise, hw += 2 dt p;
+// with Q0 the result is averaged with it: Qout = (Q0 + corrected) / 2.
+__global__ __launch_bounds__(CB) void k_correct(int nx, int ny, double dx, double dy, double dt,
+                                                const double* __restrict__ Qs,
+                                                const double* __restrict__ p,
+                                                const double* __restrict__ Q0,
+                                                double* __restrict__ Qout) {
+    const int n = nx * ny;
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c >= n) return;
+    int i = c / ny, j = c - i * ny;
+    const double *b = Qs, *h = Qs + n;
+    const double hc = h[c], bc = b[c], pc = p[c];
+    double fx = 0.0, fy = 0.0;
+    if (i + 1 < nx) {  // face (c, c+ny) oriented +x
+        int o = c + ny;
+        fx += ((h[o] + b[o] - bc) * p[o] - (hc + bc - b[o]) * pc) / dx;
+    }
+    if (i > 0) {
+        int o = c - ny;
+        fx += ((hc + bc - b[o]) * pc - (h[o] + b[o] - bc) * p[o]) / dx;
+    }
+    if (j + 1 < ny) {
+        int o = c + 1;
+        fy += ((h[o] + b[o] - bc) * p[o] - (hc + bc - b[o]) * pc) / dy;
+    }
+    if (j > 0) {
+        int o = c - 1;
+        fy += ((hc + bc - b[o]) * pc - (h[o] + b[o] - bc) * p[o]) / dy;
+    }
+    double q0 = bc, q1 = hc;
+    double q2 = Qs[2 * n + c] - dt * 0.5 * fx;
+    double q3 = Qs[3 * n + c] - dt * 0.5 * fy;
+    double q4 = Qs[4 * n + c] + 2.0 * dt * pc;
+    if (Q0) {
+        q0 = 0.5 * (Q0[c] + q0);
+        q1 = 0.5 * (Q0[n + c] + q1);
+        q2 = 0.5 * (Q0[2 * n + c] + q2);
+        q3 = 0.5 * (Q0[3 * n + c] + q3);
+        q4 = 0.5 * (Q0[4 * n + c] + q4);
+    }
+    Qout[c] = q0;
+    Qout[n + c] = q1;
+    Qout[2 * n + c] = q2;
+    Qout[3 * n + c] = q3;
+    Qout[4 * n + c] = q4;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_fill(int n, T v, T* __restrict__ o) {
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c < n) o[c] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(CB) void k_cast(int n, const double* __restrict__ a, T* __restrict__ o) {
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c < n) o[c] = (T)a[c];
+}
+
+// x64 += e32 (iterative refinement update)
+__global__ __launch_bounds__(CB) void k_add_f32(int n, const float* __restrict__ e,
+                                                double* __restrict__ x) {
+    int c = blockIdx.x * CB + threadIdx.x;
+    if (c < n) x[c] += (double)e[c];
+}
+
+inline int nblk(long n) { return (int)((n + CB - 1) / CB); }
+inline int nred(long n) { return (int)std::min<long>(RED_MAX, (n + CB - 1) / CB); }
+
+}  // namespace
+
+// ═════════════════════════════ host side ═══════════════════════════════════
+#define HIPCHK(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) {                                       \
+            fprintf(stderr, "zoomy_hip: %s failed: %s\n", #x,         \
+                    hipGetErrorString(e_));                           \
+            return ZJ_EHIP;                                           \
+        }                                                             \
+    } while (0)
+
+namespace zj {
+
+int Hier::alloc(int nx0, int ny0, double dx0, double dy0, int min_coarse, size_t esz_) {
+    esz = esz_;
+    levels.clear();
+    int nx = nx0, ny = ny0;
+    double dx = dx0, dy = dy0;
+    while (true) {
+        HLevel L;
+        L.nx = nx; L.ny = ny; L.dx = dx; L.dy = dy;
+        levels.push_back(L);
+        if ((nx & 1) || (ny & 1) || std::min(nx, ny) / 2 < min_coarse) break;
+        nx /= 2; ny /= 2; dx *= 2; dy *= 2;
+    }
+    size_t tot = 0;
+    for (size_t l = 0; l < levels.size(); ++l) {
+        size_t n = (size_t)levels[l].nx * levels[l].ny;
+        tot += 6 * n * esz + (l ? 2 * n * sizeof(double) : 0) + 3 * 256;
+    }
+    if (hipMalloc(&mem, tot) != hipSuccess) return ZJ_ENOMEM;
+    char* p = (char*)mem;
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    for (size_t l = 0; l < levels.size(); ++l) {
+        HLevel& L = levels[l];
+        size_t n = (size_t)L.nx * L.ny;
+        L.diag = take(n * esz); L.cE = take(n * esz); L.cN = take(n * esz);
+        L.x = take(n * esz); L.f = take(n * esz); L.r = take(n * esz);
+        if (l) { L.h = (double*)take(n * 8); L.b = (double*)take(n * 8); }
+    }
+    return ZJ_OK;
+}
+
+void Hier::release() {
+    if (mem) (void)hipFree(mem);
+    mem = nullptr;
+}
+
+template <typename T>
+static void build_levels(Hier& H, const double* h0, const double* b0, double eps, hipStream_t st) {
+    for (size_t l = 0; l < H.levels.size(); ++l) {
+        HLevel& L = H.levels[l];
+        const int n = L.nx * L.ny;
+        const double* h = h0;
+        const double* b = b0;
+        if (l) {
+            const HLevel& F = H.levels[l - 1];
+            const double* hf = l == 1 ? h0 : F.h;
+            const double* bf = l == 1 ? b0 : F.b;
+            k_hb_avg<<<nblk(n), CB, 0, st>>>(L.nx, L.ny, hf, bf, L.h, L.b);
+            h = L.h;
+            b = L.b;
+        }
+        k_coef<T><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, L.dx, L.dy, eps, h, b, (T*)L.diag,
+                                          (T*)L.cE, (T*)L.cN);
+    }
+}
+
+// z = V(r) on the hierarchy; z and r are level-0 vectors (not L.x / L.f).
+template <typename T>
+static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const int* done,
+                   hipStream_t st) {
+    const int nl = (int)H.levels.size();
+    for (int l = 0; l < nl - 1; ++l) {
+        HLevel& L = H.levels[l];
+        T* x = l ? (T*)L.x : z0;
+        const T* f = l ? (const T*)L.f : r0;
+        const int half = L.nx * ((L.ny + 1) >> 1);
+        for (int s = 0; s < nu; ++s) {
+            k_rbgs<T><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x,
+                                                 f, 0, s == 0, done);
+            k_rbgs<T><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x,
+                                                 f, 1, 0, done);
+        }
+        const int n = L.nx * L.ny;
+        k_resid<T><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x, f,
+                                           (T*)L.r, done);
+        HLevel& C = H.levels[l + 1];
+        k_restrict<T><<<nblk(C.nx * C.ny), CB, 0, st>>>(C.nx, C.ny, (T*)L.r, (T*)C.f, done);
+    }
+    {
+        HLevel& C = H.levels[nl - 1];
+        T* x = nl > 1 ? (T*)C.x : z0;
+        const T* f = nl > 1 ? (const T*)C.f : r0;
+        k_coarse<T><<<1, CB, 0, st>>>(C.nx, C.ny, (T*)C.diag, (T*)C.cE, (T*)C.cN, x, f,
+                                      coarse_sweeps, done);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        HLevel& L = H.levels[l];
+        HLevel& C = H.levels[l + 1];
+        T* x = l ? (T*)L.x : z0;
+        const T* f = l ? (const T*)L.f : r0;
+        const int n = L.nx * L.ny;
+        k_prolong<T><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (T*)C.x, x, done);
+        const int half = L.nx * ((L.ny + 1) >> 1);
+        for (int s = 0; s < nu; ++s) {
+            k_rbgs<T><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x,
+                                                 f, 1, 0, done);
+            k_rbgs<T><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x,
+                                                 f, 0, 0, done);
+        }
+    }
+}
+
+}  // namespace zj
+
+struct zj_chorin {
+    zj_solver* fv = nullptr;
+    int nx = 0, ny = 0, N = 0;
+    double dx = 0, dy = 0, eps = 1e-8;
+    zj_chorin_params prm{};
+    zj::Hier H64, H32;
+    // level-0 FP64 vectors
+    double *Qs = nullptr, *Q1 = nullptr, *dQ = nullptr, *rhs = nullptr, *p = nullptr;
+    double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
+    // FP32 vectors of the inner solve of the refinement
+    float *r32 = nullptr, *x32 = nullptr, *s32 = nullptr, *z32 = nullptr, *d32 = nullptr,
+          *q32 = nullptr, *ro32 = nullptr;
+    double* part = nullptr;  // reduction partials (2 * RED_MAX)
+    double* sc = nullptr;    // CG scalars
+    int* fl = nullptr;       // {done, iterations}
+    int* host_fl = nullptr;  // pinned copy of fl
+    double* host_sc = nullptr;
+    zj_mpnn* net = nullptr;  // learned preconditioner (may be NULL)
+    int last_iters = 8, last_iters32 = 8;
+    zj_chorin_stats stats{};
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+static int dalloc(zj_chorin* c, T** p, size_t n) {
+    void* v = nullptr;
+    if (hipMalloc(&v, n * sizeof(T)) != hipSuccess) return ZJ_ENOMEM;
+    c->allocs.push_back(v);
+    *p = (T*)v;
+    return ZJ_OK;
+}
+
+static int read_flags(zj_chorin* c, hipStream_t st) {
+    HIPCHK(hipMemcpyAsync(c->host_fl, c->fl, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->host_sc, c->sc, CG_NSLOT * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+    return ZJ_OK;
+}
+
+// Preconditioned CG on level 0 of H (T = double: plain MG-CG; T = float with
+// `learned`: flexible CG with the learned preconditioner).  x holds the
+// initial guess when x0 != 0.  Host-synchronous once per chunk of iterations.
+template <typename T>
+static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z, T* d, T* q,
+               T* r_old, double tol, int maxit, int learned, hipStream_t st, int* iters,
+               double* relres) {
+    zj::HLevel& L = H.levels[0];
+    const int n = L.nx * L.ny;
+    const int nr = nred(n);
+    const T *D = (T*)L.diag, *E = (T*)L.cE, *Nn = (T*)L.cN;
+    const double tol2 = tol * tol;
+    auto precond = [&](const T* rin, T* zout) -> int {
+        if (learned) {
+            int rc = zj::mpnn_apply_f32(c->net, c, (const float*)rin, (float*)zout, c->fl, st);
+            if (rc) return rc;
+        } else {
+            zj::vcycle<T>(H, rin, zout, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
+        }
+        return ZJ_OK;
+    };
+    k_cg_init<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, c->part);
+    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_B2, tol2, maxit, c->sc, c->fl);
+    if (x0) {
+        k_dot<T><<<nr, CB, 0, st>>>(n, r, r, c->part, c->fl);
+        k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR0, tol2, maxit, c->sc, c->fl);
+    }
+    if (int rc = precond(r, z)) return rc;
+    k_dot<T><<<nr, CB, 0, st>>>(n, r, z, c->part, c->fl);
+    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ0, tol2, maxit, c->sc, c->fl);
+    k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, c->fl);  // beta = 0: d = z
+    HIPCHK(hipGetLastError());
+    int launched = 0;
+    int& last = learned ? c->last_iters32 : c->last_iters;
+    int chunk = std::max(1, std::min(maxit, last));
+    while (true) {
+        for (int k = 0; k < chunk; ++k) {
+            k_spmv<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, c->part, c->fl);
+            k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_PQ, tol2, maxit, c->sc, c->fl);
+            k_axpy2<T><<<nr, CB, 0, st>>>(n, x, r, learned ? r_old : nullptr, d, q, c->sc,
+                                          c->part, c->fl);
+            k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR, tol2, maxit, c->sc, c->fl);
+            if (int rc = precond(r, z)) return rc;
+            if (learned) {
+                k_dot2<T><<<nr, CB, 0, st>>>(n, z, r, r_old, c->part, c->fl);
+                k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ_FLEX, tol2, maxit, c->sc,
+                                                c->fl);
+            } else {
+                k_dot<T><<<nr, CB, 0, st>>>(n, z, r, c->part, c->fl);
+                k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ, tol2, maxit, c->sc, c->fl);
+            }
+            k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, c->fl);
+        }
+        HIPCHK(hipGetLastError());
+        launched += chunk;
+        if (int rc = read_flags(c, st)) return rc;
+        if (c->host_fl[0] || launched >= maxit) break;
+        chunk = std::min(2, maxit - launched);
+    }
+    *iters = c->host_fl[1];
+    last = std::max(1, *iters);
+    double b2 = c->host_sc[CG_B2];
+    *relres = b2 > 0 ? std::sqrt(c->host_sc[CG_RR] / b2) : 0.0;
+    if (c->host_fl[0] == 2) return ZJ_ESOLVE;
+    return ZJ_OK;
+}
+
+// FP64 residual r = rhs - S x and its relative norm (host-synchronous).
+static int residual64(zj_chorin* c, const double* rhs, const double* x, double* r, double* rel,
+                      hipStream_t st) {
+    zj::HLevel& L = c->H64.levels[0];
+    const int nr = nred(c->N);
+    k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, (double*)L.diag, (double*)L.cE,
+                                         (double*)L.cN, (double*)x, 0, rhs, r, c->part);
+    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_B2, 0.0, 1, c->sc, c->fl);
+    k_dot<double><<<nr, CB, 0, st>>>(c->N, r, r, c->part, nullptr);
+    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR0, 0.0, 1, c->sc, c->fl);
+    HIPCHK(hipGetLastError());
+    if (int rc = read_flags(c, st)) return rc;
+    double b2 = c->host_sc[CG_B2];
+    *rel = b2 > 0 ? std::sqrt(c->host_sc[CG_RR] / b2) : 0.0;
+    return ZJ_OK;
+}
+
+// Solve S p = rhs on the current hierarchy.  precond: plain FP64 MG-CG, or
+// FP32 flexible CG with the learned preconditioner inside FP64 iterative
+// refinement, redone plainly when the refined residual misses the target.
+static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStream_t st) {
+    zj_chorin_stats& S = c->stats;
+    const double tol = c->prm.tol;
+    int its = 0;
+    double rel = 0.0;
+    S.solves += 1;
+    if (precond == ZJ_PRECOND_LEARNED && c->net) {
+        // outer FP64 refinement from p = 0 (r = rhs)
+        int outer = 0, inner_total = 0, rc = ZJ_OK;
+        k_fill<double><<<nblk(c->N), CB, 0, st>>>(c->N, 0.0, p);
+        const double* res = rhs;
+        while (outer < c->prm.max_refine) {
+            k_cast<float><<<nblk(c->N), CB, 0, st>>>(c->N, res, c->r32);
+            int it = 0;
+            double rel32 = 0.0;
+            rc = pcg<float>(c, c->H32, c->r32, c->x32, 0, c->s32, c->z32, c->d32, c->q32,
+                            c->ro32, c->prm.inner_tol, c->prm.maxiter, 1, st, &it, &rel32);
+            inner_total += it;
+            if (rc && rc != ZJ_ESOLVE) return rc;
+            k_add_f32<<<nblk(c->N), CB, 0, st>>>(c->N, c->x32, p);
+            ++outer;
+            if ((rc = residual64(c, rhs, p, c->r, &rel, st))) return rc;
+            res = c->r;
+            if (rel <= tol) break;
+        }
+        S.learned_inner += inner_total;
+        S.learned_outer += outer;
+        S.learned_relres = std::max(S.learned_relres, rel);
+        if (rel <= tol) {
+            S.iterations += inner_total;
+            S.relres = std::max(S.relres, rel);
+            return ZJ_OK;
+        }
+        S.fallbacks += 1;  // redo plainly
+    }
+    int rc = pcg<double>(c, c->H64, rhs, p, 0, c->r, c->z, c->d, c->q, nullptr, tol,
+                         c->prm.maxiter, 0, st, &its, &rel);
+    if (rc && rc != ZJ_ESOLVE) return rc;
+    S.iterations += its;
+    S.relres = std::max(S.relres, rel);
+    if (rc == ZJ_ESOLVE || !(rel <= std::max(1e-6, 10.0 * tol))) return ZJ_ESOLVE;
+    return ZJ_OK;
+}
+
+static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStream_t st) {
+    zj::build_levels<double>(c->H64, h, b, c->eps, st);
+    if (c->prm.precond == ZJ_PRECOND_LEARNED) zj::build_levels<float>(c->H32, h, b, c->eps, st);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+static int stage(zj_chorin* c, double dt, const double* Qin, double* Qout, const double* Q0,
+                 hipStream_t st) {
+    const int n = c->N;
+    if (int rc = zj_flux_operator(c->fv, Qin, c->dQ, st)) return rc;
+    k_predict<<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Qin, c->dQ, c->Qs);
+    if (int rc = setup_operator(c, c->Qs, c->Qs + n, st)) return rc;
+    k_rhs<double><<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, c->Qs, c->rhs);
+    HIPCHK(hipGetLastError());
+    if (int rc = solve(c, c->rhs, c->p, c->prm.precond, st)) return rc;
+    k_correct<<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->Qs, c->p, Q0, Qout);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+}  // namespace
+
+// Accessors used by the learned-preconditioner translation unit.
+namespace zj {
+Hier& chorin_h32(zj_chorin* c) { return c->H32; }
+int chorin_nx(zj_chorin* c) { return c->nx; }
+int chorin_ny(zj_chorin* c) { return c->ny; }
+}  // namespace zj
+
+extern "C" {
+
+int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy,
+                     const zj_chorin_params* prm, zj_chorin** out) {
+    if (!out || nx < 2 || ny < 2 || !(dx > 0) || !(dy > 0) || !prm) return ZJ_EINVAL;
+    if (prm->tol <= 0 || prm->maxiter < 1 || prm->nu < 1 || prm->coarse_sweeps < 1 ||
+        prm->min_coarse < 1 || (prm->time_order != 1 && prm->time_order != 2))
+        return ZJ_EINVAL;
+    zj_chorin* c = new (std::nothrow) zj_chorin();
+    if (!c) return ZJ_ENOMEM;
+    c->fv = fv;
+    c->nx = nx;
+    c->ny = ny;
+    c->N = nx * ny;
+    c->dx = dx;
+    c->dy = dy;
+    c->eps = prm->eps_wet > 0 ? prm->eps_wet : 1e-8;
+    c->prm = *prm;
+    if (c->prm.max_refine < 1) c->prm.max_refine = 4;
+    if (!(c->prm.inner_tol > 0)) c->prm.inner_tol = 1e-4;
+    const size_t N = c->N;
+    int rc = c->H64.alloc(nx, ny, dx, dy, prm->min_coarse, sizeof(double));
+    if (!rc) rc = c->H32.alloc(nx, ny, dx, dy, prm->min_coarse, sizeof(float));
+    if (!rc) rc = dalloc(c, &c->Qs, 5 * N);
+    if (!rc) rc = dalloc(c, &c->Q1, 5 * N);
+    if (!rc) rc = dalloc(c, &c->dQ, 4 * N);
+    if (!rc) rc = dalloc(c, &c->rhs, N);
+    if (!rc) rc = dalloc(c, &c->p, N);
+    if (!rc) rc = dalloc(c, &c->r, N);
+    if (!rc) rc = dalloc(c, &c->z, N);
+    if (!rc) rc = dalloc(c, &c->d, N);
+    if (!rc) rc = dalloc(c, &c->q, N);
+    if (!rc) rc = dalloc(c, &c->r32, N);
+    if (!rc) rc = dalloc(c, &c->x32, N);
+    if (!rc) rc = dalloc(c, &c->s32, N);
+    if (!rc) rc = dalloc(c, &c->z32, N);
+    if (!rc) rc = dalloc(c, &c->d32, N);
+    if (!rc) rc = dalloc(c, &c->q32, N);
+    if (!rc) rc = dalloc(c, &c->ro32, N);
+    if (!rc) rc = dalloc(c, &c->part, 2 * RED_MAX);
+    if (!rc) rc = dalloc(c, &c->sc, CG_NSLOT);
+    if (!rc) rc = dalloc(c, &c->fl, 4);
+    if (!rc && hipHostMalloc((void**)&c->host_fl, 4 * sizeof(int)) != hipSuccess) rc = ZJ_ENOMEM;
+    if (!rc && hipHostMalloc((void**)&c->host_sc, CG_NSLOT * sizeof(double)) != hipSuccess)
+        rc = ZJ_ENOMEM;
+    if (rc) {
+        zj_chorin_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return ZJ_OK;
+}
+
+int zj_chorin_destroy(zj_chorin* c) {
+    if (!c) return ZJ_OK;
+    (void)hipDeviceSynchronize();
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->H64.release();
+    c->H32.release();
+    if (c->host_fl) (void)hipHostFree(c->host_fl);
+    if (c->host_sc) (void)hipHostFree(c->host_sc);
+    delete c;
+    return ZJ_OK;
+}
+
+int zj_chorin_set_precond(zj_chorin* c, int32_t precond, zj_mpnn* net) {
+    if (!c || (precond != ZJ_PRECOND_MG && precond != ZJ_PRECOND_LEARNED)) return ZJ_EINVAL;
+    if (precond == ZJ_PRECOND_LEARNED && !net) return ZJ_EINVAL;
+    c->prm.precond = precond;
+    c->net = net;
+    return ZJ_OK;
+}
+
+int zj_chorin_setup_operator(zj_chorin* c, const double* b, const double* h, void* stream) {
+    if (!c || !b || !h) return ZJ_EINVAL;
+    return setup_operator(c, b, h, (hipStream_t)stream);
+}
+
+int zj_chorin_apply(zj_chorin* c, const double* x, double* y, void* stream) {
+    if (!c || !x || !y) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    zj::HLevel& L = c->H64.levels[0];
+    k_spmv<double><<<nred(c->N), CB, 0, st>>>(L.nx, L.ny, (double*)L.diag, (double*)L.cE,
+                                              (double*)L.cN, x, y, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_chorin_vcycle(zj_chorin* c, const void* r, void* z, int32_t dtype, void* stream) {
+    if (!c || !r || !z) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == ZJ_F32)
+        zj::vcycle<float>(c->H32, (const float*)r, (float*)z, c->prm.nu, c->prm.coarse_sweeps,
+                          nullptr, st);
+    else
+        zj::vcycle<double>(c->H64, (const double*)r, (double*)z, c->prm.nu,
+                           c->prm.coarse_sweeps, nullptr, st);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_chorin_levels(zj_chorin* c, int32_t* n_levels, int32_t* dims_out, int32_t max_levels) {
+    if (!c || !n_levels) return ZJ_EINVAL;
+    *n_levels = (int32_t)c->H64.levels.size();
+    for (int l = 0; dims_out && l < *n_levels && l < max_levels; ++l) {
+        dims_out[2 * l] = c->H64.levels[l].nx;
+        dims_out[2 * l + 1] = c->H64.levels[l].ny;
+    }
+    return ZJ_OK;
+}
+
+int zj_chorin_solve(zj_chorin* c, const double* rhs, double* p, void* stream) {
+    if (!c || !rhs || !p) return ZJ_EINVAL;
+    c->stats = zj_chorin_stats{};
+    return solve(c, rhs, p, c->prm.precond, (hipStream_t)stream);
+}
+
+int zj_chorin_rhs(zj_chorin* c, double dt, const double* Qs, double* rhs, void* stream) {
+    if (!c || !Qs || !rhs || !(dt > 0)) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    k_rhs<double><<<nblk(c->N), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Qs, rhs);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_chorin_correct(zj_chorin* c, double dt, const double* Qs, const double* p,
+                      const double* Q0, double* Qout, void* stream) {
+    if (!c || !Qs || !p || !Qout || !(dt > 0)) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    k_correct<<<nblk(c->N), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, Qs, p, Q0, Qout);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_chorin_predict(zj_chorin* c, double dt, const double* Q, double* Qs, void* stream) {
+    if (!c || !Q || !Qs || !(dt > 0)) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = zj_flux_operator(c->fv, Q, c->dQ, st)) return rc;
+    k_predict<<<nblk(c->N), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q, c->dQ, Qs);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_chorin_stage(zj_chorin* c, double dt, const double* Q, double* Qout, const double* Q0,
+                    void* stream) {
+    if (!c || !Q || !Qout || !(dt > 0) || Q == Qout) return ZJ_EINVAL;
+    c->stats = zj_chorin_stats{};
+    return stage(c, dt, Q, Qout, Q0, (hipStream_t)stream);
+}
+
+int zj_chorin_step(zj_chorin* c, double dt, const double* Q, double* Qout, void* stream) {
+    if (!c || !Q || !Qout || !(dt > 0) || Q == Qout) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    c->stats = zj_chorin_stats{};
+    if (c->prm.time_order == 1) return stage(c, dt, Q, Qout, nullptr, st);
+    int rc = stage(c, dt, Q, c->Q1, nullptr, st);
+    if (rc) return rc;
+    return stage(c, dt, c->Q1, Qout, Q, st);
+}
+
+int zj_chorin_get_stats(zj_chorin* c, zj_chorin_stats* out) {
+    if (!c || !out) return ZJ_EINVAL;
+    *out = c->stats;
+    return ZJ_OK;
+}
+
+}  // extern "C"

@@ -11,7 +11,8 @@ import pathlib
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libzoomy_hip.so"
 
-ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP = 0, 1, 2, 3, 4
+ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP, ZJ_ESOLVE = 0, 1, 2, 3, 4, 5
+PRECOND = {"mg": 0, "learned": 1}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
@@ -45,6 +46,24 @@ class Params(ctypes.Structure):
                 ("manning_n", ctypes.c_double), ("order", ctypes.c_int32),
                 ("limiter", ctypes.c_int32), ("riemann", ctypes.c_int32),
                 ("recon", ctypes.c_int32), ("dtype", ctypes.c_int32)]
+
+
+class ChorinParams(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("eps_wet", ctypes.c_double), ("maxiter", ctypes.c_int32),
+                ("nu", ctypes.c_int32), ("coarse_sweeps", ctypes.c_int32),
+                ("min_coarse", ctypes.c_int32), ("precond", ctypes.c_int32),
+                ("time_order", ctypes.c_int32),
+                ("max_refine", ctypes.c_int32)]
+
+
+class ChorinStats(ctypes.Structure):
+    _fields_ = [("solves", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("fallbacks", ctypes.c_int32),
+                ("learned_inner", ctypes.c_int32),
+                ("learned_outer", ctypes.c_int32),
+                ("relres", ctypes.c_double),
+                ("learned_relres", ctypes.c_double)]
 
 
 _lib = None
@@ -82,8 +101,27 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_void_p]),
 }
 
-# the Poisson / Chorin / learned-preconditioner entry points
-_SIGS_POISSON = {}
+_V = ctypes.c_void_p
+_D = ctypes.c_double
+# the Chorin / Poisson / learned-preconditioner entry points
+_SIGS_POISSON = {
+    "zj_chorin_create": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,
+                                        _D, _D, ctypes.POINTER(ChorinParams),
+                                        ctypes.POINTER(_V)]),
+    "zj_chorin_destroy": (ctypes.c_int, [_V]),
+    "zj_chorin_set_precond": (ctypes.c_int, [_V, ctypes.c_int32, _V]),
+    "zj_chorin_step": (ctypes.c_int, [_V, _D, _V, _V, _V]),
+    "zj_chorin_stage": (ctypes.c_int, [_V, _D, _V, _V, _V, _V]),
+    "zj_chorin_predict": (ctypes.c_int, [_V, _D, _V, _V, _V]),
+    "zj_chorin_rhs": (ctypes.c_int, [_V, _D, _V, _V, _V]),
+    "zj_chorin_correct": (ctypes.c_int, [_V, _D, _V, _V, _V, _V, _V]),
+    "zj_chorin_setup_operator": (ctypes.c_int, [_V, _V, _V, _V]),
+    "zj_chorin_apply": (ctypes.c_int, [_V, _V, _V, _V]),
+    "zj_chorin_vcycle": (ctypes.c_int, [_V, _V, _V, ctypes.c_int32, _V]),
+    "zj_chorin_solve": (ctypes.c_int, [_V, _V, _V, _V]),
+    "zj_chorin_levels": (ctypes.c_int, [_V, _V, _V, ctypes.c_int32]),
+    "zj_chorin_get_stats": (ctypes.c_int, [_V, ctypes.POINTER(ChorinStats)]),
+}
 
 
 def exported_symbols():
@@ -115,5 +153,6 @@ def check(rc, what=""):
     if rc != ZJ_OK:
         names = {ZJ_EINVAL: "invalid argument", ZJ_EHIP: "HIP runtime error",
                  ZJ_ENOMEM: "out of device memory",
-                 ZJ_EUNSUP: "unsupported option"}
+                 ZJ_EUNSUP: "unsupported option",
+                 ZJ_ESOLVE: "pressure solve missed its accept gate"}
         raise ZoomyHipError(f"{what}: {names.get(rc, rc)} (code {rc})")
