@@ -135,6 +135,82 @@ def summarize_leg(r, w, N, steps):
         step_frac=cups * B["cell_update"] / PEAK_HBM)
 
 
+CHORIN_N = 1024
+CHORIN_L = 64.0
+
+
+def chorin_state(n=CHORIN_N, L=CHORIN_L):
+    """Solitary wave (amplitude 0.2, depth 1) over b = 0.1 sin(2 pi k.x +
+    phi), k and phi seeded (seed 4), on [0, L]^2 with walls."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    kx = rng.integers(1, 5) / L
+    ky = rng.integers(1, 5) / L
+    ph = rng.uniform(0.0, 2.0 * np.pi)
+    dx = L / n
+    X, Y = np.meshgrid((np.arange(n) + 0.5) * dx, (np.arange(n) + 0.5) * dx,
+                       indexing="ij")
+    a, d = 0.2, 1.0
+    k = np.sqrt(3.0 * a / (4.0 * d ** 3))
+    c = np.sqrt(9.81 * (d + a))
+    eta = a / np.cosh(k * (X - 0.25 * L)) ** 2
+    b = 0.1 * np.sin(2.0 * np.pi * (kx * X + ky * Y) + ph)
+    Q = np.zeros((5, n * n))
+    Q[0] = b.ravel()
+    Q[1] = (d + eta - b).ravel()
+    Q[2] = (c * eta).ravel()
+    return Q
+
+
+def chorin_leg(precond, steps, warmup, net=None, n=CHORIN_N, L=CHORIN_L):
+    """Fixed-dt Chorin steps (two projection stages each, SSP-RK2) on the
+    n x n solitary-wave case; pressure solves to 1e-8 relative residual."""
+    import numpy as np
+    import torch
+    from zoomy_hip.fvm import ChorinSplitVAMSolver, ShallowWater
+    from zoomy_hip.fvm.chorin import fixed_dt
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.quad(n, n, (0.0, L, 0.0, L)).set_bc("wall")
+    model = ShallowWater(dimension=2, order=2, limiter="minmod",
+                         riemann="hll")
+    s = ChorinSplitVAMSolver(model, tol=1e-8, precond=precond)
+    Q = s.setup_simulation(mesh, n, n, chorin_state(n, L))
+    if net is not None:
+        s.set_preconditioner(precond, net)
+    dt = fixed_dt(L / n, 1.0)
+    for _ in range(warmup):
+        Q = s.chorin_cycle(dt, 0.0, Q)
+    torch.cuda.synchronize()
+    its, outer, fb, worst = [], [], 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Q = s.chorin_cycle(dt, 0.0, Q)
+        st = s.stats()
+        its.append(st["iterations"] / max(st["solves"], 1))
+        outer.append(st["learned_outer"] / max(st["solves"], 1))
+        fb += st["fallbacks"]
+        worst = max(worst, st["relres"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # solve-only timing on the last predicted state
+    Qs = s._hyperbolic_stage(dt, 0.0, Q)
+    s.setup_operator(Qs[0], Qs[1])
+    rhs = s.pressure_rhs(Qs, dt)
+    s.solve(rhs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        s.solve(rhs)
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t1) * 1e3 / reps
+    return {"ms_per_step": el * 1e3 / steps, "solve_ms": solve_ms,
+            "iterations_per_solve": float(np.mean(its)),
+            "refinements_per_solve": float(np.mean(outer)),
+            "fallbacks": fb, "worst_relres": worst,
+            "finite": bool(torch.isfinite(Q).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +219,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--chorin-steps", type=int, default=10)
+    ap.add_argument("--no-chorin", action="store_true")
+    ap.add_argument("--chorin-only", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (10 times): "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -164,6 +243,22 @@ def main():
         sys.exit(2)
     import torch
     from zoomy_hip.mesh import Mesh
+
+    chorin = None
+    if not args.no_chorin:
+        chorin = {"workload": f"{CHORIN_N}x{CHORIN_N} solitary wave "
+                              "(amplitude 0.2, depth 1) over a seeded "
+                              "sinusoidal bed, fixed dt, SSP-RK2 projection, "
+                              "pressure to 1e-8 rel. residual",
+                  "plain": chorin_leg("mg", args.chorin_steps, 2)}
+        from zoomy_hip.learned import MPPreconditioner
+        chorin["learned"] = chorin_leg("learned", args.chorin_steps, 2,
+                                       net=MPPreconditioner.load())
+        chorin["learned_over_plain"] = (chorin["learned"]["ms_per_step"] /
+                                        chorin["plain"]["ms_per_step"])
+        if args.chorin_only:
+            print(json.dumps({"chorin": chorin}))
+            return
 
     mesh = Mesh.quad(NX, NY)
     # cells along the Morton curve: each 256-cell block is a 16 x 16 tile
@@ -215,6 +310,8 @@ def main():
                        "checks": {"finite": r32["finite"],
                                   "troubled": r32["troubled"],
                                   "mass": r32["mass"]}}
+    if chorin is not None:
+        out["chorin"] = chorin
     if cpu is not None:
         out["cpu_baseline"] = {
             "value": cpu["value"], "unit": "cell-updates/s", "cores": 1,

@@ -191,6 +191,44 @@ int zj_chorin_solve(zj_chorin *c, const double *rhs, double *p, void *stream);
 int zj_chorin_levels(zj_chorin *c, int32_t *n_levels, int32_t *dims_out, int32_t max_levels);
 /* Statistics of the last step / stage / solve call. */
 int zj_chorin_get_stats(zj_chorin *c, zj_chorin_stats *out);
+/* The learned V-cycle z = M r (FP32, level 0 of the current operator; needs
+ * the network set by zj_chorin_set_precond):
+ *   x0 = red-black sweep pair from 0;  x1 = x0 + N(r - S x0);
+ *   x2 = x1 + P Vc R (r - S x1);      z = x2 + N(r - S x2).
+ * Placement follows the reference's learned V-cycle smoother prior
+ * (zoomy_jax/gnn_blueprint/vcycle_structured_gnn.py:74-122, :145-200).
+ * train != 0 keeps the activations for the backward below. */
+int zj_chorin_learned_vcycle(zj_chorin *c, const float *r, float *z, int32_t train,
+                             void *stream);
+/* Accumulates into the network's gradient the parameter gradient of
+ * sum(g * z) for the last training zj_chorin_learned_vcycle. */
+int zj_chorin_learned_vcycle_backward(zj_chorin *c, const float *g, void *stream);
+
+/* ───────────────── learned message-passing preconditioner ─────────────────
+ * Network on 2x2 cell blocks (zoomy_jax/gnn_blueprint/graph_mp_multibranch.py
+ * :9-16 made linear in the residual): h0 = (r/diag) W_in; three layers
+ * h' = sigmoid(phi U_l) * ([h | sum_nbr h] W_l), hidden 64, BF16 MFMA;
+ * z = h3 W_out.  FP32 master parameters (zj_mpnn_param_count() floats in
+ * the order W_in (4x64) | W_l (128x64), U_l (16x64), l = 0..2 | W_out
+ * (64x4)); Adam as in the reference's training
+ * (train_vcycle_structured_poisson.py:266-312). */
+int64_t zj_mpnn_param_count(void);
+int zj_mpnn_create(zj_mpnn **out);
+int zj_mpnn_destroy(zj_mpnn *n);
+int zj_mpnn_set_params(zj_mpnn *n, const float *host, int64_t count);
+int zj_mpnn_get_params(zj_mpnn *n, float *host, int64_t count);
+int zj_mpnn_get_grad(zj_mpnn *n, float *host, int64_t count);
+int zj_mpnn_zero_grad(zj_mpnn *n, void *stream);
+int zj_mpnn_adam(zj_mpnn *n, float lr, float beta1, float beta2, float eps, void *stream);
+/* z (+)= N(r) on B stacked nx x ny grids with FP32 stencils (diag, cE, cN);
+ * slot 0/1 keep activations for zj_mpnn_backward, slot 2 does not. */
+int zj_mpnn_forward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *r,
+                    const float *diag, const float *cE, const float *cN, float *z,
+                    int32_t add, int32_t slot, void *stream);
+/* Accumulates the gradient of sum(dz * z) for the forward of `slot`; dr
+ * (may be NULL) receives the input gradient. */
+int zj_mpnn_backward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *dz,
+                     const float *diag, float *dr, int32_t slot, void *stream);
 
 /* Library identification string. */
 const char *zj_version(void);

@@ -603,15 +603,18 @@ static void build_levels(Hier& H, const double* h0, const double* b0, double eps
     }
 }
 
-// z = V(r) on the hierarchy; z and r are level-0 vectors (not L.x / L.f).
+// z = V(r) from level l0 down; r and z are the vectors of level l0 (on
+// level 0 they are caller vectors, on deeper levels L.f and L.x).
 template <typename T>
 static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const int* done,
-                   hipStream_t st) {
+                   hipStream_t st, int l0 = 0) {
     const int nl = (int)H.levels.size();
-    for (int l = 0; l < nl - 1; ++l) {
+    auto X = [&](int l) { return l == l0 ? z0 : (T*)H.levels[l].x; };
+    auto F = [&](int l) { return l == l0 ? r0 : (const T*)H.levels[l].f; };
+    for (int l = l0; l < nl - 1; ++l) {
         HLevel& L = H.levels[l];
-        T* x = l ? (T*)L.x : z0;
-        const T* f = l ? (const T*)L.f : r0;
+        T* x = X(l);
+        const T* f = F(l);
         const int half = L.nx * ((L.ny + 1) >> 1);
         for (int s = 0; s < nu; ++s) {
             k_rbgs<T><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, x,
@@ -627,16 +630,14 @@ static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const
     }
     {
         HLevel& C = H.levels[nl - 1];
-        T* x = nl > 1 ? (T*)C.x : z0;
-        const T* f = nl > 1 ? (const T*)C.f : r0;
-        k_coarse<T><<<1, CB, 0, st>>>(C.nx, C.ny, (T*)C.diag, (T*)C.cE, (T*)C.cN, x, f,
-                                      coarse_sweeps, done);
+        k_coarse<T><<<1, CB, 0, st>>>(C.nx, C.ny, (T*)C.diag, (T*)C.cE, (T*)C.cN, X(nl - 1),
+                                      F(nl - 1), coarse_sweeps, done);
     }
-    for (int l = nl - 2; l >= 0; --l) {
+    for (int l = nl - 2; l >= l0; --l) {
         HLevel& L = H.levels[l];
         HLevel& C = H.levels[l + 1];
-        T* x = l ? (T*)L.x : z0;
-        const T* f = l ? (const T*)L.f : r0;
+        T* x = X(l);
+        const T* f = F(l);
         const int n = L.nx * L.ny;
         k_prolong<T><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (T*)C.x, x, done);
         const int half = L.nx * ((L.ny + 1) >> 1);
@@ -647,6 +648,53 @@ static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const
                                                  f, 0, 0, done);
         }
     }
+}
+
+// Learned V-cycle on level 0 of the FP32 hierarchy:
+//   x0 = red-black sweep pair from zero
+//   x1 = x0 + N(r - S x0)
+//   x2 = x1 + P Vc R (r - S x1)        Vc: the V-cycle from level 1
+//   z  = x2 + N(r - S x2)
+// train != 0 keeps the network activations (slots 0 and 1) for
+// learned_vcycle_backward.  Needs at least two levels.
+static int learned_vcycle(Hier& H, zj_mpnn* net, const float* r, float* z, int nu,
+                          int coarse_sweeps, const int* done, int train, hipStream_t st) {
+    HLevel& L = H.levels[0];
+    HLevel& C = H.levels[1];
+    const float *D = (float*)L.diag, *E = (float*)L.cE, *Nn = (float*)L.cN;
+    float* t = (float*)L.r;
+    const int n = L.nx * L.ny, half = L.nx * ((L.ny + 1) >> 1);
+    k_rbgs<float><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, 0, 1, done);
+    k_rbgs<float><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, 1, 0, done);
+    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
+    if (int rc = mpnn_forward_f32(net, L.nx, L.ny, t, D, E, Nn, z, 1, train ? 0 : 2, done, st))
+        return rc;
+    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
+    k_restrict<float><<<nblk(C.nx * C.ny), CB, 0, st>>>(C.nx, C.ny, t, (float*)C.f, done);
+    vcycle<float>(H, (const float*)C.f, (float*)C.x, nu, coarse_sweeps, done, st, 1);
+    k_prolong<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (float*)C.x, z, done);
+    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
+    return mpnn_forward_f32(net, L.nx, L.ny, t, D, E, Nn, z, 1, train ? 1 : 2, done, st);
+}
+
+// Parameter gradients of sum(g * z) for the last training learned_vcycle:
+//   a3 = N2^T g;  dx2 = g - S a3;  dx1 = dx2 - S P Vc R dx2;  N1^T dx1
+// (R = P^T and Vc is symmetric).
+static int learned_vcycle_backward(Hier& H, zj_mpnn* net, const float* g, int nu,
+                                   int coarse_sweeps, hipStream_t st) {
+    HLevel& L = H.levels[0];
+    HLevel& C = H.levels[1];
+    const float *D = (float*)L.diag, *E = (float*)L.cE, *Nn = (float*)L.cN;
+    float *a3 = (float*)L.x, *dx2 = (float*)L.f, *v = (float*)L.r;
+    const int n = L.nx * L.ny;
+    if (int rc = mpnn_backward_f32(net, L.nx, L.ny, g, D, a3, 1, st)) return rc;
+    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, a3, g, dx2, nullptr);
+    k_restrict<float><<<nblk(C.nx * C.ny), CB, 0, st>>>(C.nx, C.ny, dx2, (float*)C.f, nullptr);
+    vcycle<float>(H, (const float*)C.f, (float*)C.x, nu, coarse_sweeps, nullptr, st, 1);
+    k_fill<float><<<nblk(n), CB, 0, st>>>(n, 0.f, v);
+    k_prolong<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (float*)C.x, v, nullptr);
+    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, v, dx2, a3, nullptr);
+    return mpnn_backward_f32(net, L.nx, L.ny, a3, D, nullptr, 0, st);
 }
 
 }  // namespace zj
@@ -707,7 +755,8 @@ static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z
     const double tol2 = tol * tol;
     auto precond = [&](const T* rin, T* zout) -> int {
         if (learned) {
-            int rc = zj::mpnn_apply_f32(c->net, c, (const float*)rin, (float*)zout, c->fl, st);
+            int rc = zj::learned_vcycle(H, c->net, (const float*)rin, (float*)zout, c->prm.nu,
+                                        c->prm.coarse_sweeps, c->fl, 0, st);
             if (rc) return rc;
         } else {
             zj::vcycle<T>(H, rin, zout, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
@@ -825,6 +874,7 @@ static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStr
 }
 
 static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStream_t st) {
+    zj::mpnn_invalidate_phi(c->net);
     zj::build_levels<double>(c->H64, h, b, c->eps, st);
     if (c->prm.precond == ZJ_PRECOND_LEARNED) zj::build_levels<float>(c->H32, h, b, c->eps, st);
     HIPCHK(hipGetLastError());
@@ -846,13 +896,6 @@ static int stage(zj_chorin* c, double dt, const double* Qin, double* Qout, const
 }
 
 }  // namespace
-
-// Accessors used by the learned-preconditioner translation unit.
-namespace zj {
-Hier& chorin_h32(zj_chorin* c) { return c->H32; }
-int chorin_nx(zj_chorin* c) { return c->nx; }
-int chorin_ny(zj_chorin* c) { return c->ny; }
-}  // namespace zj
 
 extern "C" {
 
@@ -953,6 +996,19 @@ int zj_chorin_vcycle(zj_chorin* c, const void* r, void* z, int32_t dtype, void* 
                            c->prm.coarse_sweeps, nullptr, st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
+}
+
+int zj_chorin_learned_vcycle(zj_chorin* c, const float* r, float* z, int32_t train,
+                             void* stream) {
+    if (!c || !r || !z || !c->net || c->H32.levels.size() < 2) return ZJ_EINVAL;
+    return zj::learned_vcycle(c->H32, c->net, r, z, c->prm.nu, c->prm.coarse_sweeps, nullptr,
+                              train, (hipStream_t)stream);
+}
+
+int zj_chorin_learned_vcycle_backward(zj_chorin* c, const float* g, void* stream) {
+    if (!c || !g || !c->net || c->H32.levels.size() < 2) return ZJ_EINVAL;
+    return zj::learned_vcycle_backward(c->H32, c->net, g, c->prm.nu, c->prm.coarse_sweeps,
+                                       (hipStream_t)stream);
 }
 
 int zj_chorin_levels(zj_chorin* c, int32_t* n_levels, int32_t* dims_out, int32_t max_levels) {

@@ -32,13 +32,18 @@ struct Hier {
     void release();
 };
 
-Hier& chorin_h32(zj_chorin* c);
-int chorin_nx(zj_chorin* c);
-int chorin_ny(zj_chorin* c);
-
-// z = M r with the learned preconditioner (FP32 vectors on level 0 of the
-// FP32 hierarchy); kernels return at once while *done != 0.
-int mpnn_apply_f32(zj_mpnn* net, zj_chorin* c, const float* r, float* z, const int* done,
-                   hipStream_t st);
+// Network of the learned preconditioner on an nx x ny grid with fine-level
+// FP32 stencil (diag, cE, cN): z (+)= N(r).  slot 0, 1 keep the activations
+// for mpnn_backward_f32; slot 2 is inference.  Kernels return at once while
+// *done != 0.
+int mpnn_forward_f32(zj_mpnn* net, int nx, int ny, const float* r, const float* diag,
+                     const float* cE, const float* cN, float* z, int add, int slot,
+                     const int* done, hipStream_t st);
+// accumulates the parameter gradients of sum(dz * N(r)); dr (may be NULL)
+// receives the input gradient
+int mpnn_backward_f32(zj_mpnn* net, int nx, int ny, const float* dz, const float* diag,
+                      float* dr, int slot, hipStream_t st);
+// the operator features must be recomputed (stencil rebuilt in place)
+void mpnn_invalidate_phi(zj_mpnn* net);
 
 }  // namespace zj

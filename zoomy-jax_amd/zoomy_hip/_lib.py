@@ -121,6 +121,24 @@ _SIGS_POISSON = {
     "zj_chorin_solve": (ctypes.c_int, [_V, _V, _V, _V]),
     "zj_chorin_levels": (ctypes.c_int, [_V, _V, _V, ctypes.c_int32]),
     "zj_chorin_get_stats": (ctypes.c_int, [_V, ctypes.POINTER(ChorinStats)]),
+    "zj_chorin_learned_vcycle": (ctypes.c_int, [_V, _V, _V, ctypes.c_int32,
+                                                _V]),
+    "zj_chorin_learned_vcycle_backward": (ctypes.c_int, [_V, _V, _V]),
+    "zj_mpnn_param_count": (ctypes.c_int64, []),
+    "zj_mpnn_create": (ctypes.c_int, [ctypes.POINTER(_V)]),
+    "zj_mpnn_destroy": (ctypes.c_int, [_V]),
+    "zj_mpnn_set_params": (ctypes.c_int, [_V, _V, ctypes.c_int64]),
+    "zj_mpnn_get_params": (ctypes.c_int, [_V, _V, ctypes.c_int64]),
+    "zj_mpnn_get_grad": (ctypes.c_int, [_V, _V, ctypes.c_int64]),
+    "zj_mpnn_zero_grad": (ctypes.c_int, [_V, _V]),
+    "zj_mpnn_adam": (ctypes.c_int, [_V, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, _V]),
+    "zj_mpnn_forward": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, _V, _V, _V, _V, _V,
+                                       ctypes.c_int32, ctypes.c_int32, _V]),
+    "zj_mpnn_backward": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, _V, _V, _V,
+                                        ctypes.c_int32, _V]),
 }
 
 
@@ -143,10 +161,6 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
-
-
-def register(name, restype, argtypes):
-    _SIGS_POISSON[name] = (restype, argtypes)
 
 
 def check(rc, what=""):

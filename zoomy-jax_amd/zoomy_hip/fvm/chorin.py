@@ -244,6 +244,26 @@ class ChorinSplitVAMSolver:
             _stream()), "zj_chorin_vcycle")
         return z
 
+    def learned_vcycle(self, r, train=False):
+        """z = M r with the learned V-cycle (FP32; the learned
+        preconditioner must be set)."""
+        if r.dtype != torch.float32 or r.numel() != self.N or \
+                not r.is_contiguous():
+            raise ValueError("r must be a contiguous float32 vector")
+        z = torch.empty_like(r)
+        self._check(L.lib().zj_chorin_learned_vcycle(
+            self.handle, _p(r), _p(z), int(bool(train)), _stream()),
+            "zj_chorin_learned_vcycle")
+        return z
+
+    def learned_vcycle_backward(self, g):
+        if g.dtype != torch.float32 or g.numel() != self.N or \
+                not g.is_contiguous():
+            raise ValueError("g must be a contiguous float32 vector")
+        self._check(L.lib().zj_chorin_learned_vcycle_backward(
+            self.handle, _p(g), _stream()),
+            "zj_chorin_learned_vcycle_backward")
+
     def solve(self, rhs):
         p = torch.empty_like(self._v(rhs))
         self._check(L.lib().zj_chorin_solve(self.handle, _p(rhs), _p(p),
