@@ -161,6 +161,14 @@ typedef struct {
 int zj_chorin_create(zj_solver *fv, int32_t nx, int32_t ny, double dx, double dy,
                      const zj_chorin_params *prm, zj_chorin **out);
 int zj_chorin_destroy(zj_chorin *c);
+/* Cell order of the FV solver fv (host array, a permutation): FV cell k is
+ * grid cell order[k] (e.g. a Morton order, so that the predictor's patch
+ * kernel gathers from compact tiles).  NULL: the grid order.  No counterpart
+ * in the reference (its Chorin solver has no device mesh layout). */
+int zj_chorin_set_fv_order(zj_chorin *c, const int32_t *order);
+/* Fused multigrid kernels on/off (default on; used when nu == 1).  Both
+ * give bit-identical V-cycles.  No counterpart in the reference. */
+int zj_chorin_set_mg_fused(zj_chorin *c, int32_t on);
 /* Select the preconditioner; net must be non-NULL for ZJ_PRECOND_LEARNED. */
 int zj_chorin_set_precond(zj_chorin *c, int32_t precond, zj_mpnn *net);
 /* chorin_cycle (solver_chorin_vam_jax.py:574-615): one step of time_order

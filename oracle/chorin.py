@@ -249,7 +249,7 @@ def rbgs(coef, x, f, colour):
     return x
 
 
-def vcycle(H: Hierarchy, f, level=0, nu=1, coarse_sweeps=8):
+def vcycle(H: Hierarchy, f, level=0, nu=1, coarse_sweeps=2):
     L = H.levels[level]
     coef = L["coef"]
     x = np.zeros_like(f)
@@ -301,7 +301,7 @@ def pcg(coef, rhs, precond=None, tol=1e-8, maxiter=500, x0=None):
     return x, it, hist
 
 
-def mg_solve(h, b, dx, dy, rhs, tol=1e-8, nu=1, coarse_sweeps=8,
+def mg_solve(h, b, dx, dy, rhs, tol=1e-8, nu=1, coarse_sweeps=2,
              min_size=4, maxiter=200):
     H = Hierarchy(h, b, dx, dy, min_size=min_size)
     return pcg(H.levels[0]["coef"], rhs,

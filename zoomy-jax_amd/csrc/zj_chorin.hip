@@ -219,19 +219,452 @@ __global__ __launch_bounds__(CB) void k_coarse(int nx, int ny, const T* __restri
     }
 }
 
+// ───────────────────────── fused multigrid kernels (nu = 1) ────────────────
+// Same arithmetic, in the same order, as the separate kernels above, so the
+// fused V-cycle is bit-identical to the unfused one.
+constexpr int TC = 16;               // coarse cells per tile side
+constexpr int WD = 2 * TC + 6;       // down-kernel fine window side (halo 3)
+constexpr int WU = 2 * TC + 4;       // up-kernel fine window side (halo 2)
+constexpr int WC = TC + 4;           // up-kernel coarse window side
+constexpr int TAIL_CELLS = 1400;     // cells of all tail levels together
+constexpr int TAIL_MAXL = 10;
+constexpr int TB = 512;              // tail kernel threads
+
+// offdiag on an LDS window of side W: cell (wi, wj) at global (gi, gj)
+template <typename T, int W>
+__device__ __forceinline__ T offdiag_lds(int wi, int wj, int gi, int gj, int nx, int ny,
+                                         const T* se, const T* sn, const T* sx) {
+    const int w = wi * W + wj;
+    T s = T(0);
+    if (gi + 1 < nx) s += se[w] * sx[w + W];
+    if (gi > 0) s += se[w - W] * sx[w - W];
+    if (gj + 1 < ny) s += sn[w] * sx[w + 1];
+    if (gj > 0) s += sn[w - 1] * sx[w - 1];
+    return s;
+}
+
+// Pre-smoothing from zero (red, black), residual and restriction of one
+// tile of TC x TC coarse cells; writes the smoothed fine iterate of the
+// tile's 2TC x 2TC fine cells and the coarse right-hand side (f is only
+// read, x and fc only written, so tiles do not interact).
+template <typename T>
+__global__ __launch_bounds__(CB) void k_mg_down(int nx, int ny, const T* __restrict__ diag,
+                                                const T* __restrict__ cE,
+                                                const T* __restrict__ cN,
+                                                const T* __restrict__ f, T* __restrict__ x,
+                                                T* __restrict__ fc,
+                                                const int* __restrict__ done) {
+    if (is_done(done)) return;
+    __shared__ T sf[WD * WD], sd[WD * WD], se[WD * WD], sn[WD * WD], sx[WD * WD];
+    const int nxc = nx >> 1, nyc = ny >> 1;
+    const int I0 = blockIdx.x * TC, J0 = blockIdx.y * TC;
+    const int gi0 = 2 * I0 - 3, gj0 = 2 * J0 - 3;
+    const int tid = threadIdx.x;
+    {
+        // all loads of the window first, then the LDS stores
+        constexpr int NIT = (WD * WD + CB - 1) / CB;
+        T vf[NIT], vd[NIT], ve[NIT], vn[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int w = tid + k * CB;
+            const int wi = w / WD, wj = w - wi * WD;
+            const int gi = gi0 + wi, gj = gj0 + wj;
+            vf[k] = vd[k] = ve[k] = vn[k] = T(0);
+            if (w < WD * WD && gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+                const int c = gi * ny + gj;
+                vf[k] = f[c];
+                vd[k] = diag[c];
+                ve[k] = cE[c];
+                vn[k] = cN[c];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int w = tid + k * CB;
+            if (w >= WD * WD) continue;
+            const int wi = w / WD, wj = w - wi * WD;
+            const int gi = gi0 + wi, gj = gj0 + wj;
+            const bool in = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+            sf[w] = vf[k];
+            sd[w] = vd[k];
+            se[w] = ve[k];
+            sn[w] = vn[k];
+            sx[w] = (!in || ((gi + gj) & 1)) ? T(0) : vf[k] / vd[k];
+        }
+    }
+    __syncthreads();
+    // black sweep on [1, WD-2]^2 (reads only red values)
+    for (int w = tid; w < WD * WD; w += CB) {
+        const int wi = w / WD, wj = w - wi * WD;
+        const int gi = gi0 + wi, gj = gj0 + wj;
+        if (wi < 1 || wi > WD - 2 || wj < 1 || wj > WD - 2) continue;
+        if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || !((gi + gj) & 1)) continue;
+        sx[w] = (sf[w] - offdiag_lds<T, WD>(wi, wj, gi, gj, nx, ny, se, sn, sx)) / sd[w];
+    }
+    __syncthreads();
+    // residual on [2, WD-3]^2, in place of f
+    for (int w = tid; w < WD * WD; w += CB) {
+        const int wi = w / WD, wj = w - wi * WD;
+        const int gi = gi0 + wi, gj = gj0 + wj;
+        if (wi < 2 || wi > WD - 3 || wj < 2 || wj > WD - 3) continue;
+        if (gi < 0 || gi >= nx || gj < 0 || gj >= ny) continue;
+        const T r = sf[w] - (sd[w] * sx[w] + offdiag_lds<T, WD>(wi, wj, gi, gj, nx, ny, se, sn, sx));
+        sf[w] = r;
+    }
+    __syncthreads();
+    {
+        const int I = I0 + tid / TC, J = J0 + (tid % TC);
+        if (I < nxc && J < nyc) {
+            const int bi = 2 * (I - I0) + 3, bj = 2 * (J - J0) + 3;
+            double s = 0.0;
+#pragma unroll
+            for (int di = -1; di <= 2; ++di) {
+                double wx = rw(I, di, nxc);
+                if (wx == 0.0) continue;
+                const T* row = sf + (bi + di) * WD + bj;
+                double sr = 0.0;
+#pragma unroll
+                for (int dj = -1; dj <= 2; ++dj) {
+                    double wy = rw(J, dj, nyc);
+                    if (wy != 0.0) sr += wy * (double)row[dj];
+                }
+                s += wx * sr;
+            }
+            fc[I * nyc + J] = (T)s;
+        }
+    }
+    for (int u = tid; u < 4 * TC * TC; u += CB) {
+        const int li = u / (2 * TC), lj = u - li * (2 * TC);
+        const int gi = 2 * I0 + li, gj = 2 * J0 + lj;
+        if (gi < nx && gj < ny) x[gi * ny + gj] = sx[(li + 3) * WD + lj + 3];
+    }
+}
+
+// Prolongation of the coarse correction and post-smoothing (black, red) of
+// one tile of 2TC x 2TC fine cells: reads the pre-smoothed iterate xin
+// (tile and halo), writes the tile of x (xin and x must not alias).
+template <typename T>
+__global__ __launch_bounds__(CB) void k_mg_up(int nx, int ny, const T* __restrict__ diag,
+                                              const T* __restrict__ cE,
+                                              const T* __restrict__ cN,
+                                              const T* __restrict__ f,
+                                              const T* __restrict__ xin, T* __restrict__ x,
+                                              const T* __restrict__ xc,
+                                              const int* __restrict__ done) {
+    if (is_done(done)) return;
+    __shared__ T sf[WU * WU], sd[WU * WU], se[WU * WU], sn[WU * WU], sx[WU * WU];
+    __shared__ T sc[WC * WC];
+    const int nxc = nx >> 1, nyc = ny >> 1;
+    const int I0 = blockIdx.x * TC, J0 = blockIdx.y * TC;
+    const int gi0 = 2 * I0 - 2, gj0 = 2 * J0 - 2;
+    const int ci0 = I0 - 2, cj0 = J0 - 2;
+    const int tid = threadIdx.x;
+    {
+        constexpr int NC = (WC * WC + CB - 1) / CB;
+        constexpr int NIT = (WU * WU + CB - 1) / CB;
+        T vc[NC], vf[NIT], vd[NIT], ve[NIT], vn[NIT], vx[NIT];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int w = tid + k * CB;
+            const int a = w / WC, b = w - a * WC;
+            const int I = ci0 + a, J = cj0 + b;
+            vc[k] = (w < WC * WC && I >= 0 && I < nxc && J >= 0 && J < nyc) ? xc[I * nyc + J]
+                                                                              : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int w = tid + k * CB;
+            const int wi = w / WU, wj = w - wi * WU;
+            const int gi = gi0 + wi, gj = gj0 + wj;
+            vf[k] = vd[k] = ve[k] = vn[k] = vx[k] = T(0);
+            if (w < WU * WU && gi >= 0 && gi < nx && gj >= 0 && gj < ny) {
+                const int c = gi * ny + gj;
+                vf[k] = f[c];
+                vd[k] = diag[c];
+                ve[k] = cE[c];
+                vn[k] = cN[c];
+                vx[k] = xin[c];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int w = tid + k * CB;
+            if (w < WC * WC) sc[w] = vc[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int w = tid + k * CB;
+            if (w >= WU * WU) continue;
+            sf[w] = vf[k];
+            sd[w] = vd[k];
+            se[w] = ve[k];
+            sn[w] = vn[k];
+            sx[w] = vx[k];
+        }
+    }
+    __syncthreads();
+    for (int w = tid; w < WU * WU; w += CB) {
+        const int wi = w / WU, wj = w - wi * WU;
+        const int i = gi0 + wi, j = gj0 + wj;
+        if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
+        int I = i >> 1, Io = I + ((i & 1) ? 1 : -1);
+        int J = j >> 1, Jo = J + ((j & 1) ? 1 : -1);
+        double wI = 1.0, wIo = 0.0, wJ = 1.0, wJo = 0.0;
+        if (Io >= 0 && Io < nxc) { wI = 0.75; wIo = 0.25; } else Io = I;
+        if (Jo >= 0 && Jo < nyc) { wJ = 0.75; wJo = 0.25; } else Jo = J;
+        const int a = I - ci0, ao = Io - ci0, b = J - cj0, bo = Jo - cj0;
+        double v = wI * (wJ * (double)sc[a * WC + b] + wJo * (double)sc[a * WC + bo]) +
+                   wIo * (wJ * (double)sc[ao * WC + b] + wJo * (double)sc[ao * WC + bo]);
+        sx[w] = (T)((double)sx[w] + v);
+    }
+    __syncthreads();
+    for (int colour = 1; colour >= 0; --colour) {
+        const int lo = colour ? 1 : 2, hi = colour ? WU - 2 : WU - 3;
+        for (int w = tid; w < WU * WU; w += CB) {
+            const int wi = w / WU, wj = w - wi * WU;
+            const int gi = gi0 + wi, gj = gj0 + wj;
+            if (wi < lo || wi > hi || wj < lo || wj > hi) continue;
+            if (gi < 0 || gi >= nx || gj < 0 || gj >= ny || ((gi + gj) & 1) != colour) continue;
+            sx[w] = (sf[w] - offdiag_lds<T, WU>(wi, wj, gi, gj, nx, ny, se, sn, sx)) / sd[w];
+        }
+        __syncthreads();
+    }
+    for (int u = tid; u < 4 * TC * TC; u += CB) {
+        const int li = u / (2 * TC), lj = u - li * (2 * TC);
+        const int gi = 2 * I0 + li, gj = 2 * J0 + lj;
+        if (gi < nx && gj < ny) x[gi * ny + gj] = sx[(li + 2) * WU + lj + 2];
+    }
+}
+
+template <typename T>
+struct TailDesc {
+    int nl;
+    int nx[TAIL_MAXL], ny[TAIL_MAXL];
+    const T *diag[TAIL_MAXL], *cE[TAIL_MAXL], *cN[TAIL_MAXL];
+    const T* f;  // right-hand side of the first tail level
+    T* x;        // its solution
+};
+
+// The V-cycle of the last levels (together <= TAIL_CELLS cells) in one
+// workgroup with every array in LDS.  Level sizes and LDS offsets live in
+// LDS too (a runtime-indexed kernel-argument array would go to scratch).
+template <typename T>
+__global__ __launch_bounds__(TB) void k_mg_tail(TailDesc<T> d, int coarse_sweeps,
+                                                const int* __restrict__ done) {
+    if (is_done(done)) return;
+    extern __shared__ __align__(16) unsigned char tail_lds[];
+    __shared__ int snx[TAIL_MAXL], sny[TAIL_MAXL], soff[TAIL_MAXL];
+    T* base = (T*)tail_lds;
+    const int tid = threadIdx.x;
+    const int nl = d.nl;
+    {
+        int off = 0;
+#pragma unroll
+        for (int l = 0; l < TAIL_MAXL; ++l) {
+            if (l >= nl) break;
+            const int n = d.nx[l] * d.ny[l];
+            if (tid == 0) {
+                snx[l] = d.nx[l];
+                sny[l] = d.ny[l];
+                soff[l] = off;
+            }
+            T* SD = base + off;
+            for (int c = tid; c < n; c += TB) {
+                SD[c] = d.diag[l][c];
+                SD[n + c] = d.cE[l][c];
+                SD[2 * n + c] = d.cN[l][c];
+                if (l == 0) SD[4 * n + c] = d.f[c];
+            }
+            off += 5 * n;
+        }
+    }
+    __syncthreads();
+    // arrays of level l: diag, cE, cN, x, f at base + soff[l] + k n
+    auto arr = [&](int l, int k) { return base + soff[l] + k * snx[l] * sny[l]; };
+    auto od = [&](int nx, int ny, const T* SE, const T* SN, int c, int i, int j, const T* xv) {
+        T s = T(0);
+        if (i + 1 < nx) s += SE[c] * xv[c + ny];
+        if (i > 0) s += SE[c - ny] * xv[c - ny];
+        if (j + 1 < ny) s += SN[c] * xv[c + 1];
+        if (j > 0) s += SN[c - 1] * xv[c - 1];
+        return s;
+    };
+    const int L = nl - 1;
+    for (int l = 0; l < L; ++l) {
+        const int nx = snx[l], ny = sny[l], n = nx * ny;
+        const T *SD = arr(l, 0), *SE = arr(l, 1), *SN = arr(l, 2);
+        T *SX = arr(l, 3), *SF = arr(l, 4);
+        for (int c = tid; c < n; c += TB) {
+            const int i = c / ny, j = c - i * ny;
+            if (!((i + j) & 1)) SX[c] = SF[c] / SD[c];
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += TB) {
+            const int i = c / ny, j = c - i * ny;
+            if ((i + j) & 1) SX[c] = (SF[c] - od(nx, ny, SE, SN, c, i, j, SX)) / SD[c];
+        }
+        __syncthreads();
+        const int nxc = snx[l + 1], nyc = sny[l + 1];
+        T* SFc = arr(l + 1, 4);
+        for (int cc = tid; cc < nxc * nyc; cc += TB) {
+            const int I = cc / nyc, J = cc - I * nyc;
+            double s = 0.0;
+            for (int di = -1; di <= 2; ++di) {
+                double wx = rw(I, di, nxc);
+                if (wx == 0.0) continue;
+                const int i = 2 * I + di;
+                double sr = 0.0;
+                for (int dj = -1; dj <= 2; ++dj) {
+                    double wy = rw(J, dj, nyc);
+                    if (wy == 0.0) continue;
+                    const int j = 2 * J + dj, c = i * ny + j;
+                    const T r = SF[c] - (SD[c] * SX[c] + od(nx, ny, SE, SN, c, i, j, SX));
+                    sr += wy * (double)r;
+                }
+                s += wx * sr;
+            }
+            SFc[cc] = (T)s;
+        }
+        __syncthreads();
+    }
+    {
+        const int nx = snx[L], ny = sny[L], n = nx * ny;
+        const T *SD = arr(L, 0), *SE = arr(L, 1), *SN = arr(L, 2), *SF = arr(L, 4);
+        T* SX = arr(L, 3);
+        for (int c = tid; c < n; c += TB) SX[c] = T(0);
+        __syncthreads();
+        for (int sw = 0; sw < coarse_sweeps; ++sw)
+            for (int q = 0; q < 4; ++q) {
+                const int colour = (q == 0 || q == 3) ? 0 : 1;
+                for (int c = tid; c < n; c += TB) {
+                    const int i = c / ny, j = c - i * ny;
+                    if (((i + j) & 1) != colour) continue;
+                    SX[c] = (SF[c] - od(nx, ny, SE, SN, c, i, j, SX)) / SD[c];
+                }
+                __syncthreads();
+            }
+    }
+    for (int l = L - 1; l >= 0; --l) {
+        const int nx = snx[l], ny = sny[l], n = nx * ny;
+        const int nxc = snx[l + 1], nyc = sny[l + 1];
+        const T *SD = arr(l, 0), *SE = arr(l, 1), *SN = arr(l, 2), *SF = arr(l, 4);
+        T* SX = arr(l, 3);
+        const T* xc = arr(l + 1, 3);
+        for (int c = tid; c < n; c += TB) {
+            const int i = c / ny, j = c - i * ny;
+            int I = i >> 1, Io = I + ((i & 1) ? 1 : -1);
+            int J = j >> 1, Jo = J + ((j & 1) ? 1 : -1);
+            double wI = 1.0, wIo = 0.0, wJ = 1.0, wJo = 0.0;
+            if (Io >= 0 && Io < nxc) { wI = 0.75; wIo = 0.25; } else Io = I;
+            if (Jo >= 0 && Jo < nyc) { wJ = 0.75; wJo = 0.25; } else Jo = J;
+            double v = wI * (wJ * (double)xc[I * nyc + J] + wJo * (double)xc[I * nyc + Jo]) +
+                       wIo * (wJ * (double)xc[Io * nyc + J] + wJo * (double)xc[Io * nyc + Jo]);
+            SX[c] = (T)((double)SX[c] + v);
+        }
+        __syncthreads();
+        for (int colour = 1; colour >= 0; --colour) {
+            for (int c = tid; c < n; c += TB) {
+                const int i = c / ny, j = c - i * ny;
+                if (((i + j) & 1) != colour) continue;
+                SX[c] = (SF[c] - od(nx, ny, SE, SN, c, i, j, SX)) / SD[c];
+            }
+            __syncthreads();
+        }
+    }
+    const int n0 = snx[0] * sny[0];
+    const T* SX0 = arr(0, 3);
+    for (int c = tid; c < n0; c += TB) d.x[c] = SX0[c];
+}
+
 // ───────────────────────── CG kernels ──────────────────────────────────────
-__device__ __forceinline__ void block_sum_store(double v, double* part) {
+// Every reduction kernel stores one partial per block; k_fin folds them in
+// a fixed order and performs the scalar step of CG.  (Folding them in the
+// last block of the reduction kernel needs a device-scope release fence per
+// block, which writes back the L2 of every XCD and costs more than the
+// extra launch.)
+struct Red {
+    double* part;       // partials (2 x RED_MAX)
+    double* sc;         // CG_* slots
+    int* fl;            // {done, iterations}
+    double tol2;
+    int maxit;
+    int op;             // FIN_* step
+};
+
+__device__ void scalar_step(const Red& R, double v, double v2) {
+    int* fl = R.fl;
+    double* sc = R.sc;
+    if (R.op != FIN_B2 && fl[0]) return;
+    switch (R.op) {
+        case FIN_B2:
+            sc[CG_B2] = v;
+            sc[CG_RR] = v;
+            sc[CG_BETA] = 0.0;
+            fl[1] = 0;
+            fl[0] = !(v > 0.0);
+            break;
+        case FIN_RR0:  // residual of a non-zero initial guess
+            sc[CG_RR] = v;
+            if (v <= R.tol2 * sc[CG_B2]) fl[0] = 1;
+            break;
+        case FIN_RZ0:
+            sc[CG_RZ] = v;
+            sc[CG_BETA] = 0.0;
+            break;
+        case FIN_PQ:
+            sc[CG_ALPHA] = sc[CG_RZ] / v;
+            if (!(v > 0.0)) fl[0] = 2;  // breakdown: not positive definite
+            break;
+        case FIN_RR:
+            sc[CG_RR] = v;
+            fl[1] += 1;
+            if (v <= R.tol2 * sc[CG_B2] || fl[1] >= R.maxit) fl[0] = 1;
+            break;
+        case FIN_RZ:
+            sc[CG_BETA] = v / sc[CG_RZ];
+            sc[CG_RZ] = v;
+            break;
+        case FIN_RZ_FLEX:  // Polak-Ribiere: beta = z.(r - r_old) / rz_old
+            sc[CG_BETA] = (v - v2) / sc[CG_RZ];
+            sc[CG_RZ] = v;
+            break;
+    }
+}
+
+// per-block partial sums of v (and v2 when two), folded by k_fin
+__device__ void reduce_finish(const Red& R, double v, double v2, bool two) {
     __shared__ double lds[CB / 64];
     auto add = [](double a, double b) { return a + b; };
     v = block_reduce<double, decltype(add), CB>(v, add, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = v;
+    __syncthreads();
+    if (two) v2 = block_reduce<double, decltype(add), CB>(v2, add, lds);
+    if (threadIdx.x == 0) {
+        R.part[blockIdx.x] = v;
+        if (two) R.part[gridDim.x + blockIdx.x] = v2;
+    }
+}
+
+// Sums the np partials of the preceding reduction in a fixed order and
+// performs the scalar step of CG.
+__global__ __launch_bounds__(RED_MAX) void k_fin(Red R, int np, int two) {
+    if (R.op != FIN_B2 && R.fl[0]) return;
+    __shared__ double lds[RED_MAX / 64];
+    auto add = [](double a, double b) { return a + b; };
+    double v = 0.0, v2 = 0.0;
+    for (int i = threadIdx.x; i < np; i += RED_MAX) v += R.part[i];
+    v = block_reduce<double, decltype(add), RED_MAX>(v, add, lds);
+    if (two) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < np; i += RED_MAX) v2 += R.part[np + i];
+        v2 = block_reduce<double, decltype(add), RED_MAX>(v2, add, lds);
+    }
+    if (threadIdx.x == 0) scalar_step(R, v, v2);
 }
 
 template <typename T>
 __global__ __launch_bounds__(CB) void k_spmv(int nx, int ny, const T* __restrict__ diag,
                                              const T* __restrict__ cE, const T* __restrict__ cN,
-                                             const T* __restrict__ x, T* __restrict__ y,
-                                             double* __restrict__ part,
+                                             const T* __restrict__ x, T* __restrict__ y, Red R,
                                              const int* __restrict__ done) {
     if (is_done(done)) return;
     const int n = nx * ny;
@@ -242,31 +675,28 @@ __global__ __launch_bounds__(CB) void k_spmv(int nx, int ny, const T* __restrict
         y[c] = v;
         acc += (double)x[c] * (double)v;
     }
-    if (part) block_sum_store(acc, part);
+    if (R.part) reduce_finish(R, acc, 0.0, false);
 }
 
 template <typename T>
 __global__ __launch_bounds__(CB) void k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
-                                            double* __restrict__ part,
-                                            const int* __restrict__ done) {
+                                            Red R, const int* __restrict__ done) {
     if (is_done(done)) return;
     double acc = 0.0;
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB)
         acc += (double)a[c] * (double)b[c];
-    block_sum_store(acc, part);
+    reduce_finish(R, acc, 0.0, false);
 }
 
 // x += alpha d, r -= alpha q (r_old keeps r before the update when given);
-// partial sums of r.r
+// sum of r.r
 template <typename T>
 __global__ __launch_bounds__(CB) void k_axpy2(int n, T* __restrict__ x, T* __restrict__ r,
                                               T* __restrict__ r_old, const T* __restrict__ d,
-                                              const T* __restrict__ q,
-                                              const double* __restrict__ sc,
-                                              double* __restrict__ part,
+                                              const T* __restrict__ q, Red R,
                                               const int* __restrict__ done) {
     if (is_done(done)) return;
-    const T alpha = (T)sc[CG_ALPHA];
+    const T alpha = (T)R.sc[CG_ALPHA];
     double acc = 0.0;
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
         x[c] += alpha * d[c];
@@ -276,28 +706,28 @@ __global__ __launch_bounds__(CB) void k_axpy2(int n, T* __restrict__ x, T* __res
         r[c] = rv;
         acc += (double)rv * (double)rv;
     }
-    block_sum_store(acc, part);
+    reduce_finish(R, acc, 0.0, false);
 }
 
-// d = z + beta d
+// d = z + beta d; first != 0: d = z (d holds no value yet: 0 * d could
+// be NaN)
 template <typename T>
 __global__ __launch_bounds__(CB) void k_xpby(int n, const T* __restrict__ z, T* __restrict__ d,
-                                             const double* __restrict__ sc,
+                                             const double* __restrict__ sc, int first,
                                              const int* __restrict__ done) {
     if (is_done(done)) return;
     const T beta = (T)sc[CG_BETA];
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB)
-        d[c] = z[c] + beta * d[c];
+        d[c] = first ? z[c] : z[c] + beta * d[c];
 }
 
-// r = rhs - S x (x may be NULL: zero initial guess, then x is zeroed);
-// partial sums of rhs.rhs
+// r = rhs - S x (zero_x: zero initial guess, x is zeroed); sum of rhs.rhs
 template <typename T>
 __global__ __launch_bounds__(CB) void k_cg_init(int nx, int ny, const T* __restrict__ diag,
                                                 const T* __restrict__ cE,
                                                 const T* __restrict__ cN, T* __restrict__ x,
                                                 int zero_x, const T* __restrict__ rhs,
-                                                T* __restrict__ r, double* __restrict__ part) {
+                                                T* __restrict__ r, Red R) {
     const int n = nx * ny;
     double acc = 0.0;
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
@@ -311,15 +741,14 @@ __global__ __launch_bounds__(CB) void k_cg_init(int nx, int ny, const T* __restr
         }
         acc += (double)f * (double)f;
     }
-    block_sum_store(acc, part);
+    reduce_finish(R, acc, 0.0, false);
 }
 
 // Two sums (a.b, a.c) in one pass (flexible CG: z.r and z.r_old).
 template <typename T>
 __global__ __launch_bounds__(CB) void k_dot2(int n, const T* __restrict__ a,
                                              const T* __restrict__ b, const T* __restrict__ c2,
-                                             double* __restrict__ part,
-                                             const int* __restrict__ done) {
+                                             Red R, const int* __restrict__ done) {
     if (is_done(done)) return;
     double s1 = 0.0, s2 = 0.0;
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
@@ -327,80 +756,28 @@ __global__ __launch_bounds__(CB) void k_dot2(int n, const T* __restrict__ a,
         s1 += av * (double)b[c];
         s2 += av * (double)c2[c];
     }
-    block_sum_store(s1, part);
-    __syncthreads();
-    block_sum_store(s2, part + gridDim.x);
-}
-
-// Scalar step of CG; sums the partials of the preceding reduction in a fixed
-// order.  sc: CG_* slots, fl: {done, iterations}.
-__global__ __launch_bounds__(RED_MAX) void k_cg_fin(const double* __restrict__ part, int np,
-                                                    int op, double tol2, int maxit,
-                                                    double* __restrict__ sc, int* __restrict__ fl) {
-    if (op != FIN_B2 && fl[0]) return;
-    __shared__ double lds[RED_MAX / 64];
-    auto add = [](double a, double b) { return a + b; };
-    double v = 0.0, v2 = 0.0;
-    for (int i = threadIdx.x; i < np; i += RED_MAX) v += part[i];
-    v = block_reduce<double, decltype(add), RED_MAX>(v, add, lds);
-    if (op == FIN_RZ_FLEX) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < np; i += RED_MAX) v2 += part[np + i];
-        v2 = block_reduce<double, decltype(add), RED_MAX>(v2, add, lds);
-    }
-    if (threadIdx.x) return;
-    switch (op) {
-        case FIN_B2:
-            sc[CG_B2] = v;
-            sc[CG_RR] = v;
-            sc[CG_BETA] = 0.0;
-            fl[1] = 0;
-            fl[0] = !(v > 0.0);
-            break;
-        case FIN_RR0:  // residual of a non-zero initial guess
-            sc[CG_RR] = v;
-            if (v <= tol2 * sc[CG_B2]) fl[0] = 1;
-            break;
-        case FIN_RZ0:
-            sc[CG_RZ] = v;
-            sc[CG_BETA] = 0.0;
-            break;
-        case FIN_PQ:
-            sc[CG_ALPHA] = sc[CG_RZ] / v;
-            if (!(v > 0.0)) fl[0] = 2;  // breakdown: not positive definite
-            break;
-        case FIN_RR:
-            sc[CG_RR] = v;
-            fl[1] += 1;
-            if (v <= tol2 * sc[CG_B2] || fl[1] >= maxit) fl[0] = 1;
-            break;
-        case FIN_RZ:
-            sc[CG_BETA] = v / sc[CG_RZ];
-            sc[CG_RZ] = v;
-            break;
-        case FIN_RZ_FLEX:  // Polak-Ribiere: beta = z.(r - r_old) / rz_old
-            sc[CG_BETA] = (v - v2) / sc[CG_RZ];
-            sc[CG_RZ] = v;
-            break;
-    }
+    reduce_finish(R, s1, s2, true);
 }
 
 // ───────────────────────── Chorin stage kernels ────────────────────────────
 // Qs = Q + dt dQ (rows b, h, hu, hv) and the upwind advection of hw with the
 // central face mass flux of Q.
+// (dQ in the FV solver's cell order: cell c at inv[c], or c when inv is NULL)
 __global__ __launch_bounds__(CB) void k_predict(int nx, int ny, double dx, double dy, double dt,
                                                 double eps, const double* __restrict__ Q,
                                                 const double* __restrict__ dQ,
+                                                const int* __restrict__ inv,
                                                 double* __restrict__ Qs) {
     const int n = nx * ny;
     int c = blockIdx.x * CB + threadIdx.x;
     if (c >= n) return;
     int i = c / ny, j = c - i * ny;
     const double *h = Q + n, *hu = Q + 2 * n, *hv = Q + 3 * n, *hw = Q + 4 * n;
-    Qs[c] = Q[c] + dt * dQ[c];
-    Qs[n + c] = h[c] + dt * dQ[n + c];
-    Qs[2 * n + c] = hu[c] + dt * dQ[2 * n + c];
-    Qs[3 * n + c] = hv[c] + dt * dQ[3 * n + c];
+    const int cf = inv ? inv[c] : c;
+    Qs[c] = Q[c] + dt * dQ[cf];
+    Qs[n + c] = h[c] + dt * dQ[n + cf];
+    Qs[2 * n + c] = hu[c] + dt * dQ[2 * n + cf];
+    Qs[3 * n + c] = hv[c] + dt * dQ[3 * n + cf];
     const double wc = hw[c] * hinv_of(h[c], eps);
     double acc = 0.0;
     if (i + 1 < nx) {
@@ -507,6 +884,17 @@ __global__ __launch_bounds__(CB) void k_correct(int nx, int ny, double dx, doubl
     Qout[4 * n + c] = q4;
 }
 
+// Qm[r][k] = Q[r][ord[k]] for the four FV rows
+__global__ __launch_bounds__(CB) void k_gather4(int n, const int* __restrict__ ord,
+                                                const double* __restrict__ Q,
+                                                double* __restrict__ Qm) {
+    int k = blockIdx.x * CB + threadIdx.x;
+    if (k >= n) return;
+    const int c = ord[k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Qm[(size_t)r * n + k] = Q[(size_t)r * n + c];
+}
+
 template <typename T>
 __global__ __launch_bounds__(CB) void k_fill(int n, T v, T* __restrict__ o) {
     int c = blockIdx.x * CB + threadIdx.x;
@@ -605,9 +993,64 @@ static void build_levels(Hier& H, const double* h0, const double* b0, double eps
 
 // z = V(r) from level l0 down; r and z are the vectors of level l0 (on
 // level 0 they are caller vectors, on deeper levels L.f and L.x).
+static int cells_of(const Hier& H, int l0) {
+    int c = 0;
+    for (size_t l = l0; l < H.levels.size(); ++l) c += H.levels[l].nx * H.levels[l].ny;
+    return c;
+}
+
+// Fused V-cycle (nu = 1): one down kernel per level above the tail, the
+// tail levels in one workgroup, one up kernel per level.  Returns false
+// when the tail does not fit (then the unfused V-cycle runs).
+template <typename T>
+static bool vcycle_fused(Hier& H, const T* r0, T* z0, int coarse_sweeps, const int* done,
+                         hipStream_t st, int l0) {
+    const int nl = (int)H.levels.size();
+    int lt = nl, cells = 0;
+    for (int l = nl - 1; l >= l0; --l) {
+        cells += H.levels[l].nx * H.levels[l].ny;
+        if (cells > TAIL_CELLS || nl - l > TAIL_MAXL) break;
+        lt = l;
+    }
+    if (lt == nl) return false;
+    auto X = [&](int l) { return l == l0 ? z0 : (T*)H.levels[l].x; };
+    auto F = [&](int l) { return l == l0 ? r0 : (const T*)H.levels[l].f; };
+    // the pre-smoothed iterate goes to the level's r array: the up kernel
+    // reads it in halos that other blocks of the same launch write in X
+    auto S = [&](int l) { return (T*)H.levels[l].r; };
+    for (int l = l0; l < lt; ++l) {
+        HLevel& L = H.levels[l];
+        HLevel& C = H.levels[l + 1];
+        dim3 g((C.nx + TC - 1) / TC, (C.ny + TC - 1) / TC);
+        k_mg_down<T><<<g, CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, F(l), S(l),
+                                       (T*)C.f, done);
+    }
+    TailDesc<T> d{};
+    d.nl = nl - lt;
+    for (int k = 0; k < d.nl; ++k) {
+        HLevel& L = H.levels[lt + k];
+        d.nx[k] = L.nx;
+        d.ny[k] = L.ny;
+        d.diag[k] = (const T*)L.diag;
+        d.cE[k] = (const T*)L.cE;
+        d.cN[k] = (const T*)L.cN;
+    }
+    d.f = F(lt);
+    d.x = X(lt);
+    k_mg_tail<T><<<1, TB, (size_t)5 * cells_of(H, lt) * sizeof(T), st>>>(d, coarse_sweeps, done);
+    for (int l = lt - 1; l >= l0; --l) {
+        HLevel& L = H.levels[l];
+        dim3 g((L.nx / 2 + TC - 1) / TC, (L.ny / 2 + TC - 1) / TC);
+        k_mg_up<T><<<g, CB, 0, st>>>(L.nx, L.ny, (T*)L.diag, (T*)L.cE, (T*)L.cN, F(l), S(l),
+                                     X(l), X(l + 1), done);
+    }
+    return true;
+}
+
 template <typename T>
 static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const int* done,
                    hipStream_t st, int l0 = 0) {
+    if (nu == 1 && H.fused && vcycle_fused<T>(H, r0, z0, coarse_sweeps, done, st, l0)) return;
     const int nl = (int)H.levels.size();
     auto X = [&](int l) { return l == l0 ? z0 : (T*)H.levels[l].x; };
     auto F = [&](int l) { return l == l0 ? r0 : (const T*)H.levels[l].f; };
@@ -707,6 +1150,9 @@ struct zj_chorin {
     zj::Hier H64, H32;
     // level-0 FP64 vectors
     double *Qs = nullptr, *Q1 = nullptr, *dQ = nullptr, *rhs = nullptr, *p = nullptr;
+    // FV solver cell order (ord[k] = grid cell of FV cell k; inv its inverse)
+    int *ord = nullptr, *inv = nullptr;
+    double* Qm = nullptr;
     double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
     // FP32 vectors of the inner solve of the refinement
     float *r32 = nullptr, *x32 = nullptr, *s32 = nullptr, *z32 = nullptr, *d32 = nullptr,
@@ -763,37 +1209,38 @@ static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z
         }
         return ZJ_OK;
     };
-    k_cg_init<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, c->part);
-    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_B2, tol2, maxit, c->sc, c->fl);
+    auto red = [&](int op) { return Red{c->part, c->sc, c->fl, tol2, maxit, op}; };
+    auto fin = [&](int op, int two = 0) { k_fin<<<1, RED_MAX, 0, st>>>(red(op), nr, two); };
+    k_cg_init<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, red(FIN_B2));
+    fin(FIN_B2);
     if (x0) {
-        k_dot<T><<<nr, CB, 0, st>>>(n, r, r, c->part, c->fl);
-        k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR0, tol2, maxit, c->sc, c->fl);
+        k_dot<T><<<nr, CB, 0, st>>>(n, r, r, red(FIN_RR0), c->fl);
+        fin(FIN_RR0);
     }
     if (int rc = precond(r, z)) return rc;
-    k_dot<T><<<nr, CB, 0, st>>>(n, r, z, c->part, c->fl);
-    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ0, tol2, maxit, c->sc, c->fl);
-    k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, c->fl);  // beta = 0: d = z
+    k_dot<T><<<nr, CB, 0, st>>>(n, r, z, red(FIN_RZ0), c->fl);
+    fin(FIN_RZ0);
+    k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
     HIPCHK(hipGetLastError());
     int launched = 0;
     int& last = learned ? c->last_iters32 : c->last_iters;
     int chunk = std::max(1, std::min(maxit, last));
     while (true) {
         for (int k = 0; k < chunk; ++k) {
-            k_spmv<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, c->part, c->fl);
-            k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_PQ, tol2, maxit, c->sc, c->fl);
-            k_axpy2<T><<<nr, CB, 0, st>>>(n, x, r, learned ? r_old : nullptr, d, q, c->sc,
-                                          c->part, c->fl);
-            k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR, tol2, maxit, c->sc, c->fl);
+            k_spmv<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, red(FIN_PQ), c->fl);
+            fin(FIN_PQ);
+            k_axpy2<T><<<nr, CB, 0, st>>>(n, x, r, learned ? r_old : nullptr, d, q, red(FIN_RR),
+                                          c->fl);
+            fin(FIN_RR);
             if (int rc = precond(r, z)) return rc;
             if (learned) {
-                k_dot2<T><<<nr, CB, 0, st>>>(n, z, r, r_old, c->part, c->fl);
-                k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ_FLEX, tol2, maxit, c->sc,
-                                                c->fl);
+                k_dot2<T><<<nr, CB, 0, st>>>(n, z, r, r_old, red(FIN_RZ_FLEX), c->fl);
+                fin(FIN_RZ_FLEX, 1);
             } else {
-                k_dot<T><<<nr, CB, 0, st>>>(n, z, r, c->part, c->fl);
-                k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RZ, tol2, maxit, c->sc, c->fl);
+                k_dot<T><<<nr, CB, 0, st>>>(n, z, r, red(FIN_RZ), c->fl);
+                fin(FIN_RZ);
             }
-            k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, c->fl);
+            k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, 0, c->fl);
         }
         HIPCHK(hipGetLastError());
         launched += chunk;
@@ -814,11 +1261,13 @@ static int residual64(zj_chorin* c, const double* rhs, const double* x, double* 
                       hipStream_t st) {
     zj::HLevel& L = c->H64.levels[0];
     const int nr = nred(c->N);
+    const Red rb{c->part, c->sc, c->fl, 0.0, 1, FIN_B2};
+    const Red rr{c->part, c->sc, c->fl, 0.0, 1, FIN_RR0};
     k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, (double*)L.diag, (double*)L.cE,
-                                         (double*)L.cN, (double*)x, 0, rhs, r, c->part);
-    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_B2, 0.0, 1, c->sc, c->fl);
-    k_dot<double><<<nr, CB, 0, st>>>(c->N, r, r, c->part, nullptr);
-    k_cg_fin<<<1, RED_MAX, 0, st>>>(c->part, nr, FIN_RR0, 0.0, 1, c->sc, c->fl);
+                                         (double*)L.cN, (double*)x, 0, rhs, r, rb);
+    k_fin<<<1, RED_MAX, 0, st>>>(rb, nr, 0);
+    k_dot<double><<<nr, CB, 0, st>>>(c->N, r, r, rr, nullptr);
+    k_fin<<<1, RED_MAX, 0, st>>>(rr, nr, 0);
     HIPCHK(hipGetLastError());
     if (int rc = read_flags(c, st)) return rc;
     double b2 = c->host_sc[CG_B2];
@@ -881,11 +1330,24 @@ static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStr
     return ZJ_OK;
 }
 
+static int predict(zj_chorin* c, double dt, const double* Q, double* Qs, hipStream_t st) {
+    const int n = c->N;
+    const double* Qf = Q;
+    if (c->ord) {
+        k_gather4<<<nblk(n), CB, 0, st>>>(n, c->ord, Q, c->Qm);
+        Qf = c->Qm;
+    }
+    if (int rc = zj_flux_operator(c->fv, Qf, c->dQ, st)) return rc;
+    k_predict<<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q, c->dQ, c->inv,
+                                      Qs);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
 static int stage(zj_chorin* c, double dt, const double* Qin, double* Qout, const double* Q0,
                  hipStream_t st) {
     const int n = c->N;
-    if (int rc = zj_flux_operator(c->fv, Qin, c->dQ, st)) return rc;
-    k_predict<<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Qin, c->dQ, c->Qs);
+    if (int rc = predict(c, dt, Qin, c->Qs, st)) return rc;
     if (int rc = setup_operator(c, c->Qs, c->Qs + n, st)) return rc;
     k_rhs<double><<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, c->Qs, c->rhs);
     HIPCHK(hipGetLastError());
@@ -962,6 +1424,35 @@ int zj_chorin_destroy(zj_chorin* c) {
     return ZJ_OK;
 }
 
+int zj_chorin_set_fv_order(zj_chorin* c, const int32_t* order) {
+    if (!c) return ZJ_EINVAL;
+    if (!order) {
+        c->ord = c->inv = nullptr;  // buffers stay allocated until destroy
+        return ZJ_OK;
+    }
+    const int n = c->N;
+    std::vector<int> inv(n, -1);
+    for (int k = 0; k < n; ++k) {
+        const int v = order[k];
+        if (v < 0 || v >= n || inv[v] >= 0) return ZJ_EINVAL;
+        inv[v] = k;
+    }
+    int *dord = nullptr, *dinv = nullptr;
+    if (dalloc(c, &dord, n) || dalloc(c, &dinv, n)) return ZJ_ENOMEM;
+    if (!c->Qm && dalloc(c, &c->Qm, 4 * (size_t)n)) return ZJ_ENOMEM;
+    HIPCHK(hipMemcpy(dord, order, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dinv, inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    c->ord = dord;
+    c->inv = dinv;
+    return ZJ_OK;
+}
+
+int zj_chorin_set_mg_fused(zj_chorin* c, int32_t on) {
+    if (!c) return ZJ_EINVAL;
+    c->H64.fused = c->H32.fused = on != 0;
+    return ZJ_OK;
+}
+
 int zj_chorin_set_precond(zj_chorin* c, int32_t precond, zj_mpnn* net) {
     if (!c || (precond != ZJ_PRECOND_MG && precond != ZJ_PRECOND_LEARNED)) return ZJ_EINVAL;
     if (precond == ZJ_PRECOND_LEARNED && !net) return ZJ_EINVAL;
@@ -980,7 +1471,7 @@ int zj_chorin_apply(zj_chorin* c, const double* x, double* y, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     zj::HLevel& L = c->H64.levels[0];
     k_spmv<double><<<nred(c->N), CB, 0, st>>>(L.nx, L.ny, (double*)L.diag, (double*)L.cE,
-                                              (double*)L.cN, x, y, nullptr, nullptr);
+                                              (double*)L.cN, x, y, Red{}, nullptr);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1046,11 +1537,7 @@ int zj_chorin_correct(zj_chorin* c, double dt, const double* Qs, const double* p
 
 int zj_chorin_predict(zj_chorin* c, double dt, const double* Q, double* Qs, void* stream) {
     if (!c || !Q || !Qs || !(dt > 0)) return ZJ_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    if (int rc = zj_flux_operator(c->fv, Q, c->dQ, st)) return rc;
-    k_predict<<<nblk(c->N), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q, c->dQ, Qs);
-    HIPCHK(hipGetLastError());
-    return ZJ_OK;
+    return predict(c, dt, Q, Qs, (hipStream_t)stream);
 }
 
 int zj_chorin_stage(zj_chorin* c, double dt, const double* Q, double* Qout, const double* Q0,

@@ -28,6 +28,7 @@ struct Hier {
     std::vector<HLevel> levels;
     void* mem = nullptr;
     size_t esz = 8;
+    bool fused = true;  // fused multigrid kernels when nu == 1
     int alloc(int nx, int ny, double dx, double dy, int min_coarse, size_t esz);
     void release();
 };

@@ -110,6 +110,8 @@ _SIGS_POISSON = {
                                         ctypes.POINTER(_V)]),
     "zj_chorin_destroy": (ctypes.c_int, [_V]),
     "zj_chorin_set_precond": (ctypes.c_int, [_V, ctypes.c_int32, _V]),
+    "zj_chorin_set_fv_order": (ctypes.c_int, [_V, _V]),
+    "zj_chorin_set_mg_fused": (ctypes.c_int, [_V, ctypes.c_int32]),
     "zj_chorin_step": (ctypes.c_int, [_V, _D, _V, _V, _V]),
     "zj_chorin_stage": (ctypes.c_int, [_V, _D, _V, _V, _V, _V]),
     "zj_chorin_predict": (ctypes.c_int, [_V, _D, _V, _V, _V]),

@@ -55,7 +55,7 @@ class ChorinSplitVAMSolver:
     def __init__(self, model: Optional[ShallowWater] = None, *,
                  tol: float = 1e-8, time_order: int = 2,
                  precond: str = "mg", maxiter: int = 200, nu: int = 1,
-                 coarse_sweeps: int = 8, min_coarse: int = 4,
+                 coarse_sweeps: int = 2, min_coarse: int = 4,
                  inner_tol: float = 1e-4, max_refine: int = 4):
         if precond not in L.PRECOND:
             raise ValueError(f"precond must be one of {sorted(L.PRECOND)}")
@@ -77,14 +77,22 @@ class ChorinSplitVAMSolver:
         self.net = None
 
     # ── setup ─────────────────────────────────────────────────────────
-    def setup_simulation(self, mesh: Mesh, nx: int, ny: int, Q0=None):
+    def setup_simulation(self, mesh: Mesh, nx: int, ny: int, Q0=None,
+                         fv_order: str = "morton"):
         """Bind to an ``nx x ny`` ``Mesh.quad`` grid (walls); returns the
-        state tensor (zeros, or ``Q0`` as a (4|5, N) array)."""
+        state tensor (zeros, or ``Q0`` as a (4|5, N) array).  The
+        hyperbolic predictor runs on the cells in ``fv_order`` ("morton":
+        along the Z-order curve; "grid": the grid order)."""
+        if fv_order not in ("morton", "grid"):
+            raise ValueError("fv_order must be 'morton' or 'grid'")
         if mesh.dim != 2 or mesh.n_cells != nx * ny or mesh.max_faces != 4:
             raise ValueError("the Chorin projection needs an nx x ny quad "
                              "mesh in Mesh.quad order")
         if not mesh.bc:
             mesh.set_bc("wall")
+        if any(k != 1 for k in mesh.bc.values()):
+            raise ValueError("the Chorin projection is built for walls on "
+                             "every boundary tag")
         lo = mesh.verts.min(axis=0)
         hi = mesh.verts.max(axis=0)
         dx = float(hi[0] - lo[0]) / nx
@@ -92,7 +100,12 @@ class ChorinSplitVAMSolver:
         self.mesh, self.nx, self.ny, self.dx, self.dy = mesh, nx, ny, dx, dy
         self.N = nx * ny
         self.device = _device()
-        self._dmesh = DeviceMesh(mesh)
+        order = None
+        fv_mesh = mesh
+        if fv_order == "morton":
+            order = np.ascontiguousarray(mesh.morton_permutation(), np.int32)
+            fv_mesh = mesh.reorder(order).set_bc("wall")
+        self._dmesh = DeviceMesh(fv_mesh)
         self._rt = _Runtime(self._dmesh, self.model, 0.9, torch.float64,
                             "conservative")
         h = ctypes.c_void_p()
@@ -100,6 +113,9 @@ class ChorinSplitVAMSolver:
                                          ctypes.byref(self._prm),
                                          ctypes.byref(h)), "zj_chorin_create")
         self.handle = h
+        if order is not None:
+            L.check(L.lib().zj_chorin_set_fv_order(h, order.ctypes.data),
+                    "zj_chorin_set_fv_order")
         Q = torch.zeros((5, self.N), dtype=torch.float64, device=self.device)
         if Q0 is not None:
             Q0 = torch.as_tensor(np.asarray(Q0), dtype=torch.float64)
@@ -111,6 +127,11 @@ class ChorinSplitVAMSolver:
         if h is not None and L._lib is not None:
             L._lib.zj_chorin_destroy(h)
             self.handle = None
+
+    def set_mg_fused(self, on: bool = True):
+        """Fused multigrid kernels (bit-identical to the separate ones)."""
+        L.check(L.lib().zj_chorin_set_mg_fused(self.handle, int(bool(on))),
+                "zj_chorin_set_mg_fused")
 
     def set_preconditioner(self, precond: str, net=None):
         """``"mg"``, or ``"learned"`` with a :class:`~zoomy_hip.learned.
