@@ -233,6 +233,9 @@ int zj_mpnn_adam(zj_mpnn *n, float lr, float beta1, float beta2, float eps, void
 int zj_mpnn_forward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *r,
                     const float *diag, const float *cE, const float *cN, float *z,
                     int32_t add, int32_t slot, void *stream);
+/* Inference (slot 2) in one fused launch (default) or layer by layer; both
+ * give the same result.  No counterpart in the reference. */
+int zj_mpnn_set_fused(zj_mpnn *n, int32_t on);
 /* Accumulates the gradient of sum(dz * z) for the forward of `slot`; dr
  * (may be NULL) receives the input gradient. */
 int zj_mpnn_backward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *dz,

@@ -82,6 +82,24 @@ def test_forward_backward_parity(nx, ny, B):
         o += k
 
 
+@pytest.mark.parametrize("nx,ny,B", [(16, 24, 2), (96, 80, 1), (64, 64, 1)])
+def test_fused_inference_equals_layer_by_layer(nx, ny, B):
+    net, flat = _net(11)
+    ops = [_operator(nx, ny, seed=k + 3) for k in range(B)]
+    stack = lambda arrs: _t32(np.stack(arrs))  # noqa: E731
+    r = stack([op[3].standard_normal((nx, ny)) for op in ops])
+    diag = stack([op[2][0] for op in ops])
+    cE = stack([op[2][1] for op in ops])
+    cN = stack([op[2][2] for op in ops])
+    z_layers = net.forward(r, diag, cE, cN, slot=0).cpu().numpy()
+    net.set_fused(True)
+    z_fused = net.forward(r, diag, cE, cN, slot=2).cpu().numpy()
+    np.testing.assert_array_equal(z_fused, z_layers)
+    base = torch.ones_like(r)
+    out = net.forward(r, diag, cE, cN, slot=2, z=base, add=True).cpu().numpy()
+    np.testing.assert_array_equal(out, z_layers + 1.0)
+
+
 def test_adam_step_parity():
     net, flat = _net(5)
     h, b, coef, rng = _operator(16, 16, 1)

@@ -1163,7 +1163,11 @@ struct zj_chorin {
     int* host_fl = nullptr;  // pinned copy of fl
     double* host_sc = nullptr;
     zj_mpnn* net = nullptr;  // learned preconditioner (may be NULL)
-    int last_iters = 8, last_iters32 = 8;
+    // iterations of the last solve (plain) and of each refinement sweep
+    // (learned): the size of the first chunk of iterations enqueued before
+    // the host reads the convergence flag
+    int last_iters = 8;
+    int last_iters32[8] = {4, 4, 4, 4, 4, 4, 4, 4};
     zj_chorin_stats stats{};
     std::vector<void*> allocs;
 };
@@ -1193,7 +1197,7 @@ static int read_flags(zj_chorin* c, hipStream_t st) {
 template <typename T>
 static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z, T* d, T* q,
                T* r_old, double tol, int maxit, int learned, hipStream_t st, int* iters,
-               double* relres) {
+               double* relres, int sweep = 0) {
     zj::HLevel& L = H.levels[0];
     const int n = L.nx * L.ny;
     const int nr = nred(n);
@@ -1223,7 +1227,7 @@ static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z
     k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
     HIPCHK(hipGetLastError());
     int launched = 0;
-    int& last = learned ? c->last_iters32 : c->last_iters;
+    int& last = learned ? c->last_iters32[std::min(sweep, 7)] : c->last_iters;
     int chunk = std::max(1, std::min(maxit, last));
     while (true) {
         for (int k = 0; k < chunk; ++k) {
@@ -1294,7 +1298,8 @@ static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStr
             int it = 0;
             double rel32 = 0.0;
             rc = pcg<float>(c, c->H32, c->r32, c->x32, 0, c->s32, c->z32, c->d32, c->q32,
-                            c->ro32, c->prm.inner_tol, c->prm.maxiter, 1, st, &it, &rel32);
+                            c->ro32, c->prm.inner_tol, c->prm.maxiter, 1, st, &it, &rel32,
+                            outer);
             inner_total += it;
             if (rc && rc != ZJ_ESOLVE) return rc;
             k_add_f32<<<nblk(c->N), CB, 0, st>>>(c->N, c->x32, p);

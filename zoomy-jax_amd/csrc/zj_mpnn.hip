@@ -65,13 +65,20 @@ __device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((unsign
 __device__ __forceinline__ bf16x8 ld8(const bf16_t* p) {
     return __builtin_bit_cast(bf16x8, *(const uint4*)p);
 }
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+// hardware conversion (v_cvt_pk_bf16_f32, round to nearest even)
 __device__ __forceinline__ bf16x8 pack8(const float* v) {
-    uint4 u;
-    u.x = (unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16);
-    u.y = (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16);
-    u.z = (unsigned)f2bf(v[4]) | ((unsigned)f2bf(v[5]) << 16);
-    u.w = (unsigned)f2bf(v[6]) | ((unsigned)f2bf(v[7]) << 16);
-    return __builtin_bit_cast(bf16x8, u);
+    f32x8 x = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+    return __builtin_convertvector(x, bf16x8);
+}
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
+    f32x4 x = {a, b, c, d};
+    return __builtin_bit_cast(uint2, __builtin_convertvector(x, bf16x4));
+}
+// gate: logistic sigmoid with the hardware reciprocal
+__device__ __forceinline__ float gate_of(float x) {
+    return __builtin_amdgcn_rcpf(1.f + __expf(-x));
 }
 __device__ __forceinline__ void unpack8(const bf16_t* p, float* v) {
     uint4 u = *(const uint4*)p;
@@ -237,28 +244,32 @@ __global__ __launch_bounds__(256) void k_layer_fwd(Grid g, const bf16_t* __restr
             float z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             a[0] = a[1] = a[2] = a[3] = ap = pack8(z);
         }
+        // transposed product: D^T = W^T A^T, so lane (m, kq) holds output
+        // channels nt*16 + 4 kq + i of node tile*16 + m
         f32x4 acc[4], gac[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
             acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
             gac[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) acc[nt] = mfma(a[ks], wf[nt][ks], acc[nt]);
-            gac[nt] = mfma(ap, uf[nt], gac[nt]);
+            for (int ks = 0; ks < 4; ++ks) acc[nt] = mfma(wf[nt][ks], a[ks], acc[nt]);
+            gac[nt] = mfma(uf[nt], ap, gac[nt]);
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = tile * 16 + kq * 4 + i;
-            if (row >= g.Nn) continue;
+        if (valid) {
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                const int col = nt * 16 + m;
-                const float y = acc[nt][i];
-                const float gt = 1.f / (1.f + __expf(-gac[nt][i]));
-                hout[(size_t)row * HID + col] = f2bf(gt * y);
+                const size_t o = (size_t)n * HID + nt * 16 + kq * 4;
+                f32x4 gt, hv;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    gt[i] = gate_of(gac[nt][i]);
+                    hv[i] = gt[i] * acc[nt][i];
+                }
+                *(uint2*)(hout + o) = pack4(hv[0], hv[1], hv[2], hv[3]);
                 if (y_out) {
-                    y_out[(size_t)row * HID + col] = y;
-                    g_out[(size_t)row * HID + col] = gt;
+                    *(float4*)(y_out + o) = make_float4(acc[nt][0], acc[nt][1], acc[nt][2],
+                                                        acc[nt][3]);
+                    *(float4*)(g_out + o) = make_float4(gt[0], gt[1], gt[2], gt[3]);
                 }
             }
         }
@@ -285,6 +296,186 @@ __global__ void k_out(Grid g, const float* __restrict__ P, const bf16_t* __restr
     for (int q = 0; q < 4; ++q) {
         const int c = node_cell(g, n, q);
         z[c] = add ? z[c] + o[q] : o[q];
+    }
+}
+
+
+// ───────────────────────── fused inference ─────────────────────────────────
+// z (+)= N(r) for one tile of FT x FT nodes in one launch: embedding, the
+// three layers and the output on LDS-resident activations with a 3-node
+// halo (regions of side FT+6, FT+4, FT+2, FT), each layer's BF16 weights
+// staged in LDS.  Same fragments, same MFMA sequence and same neighbour
+// summation order as the layer-by-layer kernels, hence the same result.
+constexpr int FT = 16;
+constexpr int FR0 = FT + 6;
+constexpr int FTH = 512;        // 8 waves
+constexpr int HS = HID + 8;     // padded row (144 B): 16-byte fragment reads of
+                                // 16 consecutive rows hit distinct banks
+constexpr size_t FUSED_LDS = (size_t)FR0 * FR0 * HS * 2      // region 0 / 2
+                             + (size_t)(FT + 4) * (FT + 4) * HS * 2  // region 1 / 3
+                             + (size_t)FR0 * FR0 * NF * 2             // phi
+                             + (size_t)(WF_SIZE + UF_SIZE) * 2;       // weights
+
+__global__ __launch_bounds__(FTH) void k_mpnn_fused(Grid g, const float* __restrict__ P,
+                                                    const bf16_t* __restrict__ wb,
+                                                    const bf16_t* __restrict__ phi,
+                                                    const float* __restrict__ r,
+                                                    const float* __restrict__ diag,
+                                                    float* __restrict__ z, int add,
+                                                    const int* __restrict__ done) {
+    if (done && *done) return;
+    extern __shared__ __align__(16) unsigned char fl[];
+    bf16_t* hA = (bf16_t*)fl;
+    bf16_t* hB = hA + FR0 * FR0 * HS;
+    bf16_t* sph = hB + (FT + 4) * (FT + 4) * HS;
+    bf16_t* sw = sph + FR0 * FR0 * NF;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    // this tile: nodes (a0 .. a0+FT-1, b0 ..) of grid `gs`
+    const int tiles_x = (g.nxn + FT - 1) / FT, tiles_y = (g.nyn + FT - 1) / FT;
+    const int per_grid = tiles_x * tiles_y;
+    const int gs = blockIdx.x / per_grid, tr = blockIdx.x - gs * per_grid;
+    const int a0 = (tr / tiles_y) * FT, b0 = (tr % tiles_y) * FT;
+    const int node_base = gs * g.nxn * g.nyn;
+    const int cell_base = gs * g.nx * g.ny;
+    auto in_grid = [&](int a, int b) { return a >= 0 && a < g.nxn && b >= 0 && b < g.nyn; };
+    // phi and the embedding of region 0 (origin a0-3, b0-3)
+    for (int u = tid; u < FR0 * FR0; u += FTH) {
+        const int ia = u / FR0, ib = u - ia * FR0;
+        const int a = a0 - 3 + ia, b = b0 - 3 + ib;
+        uint4* dp = (uint4*)(sph + u * NF);
+        uint4* dh = (uint4*)(hA + u * HS);
+        if (!in_grid(a, b)) {
+            const uint4 zz = {0u, 0u, 0u, 0u};
+            dp[0] = dp[1] = zz;
+            for (int k = 0; k < HID / 8; ++k) dh[k] = zz;
+            continue;
+        }
+        const int n = node_base + a * g.nyn + b;
+        const uint4* sp = (const uint4*)(phi + (size_t)n * NF);
+        dp[0] = sp[0];
+        dp[1] = sp[1];
+        float x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cell_base + (2 * a + (q >> 1)) * g.ny + 2 * b + (q & 1);
+            x[q] = r[c] / diag[c];
+        }
+        const float* W = P + OFF_WIN;
+        for (int k0 = 0; k0 < HID; k0 += 8) {
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + e;
+                v[e] = x[0] * W[k] + x[1] * W[HID + k] + x[2] * W[2 * HID + k] +
+                       x[3] * W[3 * HID + k];
+            }
+            dh[k0 / 8] = __builtin_bit_cast(uint4, pack8(v));
+        }
+    }
+    bf16_t* hin = hA;
+    bf16_t* hout = hB;
+    for (int l = 0; l < NL; ++l) {
+        // stage the layer's forward-swizzled W and U
+        {
+            const uint4* src = (const uint4*)(wb + l * BF_LAYER);
+            uint4* dst = (uint4*)sw;
+            for (int u = tid; u < (WF_SIZE + UF_SIZE) / 8; u += FTH) dst[u] = src[u];
+        }
+        __syncthreads();
+        const int sin = FR0 - 2 * l, sout = sin - 2;        // region sides
+        const int off_out = 3 - (l + 1);                    // origin of the output region
+        const int nrow = sout * sout, ntile = (nrow + 15) / 16;
+        bf16x8 wf[4][4], uf[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) wf[nt][ks] = ld8(sw + ((nt * 4 + ks) * 64 + lane) * 8);
+            uf[nt] = ld8(sw + WF_SIZE + (nt * 64 + lane) * 8);
+        }
+        for (int t = wave; t < ntile; t += FTH / 64) {
+            const int row = t * 16 + m;
+            bf16x8 a[4], ap;
+            if (row < nrow) {
+                const int ia = row / sout, ib = row - ia * sout;
+                const int ci = (ia + 1) * sin + (ib + 1);  // same node in the input region
+                const bf16_t* hs = hin + ci * HS;
+                a[0] = ld8(hs + kq * 8);
+                a[1] = ld8(hs + 32 + kq * 8);
+                const int nb[4] = {ci - sin, ci + sin, ci - 1, ci + 1};
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    float sacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float v[8];
+                        unpack8(hin + nb[q] * HS + half * 32 + kq * 8, v);
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) sacc[e] += v[e];
+                    }
+                    a[2 + half] = pack8(sacc);
+                }
+                const int pi = (ia + l + 1) * FR0 + (ib + l + 1);
+                if (kq < 2) {
+                    ap = ld8(sph + pi * NF + kq * 8);
+                } else {
+                    float zz[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    ap = pack8(zz);
+                }
+            } else {
+                float zz[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                a[0] = a[1] = a[2] = a[3] = ap = pack8(zz);
+            }
+            f32x4 acc[4], gac[4];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                gac[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) acc[nt] = mfma(wf[nt][ks], a[ks], acc[nt]);
+                gac[nt] = mfma(uf[nt], ap, gac[nt]);
+            }
+            if (row < nrow) {
+                const int ia = row / sout, ib = row - ia * sout;
+                const bool ing = in_grid(a0 - off_out + ia, b0 - off_out + ib);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    uint2 pk = {0u, 0u};
+                    if (ing) {
+                        f32x4 hv;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) hv[i] = gate_of(gac[nt][i]) * acc[nt][i];
+                        pk = pack4(hv[0], hv[1], hv[2], hv[3]);
+                    }
+                    *(uint2*)(hout + row * HS + nt * 16 + kq * 4) = pk;
+                }
+            }
+        }
+        __syncthreads();
+        bf16_t* tmp = hin;
+        hin = hout;
+        hout = tmp;
+    }
+    // output region (side FT) is in hin
+    for (int u = tid; u < FT * FT; u += FTH) {
+        const int ia = u / FT, ib = u - ia * FT;
+        const int a = a0 + ia, b = b0 + ib;
+        if (!in_grid(a, b)) continue;
+        const float* W = P + OFF_WOUT;
+        float o[4] = {0, 0, 0, 0};
+        for (int k0 = 0; k0 < HID; k0 += 8) {
+            float v[8];
+            unpack8(hin + u * HS + k0, v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) o[q] += v[e] * W[(k0 + e) * 4 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cell_base + (2 * a + (q >> 1)) * g.ny + 2 * b + (q & 1);
+            z[c] = add ? z[c] + o[q] : o[q];
+        }
     }
 }
 
@@ -605,6 +796,8 @@ struct zj_mpnn {
     MpnnCache cache[3];  // 0, 1: training slots; 2: inference
     Grid phi_grid{};
     const void* phi_key = nullptr;
+    bool fused = true;      // fused single-launch inference
+    bool lds_attr = false;  // dynamic LDS limit raised for the fused kernel
 };
 
 namespace {
@@ -675,6 +868,13 @@ int forward(zj_mpnn* net, const Grid& g, const float* r, const float* diag, cons
         k_phi<<<nb256(g.Nn), 256, 0, st>>>(g, diag, cE, cN, net->phi);
         net->phi_key = diag;
         net->phi_grid = g;
+    }
+    if (!train && net->fused) {
+        const int tiles = g.B * ((g.nxn + FT - 1) / FT) * ((g.nyn + FT - 1) / FT);
+        k_mpnn_fused<<<tiles, FTH, FUSED_LDS, st>>>(g, net->P, net->wb, net->phi, r, diag, z, add,
+                                                     done);
+        HIPCHK(hipGetLastError());
+        return ZJ_OK;
     }
     k_embed<<<nb256(g.Nn), 256, 0, st>>>(g, net->P, r, diag, train ? C.rh : nullptr, C.h[0],
                                           done);
@@ -758,11 +958,22 @@ int zj_mpnn_create(zj_mpnn** out) {
         zj_mpnn_destroy(n);
         return ZJ_ENOMEM;
     }
+    if (hipFuncSetAttribute((const void*)k_mpnn_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)FUSED_LDS) != hipSuccess) {
+        zj_mpnn_destroy(n);
+        return ZJ_EHIP;
+    }
     (void)hipMemset(n->P, 0, pb);
     (void)hipMemset(n->G, 0, pb);
     (void)hipMemset(n->M, 0, pb);
     (void)hipMemset(n->V, 0, pb);
     *out = n;
+    return ZJ_OK;
+}
+
+int zj_mpnn_set_fused(zj_mpnn* n, int32_t on) {
+    if (!n) return ZJ_EINVAL;
+    n->fused = on != 0;
     return ZJ_OK;
 }
 

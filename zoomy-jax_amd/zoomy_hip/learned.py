@@ -109,6 +109,11 @@ class MPPreconditioner:
                                          out.size), "zj_mpnn_get_grad")
         return out
 
+    def set_fused(self, on: bool = True):
+        """Single-launch inference kernel (same result as layer by layer)."""
+        L.check(L.lib().zj_mpnn_set_fused(self.handle, int(bool(on))),
+                "zj_mpnn_set_fused")
+
     def zero_grad(self):
         L.check(L.lib().zj_mpnn_zero_grad(self.handle, _stream()),
                 "zj_mpnn_zero_grad")
