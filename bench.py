@@ -137,6 +137,10 @@ def summarize_leg(r, w, N, steps):
 
 CHORIN_N = 1024
 CHORIN_L = 64.0
+# algorithmic FLOPs of one network application per 2x2-block node: three
+# layers of [h | agg h] (128) x 64 plus the 16 x 64 gate product, the 4 -> 64
+# embedding and the 64 -> 4 output
+NET_FLOP_PER_NODE = 3 * (2 * 128 * 64 + 2 * 16 * 64) + 2 * 4 * 64 + 2 * 64 * 4
 
 
 def chorin_state(n=CHORIN_N, L=CHORIN_L):
@@ -204,11 +208,47 @@ def chorin_leg(precond, steps, warmup, net=None, n=CHORIN_N, L=CHORIN_L):
         s.solve(rhs)
     torch.cuda.synchronize()
     solve_ms = (time.perf_counter() - t1) * 1e3 / reps
-    return {"ms_per_step": el * 1e3 / steps, "solve_ms": solve_ms,
-            "iterations_per_solve": float(np.mean(its)),
+    out = {"ms_per_step": el * 1e3 / steps, "solve_ms": solve_ms}
+    N = n * n
+    if net is not None:
+        # one application of the network (fused kernel) on this grid, timed
+        # with events on the launch stream
+        lv = s.levels()
+        d = torch.empty(N, dtype=torch.float32, device="cuda")
+        z = torch.empty_like(d)
+        r32 = rhs.float()
+        diag = Qs[1].float().contiguous()  # stand-in stencil of the right shape
+        e0, e1 = torch.cuda.Event(enable_timing=True), \
+            torch.cuda.Event(enable_timing=True)
+        d.copy_(diag.abs() + 1.0)
+        cE = -0.25 * d
+        cN = -0.25 * d
+        net.forward(r32.view(n, n), d.view(n, n), cE.view(n, n),
+                    cN.view(n, n), slot=2, z=z.view(n, n))
+        e0.record()
+        reps = 20
+        for _ in range(reps):
+            net.forward(r32.view(n, n), d.view(n, n), cE.view(n, n),
+                        cN.view(n, n), slot=2, z=z.view(n, n),
+                        features_unchanged=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flop = (N // 4) * NET_FLOP_PER_NODE
+        out["network"] = {"ms": ms, "flop_per_application": flop,
+                          "tflops": flop / ms / 1e9,
+                          "frac_of_bf16_dense_peak": flop / ms / 1e9 / 2500.0,
+                          "levels": len(lv)}
+    its_mean = float(np.mean(its))
+    w = 4 if net is not None else 8
+    out.update({
+            "cg_bytes_per_unknown_iteration": 17 * w,
+            "solve_effective_gbs": 17 * w * N * its_mean / (solve_ms * 1e-3) / 1e9,
+            "iterations_per_solve": its_mean,
             "refinements_per_solve": float(np.mean(outer)),
             "fallbacks": fb, "worst_relres": worst,
-            "finite": bool(torch.isfinite(Q).all())}
+            "finite": bool(torch.isfinite(Q).all())})
+    return out
 
 
 def main():

@@ -229,10 +229,14 @@ int zj_mpnn_get_grad(zj_mpnn *n, float *host, int64_t count);
 int zj_mpnn_zero_grad(zj_mpnn *n, void *stream);
 int zj_mpnn_adam(zj_mpnn *n, float lr, float beta1, float beta2, float eps, void *stream);
 /* z (+)= N(r) on B stacked nx x ny grids with FP32 stencils (diag, cE, cN);
- * slot 0/1 keep activations for zj_mpnn_backward, slot 2 does not. */
+ * slot 0/1 keep activations for zj_mpnn_backward, slot 2 does not.  The
+ * operator features are recomputed when diag or the grid differ from the
+ * last call, or after zj_mpnn_invalidate_features (call it when the stencil
+ * arrays were rewritten in place). */
 int zj_mpnn_forward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *r,
                     const float *diag, const float *cE, const float *cN, float *z,
                     int32_t add, int32_t slot, void *stream);
+int zj_mpnn_invalidate_features(zj_mpnn *n);
 /* Inference (slot 2) in one fused launch (default) or layer by layer; both
  * give the same result.  No counterpart in the reference. */
 int zj_mpnn_set_fused(zj_mpnn *n, int32_t on);

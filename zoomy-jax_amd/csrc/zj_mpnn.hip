@@ -971,6 +971,12 @@ int zj_mpnn_create(zj_mpnn** out) {
     return ZJ_OK;
 }
 
+int zj_mpnn_invalidate_features(zj_mpnn* n) {
+    if (!n) return ZJ_EINVAL;
+    n->phi_key = nullptr;
+    return ZJ_OK;
+}
+
 int zj_mpnn_set_fused(zj_mpnn* n, int32_t on) {
     if (!n) return ZJ_EINVAL;
     n->fused = on != 0;
@@ -1038,7 +1044,6 @@ int zj_mpnn_forward(zj_mpnn* n, int32_t B, int32_t nx, int32_t ny, const float* 
     if (!n || B < 1 || nx < 2 || ny < 2 || (nx & 1) || (ny & 1) || !r || !diag || !cE || !cN ||
         !z || slot < 0 || slot > 2)
         return ZJ_EINVAL;
-    n->phi_key = nullptr;  // coefficients may have changed in place
     return forward(n, make_grid(B, nx, ny), r, diag, cE, cN, z, add, slot, nullptr,
                    (hipStream_t)stream);
 }

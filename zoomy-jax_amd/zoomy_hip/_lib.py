@@ -134,6 +134,7 @@ _SIGS_POISSON = {
     "zj_mpnn_get_grad": (ctypes.c_int, [_V, _V, ctypes.c_int64]),
     "zj_mpnn_zero_grad": (ctypes.c_int, [_V, _V]),
     "zj_mpnn_set_fused": (ctypes.c_int, [_V, ctypes.c_int32]),
+    "zj_mpnn_invalidate_features": (ctypes.c_int, [_V]),
     "zj_mpnn_adam": (ctypes.c_int, [_V, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_float, _V]),
     "zj_mpnn_forward": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,

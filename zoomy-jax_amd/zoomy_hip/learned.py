@@ -139,11 +139,17 @@ class MPPreconditioner:
             raise ValueError("expected a contiguous float32 CUDA tensor")
         return t
 
-    def forward(self, r, diag, cE, cN, slot=2, z=None, add=False):
-        """z (+)= N(r) for fields of shape (B, nx, ny) (or (nx, ny))."""
+    def forward(self, r, diag, cE, cN, slot=2, z=None, add=False,
+                features_unchanged=False):
+        """z (+)= N(r) for fields of shape (B, nx, ny) (or (nx, ny)).  The
+        operator features are recomputed unless ``features_unchanged`` (the
+        stencil arrays are the ones of the previous call, unmodified)."""
         B, nx, ny = (1,) + tuple(r.shape) if r.dim() == 2 else tuple(r.shape)
         for t in (r, diag, cE, cN):
             self._f32(t)
+        if not features_unchanged:
+            L.check(L.lib().zj_mpnn_invalidate_features(self.handle),
+                    "zj_mpnn_invalidate_features")
         out = torch.zeros_like(r) if z is None else self._f32(z)
         L.check(L.lib().zj_mpnn_forward(self.handle, B, nx, ny, _p(r),
                                         _p(diag), _p(cE), _p(cN), _p(out),
