@@ -166,6 +166,11 @@ int zj_chorin_destroy(zj_chorin *c);
  * kernel gathers from compact tiles).  NULL: the grid order.  No counterpart
  * in the reference (its Chorin solver has no device mesh layout). */
 int zj_chorin_set_fv_order(zj_chorin *c, const int32_t *order);
+/* Mixed-precision predictor: the FV fluxes of the hyperbolic stage on fv32,
+ * an FP32 zj_solver of the same cells in the same order as the FP64 one
+ * (NULL: back to FP64).  The state, the pressure residual and the
+ * refinement stay FP64. */
+int zj_chorin_set_fv32(zj_chorin *c, zj_solver *fv32);
 /* Fused multigrid kernels on/off (default on; used when nu == 1).  Both
  * give bit-identical V-cycles.  No counterpart in the reference. */
 int zj_chorin_set_mg_fused(zj_chorin *c, int32_t on);
@@ -237,6 +242,12 @@ int zj_mpnn_forward(zj_mpnn *n, int32_t B, int32_t nx, int32_t ny, const float *
                     const float *diag, const float *cE, const float *cN, float *z,
                     int32_t add, int32_t slot, void *stream);
 int zj_mpnn_invalidate_features(zj_mpnn *n);
+/* Profiling: average duration (ms) of `reps` back-to-back inference
+ * launches on one nx x ny grid, timed with HIP events on `stream`.  No
+ * counterpart in the reference; used by bench.py for the MFMA rate. */
+int zj_mpnn_time_forward(zj_mpnn *n, int32_t nx, int32_t ny, const float *r, const float *diag,
+                         const float *cE, const float *cN, float *z, int32_t reps,
+                         float *ms_out, void *stream);
 /* Inference (slot 2) in one fused launch (default) or layer by layer; both
  * give the same result.  No counterpart in the reference. */
 int zj_mpnn_set_fused(zj_mpnn *n, int32_t on);

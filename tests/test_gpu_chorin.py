@@ -188,6 +188,25 @@ def test_predictor_parity():
     assert _rel(out[4], hws) < 1e-12
 
 
+def test_mixed_precision_predictor():
+    # FP32 fluxes, FP64 state: within the FP32 flux bar of the oracle
+    n = 64
+    box, Q, hw = _case(n)
+    hw = 0.02 * np.random.default_rng(12).standard_normal(n * n)
+    s, _ = _solver(n, box, flux_dtype=torch.float32)
+    co = _oracle_chorin(n, box)
+    dt = 0.45 * (box[1] / n) / (np.sqrt(9.81) + 1)
+    out = s._hyperbolic_stage(dt, 0.0, _state5(Q, hw)).cpu().numpy()
+    Qs, hws = co.predictor(Q, hw, dt)
+    for k in (1, 2):
+        inc = out[k] - Q[k]
+        ref = Qs[k] - Q[k]
+        assert np.abs(inc - ref).max() <= 1e-5 * np.abs(ref).max(), k
+    assert _rel(out[4], hws) < 1e-12
+    st = s.chorin_stage(dt, 0.0, _state5(Q, hw))
+    assert s.stats()["relres"] <= 1e-8 and bool(torch.isfinite(st).all())
+
+
 def test_predictor_same_in_grid_and_morton_order():
     n = 64
     box, Q, hw = _case(n)

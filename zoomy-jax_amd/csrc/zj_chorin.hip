@@ -762,10 +762,12 @@ __global__ __launch_bounds__(CB) void k_dot2(int n, const T* __restrict__ a,
 // ───────────────────────── Chorin stage kernels ────────────────────────────
 // Qs = Q + dt dQ (rows b, h, hu, hv) and the upwind advection of hw with the
 // central face mass flux of Q.
-// (dQ in the FV solver's cell order: cell c at inv[c], or c when inv is NULL)
+// (dQ in the FV solver's cell order: cell c at inv[c], or c when inv is NULL;
+// FP64 or, for the mixed-precision predictor, FP32)
+template <typename TQ>
 __global__ __launch_bounds__(CB) void k_predict(int nx, int ny, double dx, double dy, double dt,
                                                 double eps, const double* __restrict__ Q,
-                                                const double* __restrict__ dQ,
+                                                const TQ* __restrict__ dQ,
                                                 const int* __restrict__ inv,
                                                 double* __restrict__ Qs) {
     const int n = nx * ny;
@@ -774,10 +776,10 @@ __global__ __launch_bounds__(CB) void k_predict(int nx, int ny, double dx, doubl
     int i = c / ny, j = c - i * ny;
     const double *h = Q + n, *hu = Q + 2 * n, *hv = Q + 3 * n, *hw = Q + 4 * n;
     const int cf = inv ? inv[c] : c;
-    Qs[c] = Q[c] + dt * dQ[cf];
-    Qs[n + c] = h[c] + dt * dQ[n + cf];
-    Qs[2 * n + c] = hu[c] + dt * dQ[2 * n + cf];
-    Qs[3 * n + c] = hv[c] + dt * dQ[3 * n + cf];
+    Qs[c] = Q[c] + dt * (double)dQ[cf];
+    Qs[n + c] = h[c] + dt * (double)dQ[n + cf];
+    Qs[2 * n + c] = hu[c] + dt * (double)dQ[2 * n + cf];
+    Qs[3 * n + c] = hv[c] + dt * (double)dQ[3 * n + cf];
     const double wc = hw[c] * hinv_of(h[c], eps);
     double acc = 0.0;
     if (i + 1 < nx) {
@@ -884,15 +886,17 @@ __global__ __launch_bounds__(CB) void k_correct(int nx, int ny, double dx, doubl
     Qout[4 * n + c] = q4;
 }
 
-// Qm[r][k] = Q[r][ord[k]] for the four FV rows
+// Qm[r][k] = Q[r][ord[k]] (ord NULL: identity) for the four FV rows, in
+// the FV solver's precision
+template <typename TM>
 __global__ __launch_bounds__(CB) void k_gather4(int n, const int* __restrict__ ord,
                                                 const double* __restrict__ Q,
-                                                double* __restrict__ Qm) {
+                                                TM* __restrict__ Qm) {
     int k = blockIdx.x * CB + threadIdx.x;
     if (k >= n) return;
-    const int c = ord[k];
+    const int c = ord ? ord[k] : k;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) Qm[(size_t)r * n + k] = Q[(size_t)r * n + c];
+    for (int r = 0; r < 4; ++r) Qm[(size_t)r * n + k] = (TM)Q[(size_t)r * n + c];
 }
 
 template <typename T>
@@ -1153,6 +1157,9 @@ struct zj_chorin {
     // FV solver cell order (ord[k] = grid cell of FV cell k; inv its inverse)
     int *ord = nullptr, *inv = nullptr;
     double* Qm = nullptr;
+    // mixed-precision predictor: FP32 FV solver of the same cells
+    zj_solver* fv32 = nullptr;
+    float *Qm32 = nullptr, *dQ32 = nullptr;
     double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
     // FP32 vectors of the inner solve of the refinement
     float *r32 = nullptr, *x32 = nullptr, *s32 = nullptr, *z32 = nullptr, *d32 = nullptr,
@@ -1337,14 +1344,22 @@ static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStr
 
 static int predict(zj_chorin* c, double dt, const double* Q, double* Qs, hipStream_t st) {
     const int n = c->N;
+    if (c->fv32) {  // mixed precision: FP32 fluxes, FP64 state
+        k_gather4<float><<<nblk(n), CB, 0, st>>>(n, c->ord, Q, c->Qm32);
+        if (int rc = zj_flux_operator(c->fv32, c->Qm32, c->dQ32, st)) return rc;
+        k_predict<float><<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q,
+                                                 c->dQ32, c->inv, Qs);
+        HIPCHK(hipGetLastError());
+        return ZJ_OK;
+    }
     const double* Qf = Q;
     if (c->ord) {
-        k_gather4<<<nblk(n), CB, 0, st>>>(n, c->ord, Q, c->Qm);
+        k_gather4<double><<<nblk(n), CB, 0, st>>>(n, c->ord, Q, c->Qm);
         Qf = c->Qm;
     }
     if (int rc = zj_flux_operator(c->fv, Qf, c->dQ, st)) return rc;
-    k_predict<<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q, c->dQ, c->inv,
-                                      Qs);
+    k_predict<double><<<nblk(n), CB, 0, st>>>(c->nx, c->ny, c->dx, c->dy, dt, c->eps, Q, c->dQ,
+                                              c->inv, Qs);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1449,6 +1464,19 @@ int zj_chorin_set_fv_order(zj_chorin* c, const int32_t* order) {
     HIPCHK(hipMemcpy(dinv, inv.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     c->ord = dord;
     c->inv = dinv;
+    return ZJ_OK;
+}
+
+int zj_chorin_set_fv32(zj_chorin* c, zj_solver* fv32) {
+    if (!c) return ZJ_EINVAL;
+    if (!fv32) {
+        c->fv32 = nullptr;
+        return ZJ_OK;
+    }
+    if (zj::solver_dtype(fv32) != ZJ_F32 || zj::solver_cells(fv32) != c->N) return ZJ_EINVAL;
+    if (!c->Qm32 && (dalloc(c, &c->Qm32, 4 * (size_t)c->N) || dalloc(c, &c->dQ32, 4 * (size_t)c->N)))
+        return ZJ_ENOMEM;
+    c->fv32 = fv32;
     return ZJ_OK;
 }
 

@@ -33,6 +33,10 @@ struct Hier {
     void release();
 };
 
+// precision (ZJ_F64 / ZJ_F32) and cell count of an FV solver context
+int solver_dtype(const zj_solver* s);
+int solver_cells(const zj_solver* s);
+
 // Network of the learned preconditioner on an nx x ny grid with fine-level
 // FP32 stencil (diag, cE, cN): z (+)= N(r).  slot 0, 1 keep the activations
 // for mpnn_backward_f32; slot 2 is inference.  Kernels return at once while

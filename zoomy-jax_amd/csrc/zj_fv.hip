@@ -24,6 +24,7 @@
 
 #include "../../include/zoomy_hip.h"
 #include "zj_device.hpp"
+#include "zj_chorin.hpp"
 #include "zj_patches.hpp"
 
 using namespace zj;
@@ -1056,6 +1057,11 @@ struct zj_solver {
     double* dt_tmp;             // device dt
     int mass_blocks;
 };
+
+namespace zj {
+int solver_dtype(const zj_solver* s) { return s ? s->p.dtype : -1; }
+int solver_cells(const zj_solver* s) { return s ? s->m->d.N : -1; }
+}  // namespace zj
 
 #define HIPCHK(x)                                                     \
     do {                                                              \

@@ -1048,6 +1048,32 @@ int zj_mpnn_forward(zj_mpnn* n, int32_t B, int32_t nx, int32_t ny, const float* 
                    (hipStream_t)stream);
 }
 
+int zj_mpnn_time_forward(zj_mpnn* n, int32_t nx, int32_t ny, const float* r, const float* diag,
+                         const float* cE, const float* cN, float* z, int32_t reps,
+                         float* ms_out, void* stream) {
+    if (!n || nx < 2 || ny < 2 || (nx & 1) || (ny & 1) || !r || !diag || !cE || !cN || !z ||
+        reps < 1 || !ms_out)
+        return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const Grid g = make_grid(1, nx, ny);
+    n->phi_key = nullptr;
+    if (int rc = forward(n, g, r, diag, cE, cN, z, 0, 2, nullptr, st)) return rc;  // features
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int k = 0; k < reps; ++k)
+        if (int rc = forward(n, g, r, diag, cE, cN, z, 0, 2, nullptr, st)) return rc;
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = ms / (float)reps;
+    return ZJ_OK;
+}
+
 int zj_mpnn_backward(zj_mpnn* n, int32_t B, int32_t nx, int32_t ny, const float* dz,
                      const float* diag, float* dr, int32_t slot, void* stream) {
     if (!n || B < 1 || nx < 2 || ny < 2 || (nx & 1) || (ny & 1) || !dz || !diag || slot < 0 ||

@@ -112,6 +112,7 @@ _SIGS_POISSON = {
     "zj_chorin_set_precond": (ctypes.c_int, [_V, ctypes.c_int32, _V]),
     "zj_chorin_set_fv_order": (ctypes.c_int, [_V, _V]),
     "zj_chorin_set_mg_fused": (ctypes.c_int, [_V, ctypes.c_int32]),
+    "zj_chorin_set_fv32": (ctypes.c_int, [_V, _V]),
     "zj_chorin_step": (ctypes.c_int, [_V, _D, _V, _V, _V]),
     "zj_chorin_stage": (ctypes.c_int, [_V, _D, _V, _V, _V, _V]),
     "zj_chorin_predict": (ctypes.c_int, [_V, _D, _V, _V, _V]),
@@ -135,6 +136,9 @@ _SIGS_POISSON = {
     "zj_mpnn_zero_grad": (ctypes.c_int, [_V, _V]),
     "zj_mpnn_set_fused": (ctypes.c_int, [_V, ctypes.c_int32]),
     "zj_mpnn_invalidate_features": (ctypes.c_int, [_V]),
+    "zj_mpnn_time_forward": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,
+                                            _V, _V, _V, _V, _V, ctypes.c_int32,
+                                            _V, _V]),
     "zj_mpnn_adam": (ctypes.c_int, [_V, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_float, _V]),
     "zj_mpnn_forward": (ctypes.c_int, [_V, ctypes.c_int32, ctypes.c_int32,

@@ -56,7 +56,8 @@ class ChorinSplitVAMSolver:
                  tol: float = 1e-8, time_order: int = 2,
                  precond: str = "mg", maxiter: int = 200, nu: int = 1,
                  coarse_sweeps: int = 2, min_coarse: int = 4,
-                 inner_tol: float = 1e-4, max_refine: int = 4):
+                 inner_tol: float = 1e-4, max_refine: int = 4,
+                 flux_dtype: torch.dtype = torch.float64):
         if precond not in L.PRECOND:
             raise ValueError(f"precond must be one of {sorted(L.PRECOND)}")
         if time_order not in (1, 2):
@@ -73,6 +74,9 @@ class ChorinSplitVAMSolver:
                                    int(nu), int(coarse_sweeps),
                                    int(min_coarse), L.PRECOND[precond],
                                    int(time_order), int(max_refine))
+        if flux_dtype not in (torch.float64, torch.float32):
+            raise ValueError("flux_dtype must be torch.float64 or float32")
+        self.flux_dtype = flux_dtype
         self.handle = None
         self.net = None
 
@@ -116,6 +120,12 @@ class ChorinSplitVAMSolver:
         if order is not None:
             L.check(L.lib().zj_chorin_set_fv_order(h, order.ctypes.data),
                     "zj_chorin_set_fv_order")
+        if self.flux_dtype == torch.float32:
+            # mixed precision: FP32 fluxes on the same (reordered) cells
+            self._rt32 = _Runtime(self._dmesh, self.model, 0.9, torch.float32,
+                                  "conservative")
+            L.check(L.lib().zj_chorin_set_fv32(h, self._rt32.handle),
+                    "zj_chorin_set_fv32")
         Q = torch.zeros((5, self.N), dtype=torch.float64, device=self.device)
         if Q0 is not None:
             Q0 = torch.as_tensor(np.asarray(Q0), dtype=torch.float64)

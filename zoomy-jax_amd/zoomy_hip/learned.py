@@ -157,6 +157,18 @@ class MPPreconditioner:
                 "zj_mpnn_forward")
         return out
 
+    def time_forward(self, r, diag, cE, cN, reps=20):
+        """Average ms of one inference launch on an (nx, ny) grid (HIP
+        events, back-to-back launches)."""
+        nx, ny = r.shape
+        z = torch.empty_like(r)
+        ms = ctypes.c_float()
+        L.check(L.lib().zj_mpnn_time_forward(
+            self.handle, nx, ny, _p(self._f32(r)), _p(self._f32(diag)),
+            _p(self._f32(cE)), _p(self._f32(cN)), _p(z), int(reps),
+            ctypes.byref(ms), _stream()), "zj_mpnn_time_forward")
+        return float(ms.value)
+
     def backward(self, dz, diag, slot=0, want_dr=True):
         B, nx, ny = (1,) + tuple(dz.shape) if dz.dim() == 2 else \
             tuple(dz.shape)
