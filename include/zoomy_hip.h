@@ -101,6 +101,15 @@ int zj_compute_dt(zj_solver *s, const void *Q, double t_remaining,
 int zj_step(zj_solver *s, const double *dt, const void *Q, void *Qout,
             int32_t *troubled_out, void *stream);
 
+/* A-posteriori MOOD (solver_jax.py:1055-1071, HyperbolicSolver.mood): every
+ * order-2 step (zj_step, zj_march) first computes a candidate; if a cell has
+ * h < 0 or a non-finite component, the step is redone from the same input
+ * with those cells reconstructed at first order (phi := 0, shared face
+ * fluxes: conservative).  The redo is decided on the device (no host
+ * synchronisation).  Off by default. */
+int zj_solver_set_mood(zj_solver *s, int32_t on);
+/* {steps redone, cells demoted} since MOOD was enabled (host-synchronous). */
+int zj_solver_mood_stats(zj_solver *s, int32_t *out2);
 /* post_step aux refresh (solver_jax.py:1133-1172): Qaux rows per direction
  * d: [d q_d/dx_d, d h/dx_d], then d b/dx_d, then hinv. */
 int zj_update_aux(zj_solver *s, const void *Q, void *Qaux, void *stream);

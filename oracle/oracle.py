@@ -80,6 +80,9 @@ def lib():
         L.zjo_compute_dt.argtypes = [P(_Mesh), P(_Params), vp, d]
         L.zjo_step.restype = ctypes.c_int
         L.zjo_step.argtypes = [P(_Mesh), P(_Params), d, vp, vp]
+        L.zjo_step_mood.restype = ctypes.c_int
+        L.zjo_step_mood.argtypes = [P(_Mesh), P(_Params), d, vp, vp,
+                                    P(ctypes.c_int)]
         L.zjo_flux_operator.argtypes = [P(_Mesh), P(_Params), vp, vp]
         L.zjo_reconstruct.argtypes = [P(_Mesh), P(_Params), vp, vp, vp, vp]
         L.zjo_update_aux.argtypes = [P(_Mesh), P(_Params), vp, vp]
@@ -219,6 +222,17 @@ class Oracle:
         bad = self.L.zjo_step(ctypes.byref(self.rm.c), ctypes.byref(self.p),
                               float(dt), _ptr(Q), _ptr(out))
         return out, bad
+
+    def step_mood(self, dt, Q):
+        """Order-2 step with the a-posteriori MOOD redo; returns
+        (Q, troubled count, number of demoted cells)."""
+        Q = self._q(Q)
+        out = np.empty_like(Q)
+        nd = ctypes.c_int(0)
+        bad = self.L.zjo_step_mood(ctypes.byref(self.rm.c),
+                                   ctypes.byref(self.p), float(dt), _ptr(Q),
+                                   _ptr(out), ctypes.byref(nd))
+        return out, bad, nd.value
 
     def flux(self, Q):
         Q = self._q(Q)

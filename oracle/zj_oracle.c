@@ -196,6 +196,19 @@ static void limited_slopes(const zjo_mesh *m, const zjo_params *p,
     free(umin);
 }
 
+/* Per-cell first-order demotion (MOOD force_o1, reconstruction_jax.py
+ * :1211-1225): phi := 0 for the flagged cells, applied after the limiter
+ * ties and before reconstruction.  NULL: none. */
+static const unsigned char *g_force_o1 = NULL;
+
+static void demote(int N, double *phi)
+{
+    if (!g_force_o1) return;
+    for (int c = 0; c < N; ++c)
+        if (g_force_o1[c])
+            for (int v = 0; v < NV; ++v) phi[(size_t)v * N + c] = 0.0;
+}
+
 static void face_values(const zjo_mesh *m, const double *W,
                         const double *grads, const double *phi, double *WL,
                         double *WR, const double *bW)
@@ -256,6 +269,7 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
     double *phi = malloc(sizeof(double) * NV * N);
     if (p->recon == ZJO_RECON_CONSERVATIVE) {
         limited_slopes(m, p, Q, bfv, grads, phi);
+        demote(N, phi);
         face_values(m, Q, grads, phi, QL, QR, bfv);
     } else {
         /* W = (b, eta = h + b, u = hinv q, v = hinv q_y); ghost W through the
@@ -286,6 +300,7 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
             phi[2 * (size_t)N + c] = dmin(phi[2 * (size_t)N + c], pe);
             phi[3 * (size_t)N + c] = dmin(phi[3 * (size_t)N + c], pe);
         }
+        demote(N, phi);
         double *WL = malloc(sizeof(double) * NV * F);
         double *WR = malloc(sizeof(double) * NV * F);
         face_values(m, W, grads, phi, WL, WR, bW);
@@ -604,6 +619,32 @@ int zjo_step(const zjo_mesh *m, const zjo_params *p, double dt,
     }
     free(dQ);
     return count_troubled(N, Qout);
+}
+
+/* A-posteriori MOOD (solver_jax.py:1055-1071): the order-2 candidate; if a
+ * cell has h < 0 or a non-finite component, the whole step is redone with
+ * those cells reconstructed at first order (shared face fluxes: still
+ * conservative).  Returns the troubled count of the accepted state;
+ * *n_demoted receives the number of demoted cells (0: candidate accepted). */
+int zjo_step_mood(const zjo_mesh *m, const zjo_params *p, double dt,
+                  const double *Q, double *Qout, int *n_demoted)
+{
+    const int N = m->n_cells;
+    int bad = zjo_step(m, p, dt, Q, Qout);
+    *n_demoted = 0;
+    if (!bad || p->order < 2) return bad;
+    unsigned char *mask = calloc((size_t)N, 1);
+    for (int c = 0; c < N; ++c) {
+        int t = Qout[(size_t)N + c] < 0.0;
+        for (int v = 0; v < NV; ++v) t |= !isfinite(Qout[(size_t)v * N + c]);
+        mask[c] = (unsigned char)t;
+        *n_demoted += t;
+    }
+    g_force_o1 = mask;
+    bad = zjo_step(m, p, dt, Q, Qout);
+    g_force_o1 = NULL;
+    free(mask);
+    return bad;
 }
 
 void zjo_update_aux(const zjo_mesh *m, const zjo_params *p, const double *Q,

@@ -92,6 +92,8 @@ int zjo_step(const zjo_mesh *m, const zjo_params *p, double dt,
 
 /* Aux refresh (post_step): rows per horizontal direction d:
  * [d q_d/dx_d, d h/dx_d], then [d b/dx_d] per d, then hinv. */
+int zjo_step_mood(const zjo_mesh *m, const zjo_params *p, double dt,
+                  const double *Q, double *Qout, int *n_demoted);
 void zjo_update_aux(const zjo_mesh *m, const zjo_params *p, const double *Q,
                     double *Qaux);
 

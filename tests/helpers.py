@@ -74,3 +74,18 @@ def rel_l2(a, b):
 
 def max_rel_rows(a, b, rows=(1, 2, 3)):
     return max(rel_l2(a[r], b[r]) for r in rows)
+
+
+def wetdry_quads(n=48):
+    """Wet/dry case on n x n quads over [0, 10]^2 (walls): reservoir
+    h = 3 - b for x < 3, dry elsewhere, bed 0.3 sin(x)^2 + 0.05 y.  At order
+    2 with the eta reconstruction and CFL 0.9 its candidate steps have cells
+    with h < 0 (the MOOD test case)."""
+    import numpy as np
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.quad(n, n, (0, 10, 0, 10)).set_bc("wall")
+    xc = mesh.cell_centers()
+    Q = np.zeros((4, mesh.n_cells))
+    Q[0] = 0.3 * np.sin(xc[:, 0]) ** 2 + 0.05 * xc[:, 1]
+    Q[1] = np.where(xc[:, 0] < 3.0, np.maximum(3.0 - Q[0], 0.0), 0.0)
+    return mesh, Q

@@ -239,3 +239,59 @@ def test_dt_guard_raises_on_zero_dt():
     s2 = _solver(mesh, dt_max=0.125)
     Q = torch.zeros((4, mesh.n_cells), dtype=torch.float64, device="cuda")
     assert float(s2.compute_timestep(Q)) == 0.125
+
+
+def _mood_case():
+    from helpers import wetdry_quads
+    mesh, Q = wetdry_quads()
+    o = _oracle(mesh, order=2, limiter="vanleer", riemann="hll", recon="eta")
+    return mesh, Q, o
+
+
+def test_mood_step_parity():
+    """One MOOD step from a state whose candidate has only clearly negative
+    depths (so the device and the oracle flag the same cells)."""
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    mesh, Q, o = _mood_case()
+    for _ in range(2):
+        Q, _, _ = o.step_mood(o.dt(Q), Q)
+    dt = o.dt(Q)
+    Qc, bc = o.step(dt, Q)
+    neg = Qc[1][Qc[1] < 0]
+    assert bc > 0 and neg.max() < -1e-9
+    Qo, bad, nd = o.step_mood(dt, Q)
+    model = ShallowWater(dimension=2, order=2, limiter="vanleer", riemann="hll")
+    s = HyperbolicSolver(time_end=np.inf, cfl=0.9,
+                         reconstruction_variables="eta", mood=True)
+    s.setup_simulation(mesh, model)
+    Qg = s.step(dt, 0.0, _to_dev(Q)).cpu().numpy()
+    assert s.mood_stats() == (1, nd) and s.troubled_count() == bad == 0
+    assert max_rel_rows(Qg, Qo) < 1e-12
+
+
+def test_mood_march_keeps_depth_nonnegative_and_mass():
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    mesh, Q, o = _mood_case()
+    model = ShallowWater(dimension=2, order=2, limiter="vanleer", riemann="hll")
+    s = HyperbolicSolver(time_end=np.inf, cfl=0.9,
+                         reconstruction_variables="eta", mood=True)
+    s.setup_simulation(mesh, model)
+    Qd = _to_dev(Q)
+    m0 = float(s.compute_mass(Qd))
+    s.march_device(Qd, 10)
+    steps, cells = s.mood_stats()
+    assert steps >= 3 and cells > 0
+    assert float(Qd[1].min()) >= 0.0 and s.troubled_count() == 0
+    assert abs(float(s.compute_mass(Qd)) - m0) < 1e-12 * m0
+    # MOOD on a step without troubled cells is the plain step, bit for bit
+    from helpers import circular_dam_state
+    from zoomy_hip.mesh import Mesh
+    m2 = Mesh.quad(32, 32).set_bc("wall")
+    Q2 = circular_dam_state(m2, seed=1)
+    outs = []
+    for mood in (False, True):
+        s2 = HyperbolicSolver(time_end=np.inf, cfl=0.9, mood=mood)
+        s2.setup_simulation(m2, ShallowWater(dimension=2, order=2,
+                                             riemann="hll"))
+        outs.append(s2.step(1e-4, 0.0, _to_dev(Q2)).cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])

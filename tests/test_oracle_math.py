@@ -261,3 +261,39 @@ def test_circular_dam_symmetry():
     assert np.abs(h - h.T).max() < 1e-13
     assert np.abs(hu - hv.T).max() < 1e-13
     assert np.abs(h - h[::-1]).max() < 1e-13
+
+
+def test_mood_redo_removes_troubled_cells_and_conserves_mass():
+    """A-posteriori MOOD (solver_jax.py:1055-1071): the order-2 candidate of
+    the wet/dry case has cells with h < 0; the redo with those cells at
+    first order has none, and mass is conserved between walls."""
+    from oracle.oracle import Oracle
+    from helpers import wetdry_quads
+    mesh, Q = wetdry_quads()
+    o = Oracle(mesh, order=2, limiter="vanleer", riemann="hll", recon="eta",
+               cfl=0.9)
+    m0 = o.mass(Q)
+    redone = 0
+    for _ in range(6):
+        dt = o.dt(Q)
+        _, bad_c = o.step(dt, Q)
+        Q, bad, nd = o.step_mood(dt, Q)
+        assert bad == 0 and nd == bad_c
+        redone += nd > 0
+    assert redone >= 3
+    assert abs(o.mass(Q) - m0) < 1e-12 * m0
+
+
+def test_mood_is_the_plain_step_when_nothing_is_troubled():
+    from oracle.oracle import Oracle
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.quad(24, 24).set_bc("wall")
+    xc = mesh.cell_centers()
+    Q = np.zeros((4, mesh.n_cells))
+    Q[1] = np.where(((xc - 0.5) ** 2).sum(1) < 0.0625, 2.0, 1.0)
+    o = Oracle(mesh, order=2, limiter="minmod", riemann="hll", cfl=0.9)
+    dt = o.dt(Q)
+    a, _ = o.step(dt, Q)
+    b, bad, nd = o.step_mood(dt, Q)
+    assert nd == 0 and bad == 0
+    np.testing.assert_array_equal(a, b)

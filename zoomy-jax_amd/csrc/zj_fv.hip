@@ -477,7 +477,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __rest
 }
 
 __global__ void k_dt_init(unsigned long long* red, int* troubled) {
-    *red = (unsigned long long)__double_as_longlong(INFINITY);
+    if (red) *red = (unsigned long long)__double_as_longlong(INFINITY);
     if (troubled) *troubled = 0;
 }
 
@@ -494,13 +494,28 @@ __global__ void k_t_advance(double* t, const double* dt) { *t += *dt; }
 
 // Folds the per-wave dt minima of a stage launch into the reduction scalar.
 __global__ __launch_bounds__(1024) void k_reduce_part(const double* __restrict__ part, int n,
-                                                      unsigned long long* __restrict__ dt_red) {
+                                                      unsigned long long* __restrict__ dt_red,
+                                                      const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     __shared__ double lds[1024 / 64];
     double v = INFINITY;
     for (int i = threadIdx.x; i < n; i += 1024) v = fmin(v, part[i]);
     auto mn = [](double a, double b) { return fmin(a, b); };
     v = block_reduce<double, decltype(mn), 1024>(v, mn, lds);
     if (threadIdx.x == 0) atomicMin(dt_red, (unsigned long long)__double_as_longlong(v));
+}
+
+// MOOD: the redo's dt replaces the candidate's (only when a redo runs).
+__global__ void k_dt_reset_if(const int* __restrict__ cnt, unsigned long long* __restrict__ dt_red) {
+    if (*cnt != 0) *dt_red = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+// MOOD bookkeeping: stats[0] += steps redone, stats[1] += cells demoted.
+__global__ void k_mood_finish(const int* __restrict__ cnt, int* __restrict__ stats) {
+    if (*cnt != 0) {
+        stats[0] += 1;
+        stats[1] += *cnt;
+    }
 }
 
 // Compensated (Neumaier) mass: per-thread partials, per-block (sum, comp)
@@ -680,7 +695,11 @@ template <typename T>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_stage(
     DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
-    int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part) {
+    int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part,
+    const uint8_t* __restrict__ o1_mask, uint8_t* __restrict__ bad_out,
+    const int* __restrict__ gate) {
+    // gate: a MOOD redo launch runs only when the candidate had troubled cells
+    if (gate && *gate == 0) return;
     // dynamic LDS, 32-byte aligned (the AoS state is read as double4)
     extern __shared__ double4 smem_v4[];
     double* smem_d = reinterpret_cast<double*>(smem_v4);
@@ -877,6 +896,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(3))) voi
                 phi[2] = tmin(phi[2], phi[1]);
                 phi[3] = tmin(phi[3], phi[1]);
             }
+            if (o1_mask && o1_mask[gi]) {  // MOOD: first order for this cell
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) phi[vv] = T(0);
+            }
             T* sp = Ss + (size_t)ls * 8;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
@@ -984,6 +1007,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(3))) voi
         if (final_stage && mode != 2) {
             bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
                   !isfinite(out[2]) || !isfinite(out[3]);
+            if (bad_out) bad_out[gi] = (uint8_t)bad;
             if (dt_part) {
                 // min over faces of r_f / a_f without a division per face
                 const T hn = hinv_of(out[1], eps);
@@ -1056,6 +1080,12 @@ struct zj_solver {
     double2* mass_part;         // per-block mass partials
     double* dt_tmp;             // device dt
     int mass_blocks;
+    // a-posteriori MOOD (off by default)
+    int mood = 0;
+    void* Qc = nullptr;             // (4, N) candidate state
+    uint8_t* mood_mask = nullptr;   // per-cell first-order flags
+    int* mood_cnt = nullptr;        // troubled cells of the candidate
+    int* mood_stats = nullptr;      // {steps redone, cells demoted}
 };
 
 namespace zj {
@@ -1244,6 +1274,10 @@ int zj_solver_destroy(zj_solver* s) {
     (void)hipFree(s->troubled);
     (void)hipFree(s->dt_tmp);
     (void)hipFree(s->mass_part);
+    if (s->Qc) (void)hipFree(s->Qc);
+    if (s->mood_mask) (void)hipFree(s->mood_mask);
+    if (s->mood_cnt) (void)hipFree(s->mood_cnt);
+    if (s->mood_stats) (void)hipFree(s->mood_stats);
     delete s;
     return ZJ_OK;
 }
@@ -1255,15 +1289,17 @@ namespace {
 template <typename T>
 void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double* dt, T* aux,
                   int mode, int final_stage, int write_b, int use_slopes, int* troubled,
-                  unsigned long long* dt_red, hipStream_t st) {
+                  unsigned long long* dt_red, hipStream_t st,
+                  const uint8_t* o1_mask = nullptr, uint8_t* bad_out = nullptr,
+                  const int* gate = nullptr) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
     hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(SBLOCK), lds, st, M->d, M->pt, s->o,
                        Qin, Q0, Qout, dt, aux, mode, final_stage, write_b, use_slopes, troubled,
-                       dt_red ? s->dt_part : (double*)nullptr);
+                       dt_red ? s->dt_part : (double*)nullptr, o1_mask, bad_out, gate);
     if (dt_red)
         hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(1024), 0, st, (const double*)s->dt_part,
-                           M->pt.npatch * (SBLOCK / 64), dt_red);
+                           M->pt.npatch * (SBLOCK / 64), dt_red, gate);
 }
 
 template <typename T>
@@ -1285,8 +1321,32 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
     const bool fuse_aux = aux_of_Q && s->o.recon == RECON_CONS;
     launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, fuse_aux ? aux_of_Q : (T*)nullptr, 0, 0,
                     write_b, 1, (int*)nullptr, (unsigned long long*)nullptr, st);
-    launch_stage<T>(s, (const T*)Q1, Q, Qout, dt, (T*)nullptr, 1, 1, write_b, 1, troubled, dt_red,
-                    st);
+    if (!s->mood) {
+        launch_stage<T>(s, (const T*)Q1, Q, Qout, dt, (T*)nullptr, 1, 1, write_b, 1, troubled,
+                        dt_red, st);
+        HIPCHK(hipGetLastError());
+        return ZJ_OK;
+    }
+    // A-posteriori MOOD (solver_jax.py:1055-1071): the candidate goes to Qc
+    // with its troubled mask; a redo with the flagged cells at first order
+    // runs on the device only when the candidate had troubled cells (the
+    // redo kernels read the count and return at once when it is 0).
+    T* Qc = (T*)s->Qc;
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, (unsigned long long*)nullptr,
+                       s->mood_cnt);
+    launch_stage<T>(s, (const T*)Q1, Q, Qc, dt, (T*)nullptr, 1, 1, write_b, 1, s->mood_cnt, dt_red,
+                    st, (const uint8_t*)nullptr, s->mood_mask);
+    if (dt_red)
+        hipLaunchKernelGGL(k_dt_reset_if, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt, dt_red);
+    launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, (T*)nullptr, 0, 0, write_b, 1, (int*)nullptr,
+                    (unsigned long long*)nullptr, st, s->mood_mask, (uint8_t*)nullptr,
+                    s->mood_cnt);
+    launch_stage<T>(s, (const T*)Q1, Q, Qc, dt, (T*)nullptr, 1, 1, write_b, 1, troubled, dt_red, st,
+                    s->mood_mask, (uint8_t*)nullptr, s->mood_cnt);
+    hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt,
+                       s->mood_stats);
+    HIPCHK(hipMemcpyAsync(Qout + (write_b ? 0 : m.N), Qc + (write_b ? 0 : m.N),
+                          sizeof(T) * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1346,6 +1406,33 @@ int zj_step(zj_solver* s, const double* dt, const void* Q, void* Qout, int32_t* 
                            (unsigned long long*)nullptr, st)
                : step_impl(s, dt, (const float*)Q, (float*)Qout, troubled_out,
                            (unsigned long long*)nullptr, st);
+}
+
+int zj_solver_set_mood(zj_solver* s, int32_t on) {
+    if (!s) return ZJ_EINVAL;
+    if (on && !s->Qc) {
+        const size_t N = s->m->d.N;
+        const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+        if (hipMalloc(&s->Qc, esz * 4 * N) != hipSuccess ||
+            hipMalloc((void**)&s->mood_mask, N) != hipSuccess ||
+            hipMalloc((void**)&s->mood_cnt, sizeof(int)) != hipSuccess ||
+            hipMalloc((void**)&s->mood_stats, 2 * sizeof(int)) != hipSuccess)
+            return ZJ_ENOMEM;
+        HIPCHK(hipMemset(s->mood_stats, 0, 2 * sizeof(int)));
+    }
+    s->mood = on ? 1 : 0;
+    return ZJ_OK;
+}
+
+int zj_solver_mood_stats(zj_solver* s, int32_t* out2) {
+    if (!s || !out2) return ZJ_EINVAL;
+    if (!s->mood_stats) {
+        out2[0] = out2[1] = 0;
+        return ZJ_OK;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(out2, s->mood_stats, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    return ZJ_OK;
 }
 
 int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {

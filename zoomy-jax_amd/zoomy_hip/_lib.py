@@ -96,6 +96,8 @@ _SIGS = {
     "zj_time_kernels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_solver_set_mood": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "zj_solver_mood_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zj_imex_local": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_int32, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p]),

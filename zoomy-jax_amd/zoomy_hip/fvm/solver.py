@@ -189,7 +189,7 @@ class HyperbolicSolver:
     def __init__(self, time_end: float = 1.0, cfl: float = 0.9,
                  dtype: torch.dtype = torch.float64,
                  reconstruction_variables: str = "conservative",
-                 compute_dt: Optional[Callable] = None):
+                 compute_dt: Optional[Callable] = None, mood: bool = False):
         if reconstruction_variables not in L.RECON:
             raise ValueError(
                 f"reconstruction_variables must be one of {sorted(L.RECON)}")
@@ -198,6 +198,9 @@ class HyperbolicSolver:
         self.dtype = dtype
         self.reconstruction_variables = reconstruction_variables
         self.compute_dt = compute_dt
+        # a-posteriori MOOD (solver_jax.py:1055-1071): redo a step with the
+        # troubled cells (h < 0 or non-finite) at first order
+        self.mood = bool(mood)
 
     # ── setup ─────────────────────────────────────────────────────────
     def initialize(self, mesh: Mesh, model: ShallowWater):
@@ -229,6 +232,9 @@ class HyperbolicSolver:
         self._dmesh = DeviceMesh(mesh)
         self._rt = _Runtime(self._dmesh, model, self.cfl, self.dtype,
                             self.reconstruction_variables)
+        if self.mood:
+            L.check(L.lib().zj_solver_set_mood(self._rt.handle, 1),
+                    "zj_solver_set_mood")
         dev = _device()
         self.device = dev
         self._dt = torch.zeros((), dtype=torch.float64, device=dev)
@@ -309,6 +315,14 @@ class HyperbolicSolver:
                                 _p(Q4), _p(out), _p(self._troubled),
                                 _stream()), "zj_step")
         return self._view(out)
+
+    def mood_stats(self):
+        """(steps redone, cells demoted) since setup (MOOD on)."""
+        out = np.zeros(2, np.int32)
+        L.check(L.lib().zj_solver_mood_stats(self._rt.handle,
+                                             out.ctypes.data),
+                "zj_solver_mood_stats")
+        return int(out[0]), int(out[1])
 
     def troubled_count(self) -> int:
         """Cells of the last step with h < 0 or a non-finite component
