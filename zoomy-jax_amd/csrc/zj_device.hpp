@@ -30,7 +30,7 @@ enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2 };
 
 constexpr int MAXF = 4;      // faces per cell (quad)
 constexpr int BLOCK = 256;   // threads per block (4 wave64)
-constexpr int SBLOCK = 320;  // stage kernel: 5 wave64, one per slope cell of a quad patch
+constexpr int SBLOCK = 256;  // stage kernel: 4 wave64 (3 blocks per CU in FP64, 4 in FP32)
 
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }

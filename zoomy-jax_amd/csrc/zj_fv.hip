@@ -692,7 +692,7 @@ __device__ __forceinline__ void lds_q4(const T* Qs, int i, T* q) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_stage(
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_stage(
     DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part,
