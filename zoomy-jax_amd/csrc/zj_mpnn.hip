@@ -308,7 +308,7 @@ __global__ void k_out(Grid g, const float* __restrict__ P, const bf16_t* __restr
 // summation order as the layer-by-layer kernels, hence the same result.
 constexpr int FT = 16;
 constexpr int FR0 = FT + 6;
-constexpr int FTH = 512;        // 8 waves
+constexpr int FTH = 1024;       // 16 waves (4 per SIMD)
 constexpr int HS = HID + 8;     // padded row (144 B): 16-byte fragment reads of
                                 // 16 consecutive rows hit distinct banks
 constexpr size_t FUSED_LDS = (size_t)FR0 * FR0 * HS * 2      // region 0 / 2
@@ -386,13 +386,8 @@ __global__ __launch_bounds__(FTH) void k_mpnn_fused(Grid g, const float* __restr
         const int sin = FR0 - 2 * l, sout = sin - 2;        // region sides
         const int off_out = 3 - (l + 1);                    // origin of the output region
         const int nrow = sout * sout, ntile = (nrow + 15) / 16;
-        bf16x8 wf[4][4], uf[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks) wf[nt][ks] = ld8(sw + ((nt * 4 + ks) * 64 + lane) * 8);
-            uf[nt] = ld8(sw + WF_SIZE + (nt * 64 + lane) * 8);
-        }
+        // weight fragments are read from LDS per tile (holding all 20 in
+        // registers would cost 80 VGPRs and halve the waves per SIMD)
         for (int t = wave; t < ntile; t += FTH / 64) {
             const int row = t * 16 + m;
             bf16x8 a[4], ap;
@@ -432,8 +427,9 @@ __global__ __launch_bounds__(FTH) void k_mpnn_fused(Grid g, const float* __restr
                 acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
                 gac[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int ks = 0; ks < 4; ++ks) acc[nt] = mfma(wf[nt][ks], a[ks], acc[nt]);
-                gac[nt] = mfma(uf[nt], ap, gac[nt]);
+                for (int ks = 0; ks < 4; ++ks)
+                    acc[nt] = mfma(ld8(sw + ((nt * 4 + ks) * 64 + lane) * 8), a[ks], acc[nt]);
+                gac[nt] = mfma(ld8(sw + WF_SIZE + (nt * 64 + lane) * 8), ap, gac[nt]);
             }
             if (row < nrow) {
                 const int ia = row / sout, ib = row - ia * sout;

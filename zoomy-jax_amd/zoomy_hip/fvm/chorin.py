@@ -134,7 +134,7 @@ class ChorinSplitVAMSolver:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
+        if h is not None and L is not None and L._lib is not None:
             L._lib.zj_chorin_destroy(h)
             self.handle = None
 

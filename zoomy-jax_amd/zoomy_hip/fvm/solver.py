@@ -148,7 +148,7 @@ class DeviceMesh:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
+        if h is not None and L is not None and L._lib is not None:
             L._lib.zj_mesh_destroy(h)
             self.handle = None
 
@@ -175,7 +175,7 @@ class _Runtime:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
+        if h is not None and L is not None and L._lib is not None:
             L._lib.zj_solver_destroy(h)
             self.handle = None
 

@@ -85,7 +85,7 @@ class MPPreconditioner:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
+        if h is not None and L is not None and L._lib is not None:
             L._lib.zj_mpnn_destroy(h)
             self.handle = None
 
