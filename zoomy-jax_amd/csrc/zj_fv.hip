@@ -680,29 +680,46 @@ __device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v
     return 0.5 * area;
 }
 
-template <typename T> struct Vec4;
-template <> struct Vec4<double> { using type = double4; };
-template <> struct Vec4<float> { using type = float4; };
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
 
+// The staged state is structure-of-arrays in pairs, (b, h) for the ML
+// loaded cells followed by (hu, hv): consecutive lanes then read
+// consecutive 2-word pairs, free of LDS bank conflicts.
 template <typename T>
-__device__ __forceinline__ void lds_q4(const T* Qs, int i, T* q) {
-    using V = typename Vec4<T>::type;
-    const V x = *reinterpret_cast<const V*>(Qs + 4 * i);
-    q[0] = x.x; q[1] = x.y; q[2] = x.z; q[3] = x.w;
+__device__ __forceinline__ void lds_q4(const typename Vec2<T>::type* Qs, int ML, int i, T* q) {
+    const auto a = Qs[i];
+    const auto b = Qs[ML + i];
+    q[0] = a.x; q[1] = a.y; q[2] = b.x; q[3] = b.y;
 }
 
-template <typename T>
+// SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
+// run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
+// HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
+// the options.  Both instantiations evaluate the same formulas.
+template <typename T, int SPEC>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_stage(
-    DevMesh m, DevPatch pt, Opts o, const T* __restrict__ Qin, const T* Q0, T* Qout,
+    DevMesh m_in, DevPatch pt, Opts o_in, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part,
     const uint8_t* __restrict__ o1_mask, uint8_t* __restrict__ bad_out,
     const int* __restrict__ gate) {
     // gate: a MOOD redo launch runs only when the candidate had troubled cells
     if (gate && *gate == 0) return;
-    // dynamic LDS, 32-byte aligned (the AoS state is read as double4)
-    extern __shared__ double4 smem_v4[];
-    double* smem_d = reinterpret_cast<double*>(smem_v4);
+    DevMesh m = m_in;
+    Opts o = o_in;
+    if (SPEC) {
+        m.dim = 2;
+        m.mf = 4;
+        o.lim = LIM_MINMOD;
+        o.rs = RS_HLL;
+        o.recon = RECON_CONS;
+    }
+    using T2 = typename Vec2<T>::type;
+    // dynamic LDS, 16-byte aligned (pairs of doubles)
+    extern __shared__ double2 smem_v2[];
+    double* smem_d = reinterpret_cast<double*>(smem_v2);
     const int p = xcd_block();
     const int tid = threadIdx.x;
     const int N = m.N;
@@ -713,9 +730,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     double2* Cs = (double2*)smem_d;         // ML centres
     double2* Vs = Cs + ML;                  // MV vertices
     double* Rs = (double*)(Vs + MV);        // MS inradius (final stage)
-    T* Qs = (T*)(Rs + MS);                  // ML x 4 states (AoS)
-    T* Hs = Qs + 4 * ML;                    // ML hinv
-    T* Ss = Hs + ML;                        // MS x 8 limited slopes
+    T2* Qs = (T2*)(Rs + MS);                // 2 x ML state pairs
+    T* Hs = (T*)(Qs + 2 * ML);              // ML hinv
+    T2* Ss = (T2*)(Hs + ML);                // 4 x MS limited slope pairs (x, y)
     const T eps = T(o.eps), g = T(o.g);
     const int* lids = pt.load_ids + (size_t)p * ML;
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
@@ -725,14 +742,14 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // ── phase 1: stage cells and vertices ────────────────────────────
     for (int i = tid; i < nload; i += SBLOCK) {
         const int gi = lids[i];
-        using V = typename Vec4<T>::type;
-        V x;
-        x.x = Qin[gi];
-        x.y = Qin[N + gi];
-        x.z = Qin[2 * N + gi];
-        x.w = Qin[3 * N + gi];
-        *reinterpret_cast<V*>(Qs + 4 * i) = x;
-        Hs[i] = hinv_of(x.y, eps);
+        T2 a, b;
+        a.x = Qin[gi];
+        a.y = Qin[N + gi];
+        b.x = Qin[2 * N + gi];
+        b.y = Qin[3 * N + gi];
+        Qs[i] = a;
+        Qs[ML + i] = b;
+        Hs[i] = hinv_of(a.y, eps);
         Cs[i] = pt.cen2[gi];
         if (final_stage && i < nslope) Rs[i] = m.rin[gi];
     }
@@ -750,7 +767,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
             T q[4];
-            lds_q4(Qs, ls, q);
+            lds_q4(Qs, ML, ls, q);
             const T hic = Hs[ls];
             T w[4];
             to_w(o.recon, q, hic, w);
@@ -770,7 +787,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 const uint16_t nl = nls[k];
                 if (nl != LNONE) {
                     T qn[4], wn[4];
-                    lds_q4(Qs, nl, qn);
+                    lds_q4(Qs, ML, nl, qn);
                     to_w(o.recon, qn, Hs[nl], wn);
                     const double2 cn = Cs[nl];
                     const double rx = cn.x - cc.x, ry = cn.y - cc.y;
@@ -818,7 +835,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 const uint16_t el = te[j * MS + ls];
                 if (el == LNONE) break;
                 T qn[4], wn[4];
-                lds_q4(Qs, el, qn);
+                lds_q4(Qs, ML, el, qn);
                 to_w(o.recon, qn, Hs[el], wn);
                 const double2 cn = Cs[el];
                 const double rx = cn.x - cc.x, ry = cn.y - cc.y;
@@ -900,12 +917,13 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) phi[vv] = T(0);
             }
-            T* sp = Ss + (size_t)ls * 8;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
                 T pv = clip01(phi[vv]);
-                sp[2 * vv] = pv * gx[vv];
-                sp[2 * vv + 1] = pv * gy[vv];
+                T2 sv;
+                sv.x = pv * gx[vv];
+                sv.y = pv * gy[vv];
+                Ss[vv * MS + ls] = sv;
             }
         }
         __syncthreads();
@@ -923,12 +941,16 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
         T q[4];
-        lds_q4(Qs, ls, q);
+        lds_q4(Qs, ML, ls, q);
         const T hic = Hs[ls];
         T sc[8];
         if (use_slopes) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) sc[i] = Ss[(size_t)ls * 8 + i];
+            for (int vv = 0; vv < 4; ++vv) {
+                const T2 sv = Ss[vv * MS + ls];
+                sc[2 * vv] = sv.x;
+                sc[2 * vv + 1] = sv.y;
+            }
         }
         double2 v[MAXF];
         double fx[MAXF], fy[MAXF];
@@ -949,12 +971,16 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             T hiR = hic;
             if (nl != LNONE) {
                 T qn[4];
-                lds_q4(Qs, nl, qn);
+                lds_q4(Qs, ML, nl, qn);
                 hiR = Hs[nl];
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) sn[i] = Ss[(size_t)nl * 8 + i];
+                    for (int vv = 0; vv < 4; ++vv) {
+                        const T2 sv = Ss[vv * MS + nl];
+                        sn[2 * vv] = sv.x;
+                        sn[2 * vv + 1] = sv.y;
+                    }
                     const double2 cn = Cs[nl];
                     recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
                 } else {
@@ -1294,9 +1320,14 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   const int* gate = nullptr) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
-    hipLaunchKernelGGL(k_stage<T>, dim3(M->pt.npatch), dim3(SBLOCK), lds, st, M->d, M->pt, s->o,
-                       Qin, Q0, Qout, dt, aux, mode, final_stage, write_b, use_slopes, troubled,
-                       dt_red ? s->dt_part : (double*)nullptr, o1_mask, bad_out, gate);
+    const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
+                      (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
+                      s->o.recon == RECON_CONS;
+    auto kern = spec ? k_stage<T, 1> : k_stage<T, 0>;
+    hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
+                       lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
+                       write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
+                       o1_mask, bad_out, gate);
     if (dt_red)
         hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(1024), 0, st, (const double*)s->dt_part,
                            M->pt.npatch * (SBLOCK / 64), dt_red, gate);
