@@ -611,16 +611,12 @@ __global__ __launch_bounds__(BLOCK) void k_imex(DevMesh m, Opts o, const double*
 // ─────────────────────────────────────────────────────────────────────────
 struct DevPatch {
     int P, npatch, ML, MS, MV;
-    const int* tpl_id;
-    const int* n_load;
-    const int* n_vert;
+    const int4* meta;  // per patch: template, loaded cells, vertices, slope cells | own << 16
     const int* load_ids;
     const int* vert_ids;
     const uint16_t* tpl_nbr;
     const uint16_t* tpl_extra;
     const uint16_t* tpl_vert;
-    const int* tpl_nslope;
-    const int* tpl_nown;
     const double2* cen2;
 };
 
@@ -724,9 +720,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const int tid = threadIdx.x;
     const int N = m.N;
     const int ML = pt.ML, MS = pt.MS, MV = pt.MV;
-    const int tpl = pt.tpl_id[p];
-    const int nload = pt.n_load[p], nvert = pt.n_vert[p];
-    const int nslope = pt.tpl_nslope[tpl], nown = pt.tpl_nown[tpl];
+    const int4 md = pt.meta[p];
+    const int tpl = md.x, nload = md.y, nvert = md.z;
+    const int nslope = md.w & 0xFFFF, nown = md.w >> 16;
+    const int c0 = p * pt.P;  // own cells are the ids c0 .. c0 + nown - 1
     double2* Cs = (double2*)smem_d;         // ML centres
     double2* Vs = Cs + ML;                  // MV vertices
     double* Rs = (double*)(Vs + MV);        // MS inradius (final stage)
@@ -740,29 +737,54 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
 
     // ── phase 1: stage cells and vertices ────────────────────────────
-    for (int i = tid; i < nload; i += SBLOCK) {
-        const int gi = lids[i];
-        T2 a, b;
-        a.x = Qin[gi];
-        a.y = Qin[N + gi];
-        b.x = Qin[2 * N + gi];
-        b.y = Qin[3 * N + gi];
-        Qs[i] = a;
-        Qs[ML + i] = b;
-        Hs[i] = hinv_of(a.y, eps);
-        Cs[i] = pt.cen2[gi];
-        if (final_stage && i < nslope) Rs[i] = m.rin[gi];
-    }
+    // Two slots per thread and pass, every global load of a pass issued
+    // before the first LDS store; a slot past the end reads cell c0 /
+    // vertex 0 and stores nothing.
     {
         const int* vids = pt.vert_ids + (size_t)p * MV;
-        for (int i = tid; i < nvert; i += SBLOCK) Vs[i] = m.verts[vids[i]];
+        const int nit = nload > nvert ? nload : nvert;
+        for (int i0 = tid; i0 < nit; i0 += 2 * SBLOCK) {
+            int gc[2], gv[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int i = i0 + u * SBLOCK;
+                gc[u] = i < nown ? c0 + i : (i < nload ? lids[i] : c0);
+                gv[u] = i < nvert ? vids[i] : 0;
+            }
+            T2 a[2], b[2];
+            double2 cc[2], vx[2];
+            double rr[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int gi = gc[u];
+                a[u].x = Qin[gi];
+                a[u].y = Qin[N + gi];
+                b[u].x = Qin[2 * N + gi];
+                b[u].y = Qin[3 * N + gi];
+                cc[u] = pt.cen2[gi];
+                rr[u] = final_stage ? m.rin[gi] : 0.0;
+                vx[u] = m.verts[gv[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int i = i0 + u * SBLOCK;
+                if (i < nload) {
+                    Qs[i] = a[u];
+                    Qs[ML + i] = b[u];
+                    Hs[i] = hinv_of(a[u].y, eps);
+                    Cs[i] = cc[u];
+                    if (final_stage && i < nslope) Rs[i] = rr[u];
+                }
+                if (i < nvert) Vs[i] = vx[u];
+            }
+        }
     }
     __syncthreads();
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
     if (use_slopes) {
         for (int ls = tid; ls < nslope; ls += SBLOCK) {
-            const int gi = lids[ls];
+            const int gi = ls < nown ? c0 + ls : lids[ls];
             uint16_t nls[MAXF];
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
@@ -936,7 +958,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     int gi_keep = -1;
     const int ls = tid;
     if (ls < nown) {
-        const int gi = lids[ls];
+        const int gi = c0 + ls;
         uint16_t nls[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
@@ -1227,16 +1249,20 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
             cen2[2 * (size_t)c + 1] = d->cell_center[N + c];
         }
         rc = 0;
-        rc |= dev_upload(m, ph.tpl_id.data(), sizeof(int) * ph.npatch, &p); pt.tpl_id = (const int*)p;
-        rc |= dev_upload(m, ph.n_load.data(), sizeof(int) * ph.npatch, &p); pt.n_load = (const int*)p;
-        rc |= dev_upload(m, ph.n_vert.data(), sizeof(int) * ph.npatch, &p); pt.n_vert = (const int*)p;
+        std::vector<int32_t> meta((size_t)4 * ph.npatch);
+        for (int q = 0; q < ph.npatch; ++q) {
+            const int t = ph.tpl_id[q];
+            meta[4 * (size_t)q] = t;
+            meta[4 * (size_t)q + 1] = ph.n_load[q];
+            meta[4 * (size_t)q + 2] = ph.n_vert[q];
+            meta[4 * (size_t)q + 3] = ph.tpl_nslope[t] | (ph.tpl_nown[t] << 16);
+        }
+        rc |= dev_upload(m, meta.data(), sizeof(int32_t) * meta.size(), &p); pt.meta = (const int4*)p;
         rc |= dev_upload(m, ph.load_ids.data(), sizeof(int) * ph.load_ids.size(), &p); pt.load_ids = (const int*)p;
         rc |= dev_upload(m, ph.vert_ids.data(), sizeof(int) * ph.vert_ids.size(), &p); pt.vert_ids = (const int*)p;
         rc |= dev_upload(m, ph.tpl_nbr.data(), 2 * ph.tpl_nbr.size(), &p); pt.tpl_nbr = (const uint16_t*)p;
         rc |= dev_upload(m, ph.tpl_extra.data(), 2 * ph.tpl_extra.size(), &p); pt.tpl_extra = (const uint16_t*)p;
         rc |= dev_upload(m, ph.tpl_vert.data(), 2 * ph.tpl_vert.size(), &p); pt.tpl_vert = (const uint16_t*)p;
-        rc |= dev_upload(m, ph.tpl_nslope.data(), sizeof(int) * ph.ntpl, &p); pt.tpl_nslope = (const int*)p;
-        rc |= dev_upload(m, ph.tpl_nown.data(), sizeof(int) * ph.ntpl, &p); pt.tpl_nown = (const int*)p;
         rc |= dev_upload(m, cen2.data(), sizeof(double) * cen2.size(), &p); pt.cen2 = (const double2*)p;
         // the kernels index m.mx stencil rows: keep at least one (all -1)
         if (mx == 0) {
