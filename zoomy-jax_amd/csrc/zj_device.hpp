@@ -175,17 +175,21 @@ __device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
     return v;  // valid on thread 0
 }
 
-// Barth-Jespersen / minmod limiter of one variable over all faces with two
-// divisions: min_f (dp / d_f) over d_f > eps equals dp / max_f d_f (and
+// Barth-Jespersen / minmod limiter of one variable over all faces with one
+// division: min_f (dp / d_f) over d_f > eps equals dp / max_f d_f (and
 // dn / min_f d_f over d_f < -eps), exactly, because correctly rounded
-// division is monotone in the divisor.
+// division is monotone in the divisor; of the two ratios (both >= 0, as
+// dp >= 0 >= dn) the smaller is picked by cross-multiplication,
+// dn / dmin < dp / dmax  <=>  dn dmax > dp dmin  (dmax > 0 > dmin),
+// which can pick the other one only where the two agree to rounding.
 template <typename T>
 __device__ __forceinline__ T bj_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
     const T eps = T(1e-30);
-    T phi = T(1);
-    if (dmax_pos > eps) phi = tmin(phi, clip01(dp / dmax_pos));
-    if (dmin_neg < -eps) phi = tmin(phi, clip01(dn / dmin_neg));
-    return phi;
+    const bool a1 = dmax_pos > eps, a2 = dmin_neg < -eps;
+    if (!a1 && !a2) return T(1);
+    const bool use2 = a2 && (!a1 || dn * dmax_pos > dp * dmin_neg);
+    const T num = use2 ? dn : dp, den = use2 ? dmin_neg : dmax_pos;
+    return clip01(num / den);
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs of the chip; remap the

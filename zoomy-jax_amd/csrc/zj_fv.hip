@@ -959,6 +959,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const int ls = tid;
     if (ls < nown) {
         const int gi = c0 + ls;
+        // the step state and dt of the RK average, fetched before the faces
+        T q0[4] = {0, 0, 0, 0};
+        if (mode == 1) load_q(Q0, N, gi, q0);
+        const T dt = mode == 2 ? T(0) : T(*dtp);
         uint16_t nls[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
@@ -1032,10 +1036,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) out[vv] = -acc[vv] * ivol;
         } else {
-            const T dt = T(*dtp);
             if (mode == 1) {
-                T q0[4];
-                load_q(Q0, N, gi, q0);
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) out[vv] = T(0.5) * (q0[vv] + (q[vv] - dt * (acc[vv] * ivol)));
             } else {
@@ -1043,7 +1044,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 for (int vv = 0; vv < 4; ++vv) out[vv] = q[vv] - dt * (acc[vv] * ivol);
             }
             if (final_stage && o.manning > 0.0) {
-                const T h0 = mode == 1 ? Q0[N + gi] : q[1];
+                const T h0 = mode == 1 ? q0[1] : q[1];
                 const T hs = hinv_of(h0, eps);
                 const T u = out[2] * hs, vv2 = out[3] * hs;
                 const T sp = sqrt(u * u + vv2 * vv2);
