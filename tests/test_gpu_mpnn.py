@@ -205,3 +205,40 @@ def test_training_reduces_vcycle_loss():
     hist = train(net, pairs, s, epochs=8, batch=2, lr=3e-3)
     after = np.mean([vcycle_loss(s, net, *pr, train=False) for pr in pairs])
     assert np.isfinite(hist).all() and after < 0.8 * before
+
+
+def test_trained_network_gain_over_the_vcycle_without_it():
+    """What the shipped trained network gains, on a workload it never saw
+    (512^2, the solitary wave over the seeded bed of the Chorin tests; the
+    training used 256^2 grids and other beds): against the same learned
+    V-cycle with the network switched off (output weights zero, so
+    z = x0 + P Vc R (r - S x0)), the energy-norm error of one V-cycle and
+    the inner FCG iterations of a 1e-8 solve."""
+    from zoomy_hip.learned import MPPreconditioner, vcycle_loss
+    n = 512
+    s, Q, dx = _chorin(n)
+    Qd = torch.as_tensor(Q, device="cuda")
+    b, h = Qd[0].contiguous(), Qd[1].contiguous()
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    f = torch.randn(n * n, generator=gen, dtype=torch.float64).cuda()
+    s.set_preconditioner("mg")
+    s.setup_operator(b, h)
+    p = s.solve(f)
+    nets = {"trained": MPPreconditioner.load(),
+            "without": MPPreconditioner(params=M.init_params(3, zero_out=True))}
+    loss, inner = {}, {}
+    for k, net in nets.items():
+        s.set_preconditioner("learned", net)
+        loss[k] = vcycle_loss(s, net, b, h, f, p, train=False)
+        s.setup_operator(b, h)
+        s.solve(f)
+        st = s.stats()
+        assert st["relres"] <= 1e-8
+        # a solve that needed the plain redo did not converge on its own
+        inner[k] = st["learned_inner"] if st["fallbacks"] == 0 else 10 ** 6
+    assert inner["trained"] < 10 ** 6
+    print(f"one V-cycle energy error: trained {loss['trained']:.4f}, "
+          f"without the network {loss['without']:.4f}; inner FCG iterations "
+          f"to 1e-8: trained {inner['trained']}, without {inner['without']}")
+    assert loss["trained"] < 0.25 * loss["without"]
+    assert inner["trained"] <= inner["without"]

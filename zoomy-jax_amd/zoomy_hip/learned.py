@@ -245,6 +245,24 @@ def solver_pairs(n_pairs=5000, n=256, dx=0.0625, seeds=range(100, 110),
     return pairs[:n_pairs], s
 
 
+def error_pairs(s, pairs, n_pairs, seed=5):
+    """(b, h, S e, e) pairs with random errors e (white, or smoothed by 1
+    or 3 passes) on the operators of ``pairs`` (every other one, in turn):
+    where the solver pairs weight the smooth error the coarse correction
+    already removes, these weight the oscillatory error the network acts on.
+    No solve is needed: the right-hand side is S e."""
+    gen = torch.Generator().manual_seed(seed)
+    ops = [(b, h) for (b, h, _, _) in pairs[::2]]
+    n = int(round(ops[0][0].numel() ** 0.5))
+    out = []
+    for k in range(n_pairs):
+        b, h = ops[k % len(ops)]
+        s.setup_operator(b, h)
+        e = _smooth_noise(n, gen, (0, 0, 1, 3)[k % 4]).to(s.device)
+        out.append((b, h, s.apply(e), e))
+    return out
+
+
 def _wave_state(n, L_, seed, x0_frac):
     rng = np.random.default_rng(seed)
     kx = rng.integers(1, 5) / L_
@@ -308,14 +326,18 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="train the learned Chorin "
                                  "preconditioner on solver-generated pairs")
     ap.add_argument("--pairs", type=int, default=5000)
+    ap.add_argument("--solver-pairs", type=int, default=3000,
+                    help="of --pairs, how many come from plain solves; the "
+                         "rest are error pairs")
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
-    pairs, s = solver_pairs(a.pairs, a.n, log=log)
+    pairs, s = solver_pairs(a.solver_pairs, a.n, log=log)
+    pairs = pairs + error_pairs(s, pairs, a.pairs - len(pairs))
     net = MPPreconditioner(seed=0)
     hist = train(net, pairs, s, a.epochs, a.batch, a.lr, log=log)
     pathlib.Path(a.out).parent.mkdir(parents=True, exist_ok=True)
