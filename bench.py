@@ -79,9 +79,19 @@ def gpu_leg(mesh, Q0, dtype, steps, warmup):
     ms = s.time_kernels(Q, dt, reps=20)
     mass = float(s.compute_mass(Q))
     finite = bool(torch.isfinite(Q).all())
+    Qh = Q.double().cpu().numpy()
     del s
     return dict(elapsed=el, ms=ms, sim_t=sim_t, troubled=troubled, mass=mass,
-                finite=finite)
+                finite=finite, Q=Qh)
+
+
+def rel_l2_rows(A, B, rows=(1, 2, 3)):
+    """Relative L2 difference of rows h, hu, hv (and hw for Chorin) of A
+    against B."""
+    import numpy as np
+    names = ("b", "h", "hu", "hv", "hw")
+    return {names[r]: float(np.linalg.norm(A[r] - B[r]) /
+                            max(np.linalg.norm(B[r]), 1e-300)) for r in rows}
 
 
 def cpu_leg(budget_s=12.0):
@@ -95,6 +105,7 @@ def cpu_leg(budget_s=12.0):
     o = Oracle(m, order=2, limiter="minmod", riemann="hll", cfl=0.9)
     t0 = time.perf_counter()
     k = 0
+    Q0 = Q
     while True:
         dt = o.dt(Q)
         Q, _ = o.step(dt, Q)
@@ -103,8 +114,19 @@ def cpu_leg(budget_s=12.0):
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
+    # the FP64 device march over the same sample, checked against the oracle
+    import torch
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    model = ShallowWater(dimension=2, order=2, limiter="minmod",
+                         riemann="hll", boundary_conditions="wall")
+    s = HyperbolicSolver(time_end=float("inf"), cfl=0.9)
+    s.setup_simulation(m, model)
+    Qd = torch.as_tensor(Q0, dtype=torch.float64, device="cuda").contiguous()
+    s.march_device(Qd, k)
+    err = rel_l2_rows(Qd.cpu().numpy(), Q)
+    del s
     return dict(value=k * m.n_cells / el, steps=k, cells=m.n_cells,
-                elapsed=el)
+                elapsed=el, gpu_vs_oracle=err)
 
 
 def measured_traffic(w, kernel):
@@ -210,7 +232,8 @@ def chorin_leg(precond, steps, warmup, net=None, n=CHORIN_N, L=CHORIN_L,
         s.solve(rhs)
     torch.cuda.synchronize()
     solve_ms = (time.perf_counter() - t1) * 1e3 / reps
-    out = {"ms_per_step": el * 1e3 / steps, "solve_ms": solve_ms}
+    out = {"ms_per_step": el * 1e3 / steps, "solve_ms": solve_ms,
+           "Q": Q.double().cpu().numpy()}
     N = n * n
     if net is not None:
         # one application of the network (fused kernel) on this grid, timed
@@ -287,6 +310,11 @@ def main():
                                        flux_dtype=torch.float32)
         chorin["learned_over_plain"] = (chorin["learned"]["ms_per_step"] /
                                         chorin["plain"]["ms_per_step"])
+        # the learned, mixed-precision march against the plain FP64 one
+        # after the same steps (pressure to 1e-8 in both, FP32 fluxes)
+        chorin["learned_vs_plain_rel_l2"] = rel_l2_rows(
+            chorin["learned"].pop("Q"), chorin["plain"].pop("Q"),
+            rows=(1, 2, 3, 4))
         if args.chorin_only:
             print(json.dumps({"chorin": chorin}))
             return
@@ -333,6 +361,10 @@ def main():
         "checks": {"finite": r64["finite"], "troubled": r64["troubled"],
                    "sim_time": r64["sim_t"], "mass": r64["mass"]},
     }
+    if cpu is not None:
+        out["checks"]["gpu_vs_oracle_rel_l2"] = cpu["gpu_vs_oracle"]
+        out["checks"]["gpu_vs_oracle_sample"] = (
+            f"256x256 sample of the same workload, {cpu['steps']} steps")
     if r32 is not None:
         cups32, roof32 = summarize(r32, 4)
         out["fp32"] = {"value": cups32,
@@ -340,7 +372,9 @@ def main():
                        "roofline": roof32, "kernel_ms": r32["ms"],
                        "checks": {"finite": r32["finite"],
                                   "troubled": r32["troubled"],
-                                  "mass": r32["mass"]}}
+                                  "mass": r32["mass"],
+                                  "rel_l2_vs_fp64": rel_l2_rows(r32["Q"],
+                                                                r64["Q"])}}
     if chorin is not None:
         out["chorin"] = chorin
     if cpu is not None:
