@@ -147,12 +147,16 @@ int zj_imex_local(zj_solver *s, const double *dt, int32_t maxiter,
  * S_cc = sum_f c_f a_c^2 + 4|A_c| hinv(h_c), S_ce = -c_f a_c a_e).
  * Pressure solves are host-synchronous once per chunk of CG iterations.
  * Replaces ChorinSplitVAMSolverJax (zoomy_jax/fvm/solver_chorin_vam_jax.py). */
-enum { ZJ_PRECOND_MG = 0, ZJ_PRECOND_LEARNED = 1 };
+/* Preconditioners of the FP64 CG: the FP64 V-cycle; the FP32 V-cycle
+ * (mixed precision, flexible CG); the FP32 V-cycle plus one BF16
+ * message-passing correction (learned, flexible CG).  The low-precision
+ * forms are checked by a fresh FP64 residual and redone plainly when they
+ * miss tol. */
+enum { ZJ_PRECOND_MG = 0, ZJ_PRECOND_LEARNED = 1, ZJ_PRECOND_MIXED = 2 };
 typedef struct zj_chorin zj_chorin;
 typedef struct zj_mpnn zj_mpnn;     /* learned preconditioner (zj_mpnn_*) */
 typedef struct {
     double tol;          /* relative residual target (1e-8)                  */
-    double inner_tol;    /* FP32 inner solve target of the refinement (1e-4) */
     double eps_wet;      /* hinv desingularisation (1e-8)                    */
     int32_t maxiter;     /* CG iteration cap per solve                       */
     int32_t nu;          /* smoothing sweeps per level and direction         */
@@ -160,11 +164,14 @@ typedef struct {
     int32_t min_coarse;  /* stop coarsening before a side drops below this   */
     int32_t precond;     /* ZJ_PRECOND_*                                     */
     int32_t time_order;  /* 1: one stage; 2: two stages + average            */
-    int32_t max_refine;  /* outer FP64 refinement sweeps (learned)           */
 } zj_chorin_params;
+/* Counters of the last zj_chorin_solve / _stage / _step call.  lp_*: the
+ * solves with a low-precision preconditioner (mixed or learned), their CG
+ * iterations and worst fresh FP64 residual; fallbacks: those redone
+ * plainly.  iterations / relres: of the accepted solves. */
 typedef struct {
-    int32_t solves, iterations, fallbacks, learned_inner, learned_outer;
-    double relres, learned_relres; /* worst relative residual of the call */
+    int32_t solves, iterations, fallbacks, lp_iterations, lp_solves;
+    double relres, lp_relres; /* worst relative residual of the call */
 } zj_chorin_stats;
 /* fv: an FP64 zj_solver of the same nx x ny mesh (hyperbolic predictor). */
 int zj_chorin_create(zj_solver *fv, int32_t nx, int32_t ny, double dx, double dy,
@@ -178,7 +185,7 @@ int zj_chorin_set_fv_order(zj_chorin *c, const int32_t *order);
 /* Mixed-precision predictor: the FV fluxes of the hyperbolic stage on fv32,
  * an FP32 zj_solver of the same cells in the same order as the FP64 one
  * (NULL: back to FP64).  The state, the pressure residual and the
- * refinement stay FP64. */
+ * pressure solve stay FP64. */
 int zj_chorin_set_fv32(zj_chorin *c, zj_solver *fv32);
 /* Fused multigrid kernels on/off (default on; used when nu == 1).  Both
  * give bit-identical V-cycles.  No counterpart in the reference. */

@@ -177,45 +177,19 @@ def adam(flat, grad, m, v, t, lr=3e-3, b1=0.9, b2=0.999, eps=1e-8):
 
 
 # ------------------------------------------------- the learned V-cycle
-def _rb_pair_from_zero(coef, f):
-    from oracle.chorin import rbgs
-    x = np.zeros_like(f)
-    nx, ny = f.shape
-    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
-    red = ((I + J) & 1) == 0
-    x[red] = f[red] / coef[0][red]
-    return rbgs(coef, x, f, 1)
-
-
 def learned_vcycle(flat, H, r, bf=True):
     """z = M r on level 0 of the hierarchy H (oracle.chorin.Hierarchy):
-    x0 = red-black pair from 0; x1 = x0 + N(r - S x0);
-    x2 = x1 + P Vc R (r - S x1); z = x2 + N(r - S x2).  Returns (z, caches)."""
+    x = V r (the plain V-cycle from level 0), z = x + N(r - S x).  Returns
+    (z, cache)."""
     from oracle.chorin import apply, vcycle
-    L0 = H.levels[0]
-    coef = L0["coef"]
-    diag = coef[0]
-    phi = features(*coef)
-    x = _rb_pair_from_zero(coef, r)
-    u1, c1 = forward(flat, r - apply(coef, x), diag, phi, bf)
-    x = x + u1
-    t2 = r - apply(coef, x)
-    ec = vcycle(H, L0["Px"].T @ t2 @ L0["Py"], level=1)
-    x = x + L0["Px"] @ ec @ L0["Py"].T
-    u3, c3 = forward(flat, r - apply(coef, x), diag, phi, bf)
-    return x + u3, (c1, c3)
+    coef = H.levels[0]["coef"]
+    x = vcycle(H, r, level=0)
+    u, cache = forward(flat, r - apply(coef, x), coef[0], features(*coef), bf)
+    return x + u, cache
 
 
-def learned_vcycle_backward(flat, H, caches, g, bf=True):
-    """Parameter gradient of sum(g * z) for learned_vcycle."""
-    from oracle.chorin import apply, vcycle
-    L0 = H.levels[0]
-    coef = L0["coef"]
-    diag = coef[0]
-    c1, c3 = caches
-    G3, a3 = backward(flat, c3, g, diag, bf)
-    dx2 = g - apply(coef, a3)
-    ec = vcycle(H, L0["Px"].T @ dx2 @ L0["Py"], level=1)
-    dx1 = dx2 - apply(coef, L0["Px"] @ ec @ L0["Py"].T)
-    G1, _ = backward(flat, c1, dx1, diag, bf)
-    return G1 + G3
+def learned_vcycle_backward(flat, H, cache, g, bf=True):
+    """Parameter gradient of sum(g * z) for learned_vcycle: the network input
+    r - S V r does not depend on the parameters."""
+    G, _ = backward(flat, cache, g, H.levels[0]["coef"][0], bf)
+    return G

@@ -154,6 +154,34 @@ def test_mg_cg_solve(n):
     assert s.stats()["iterations"] == 0 and float(p0.abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("n", [64, 256])
+def test_mixed_precision_solve(n):
+    """FP32 V-cycle inside FP64 flexible CG: the fresh FP64 residual reaches
+    1e-8, the iteration count stays within one of the FP64 MG-CG's, the
+    pressure agrees with a direct solve."""
+    import scipy.sparse.linalg as spla
+    box, Q, _ = _case(n)
+    s, _ = _solver(n, box)
+    dx = box[1] / n
+    Qd = _dev(Q)
+    f = np.random.default_rng(7).standard_normal(n * n)
+    s.setup_operator(Qd[0], Qd[1])
+    s.solve(_dev(f))
+    it_plain = s.stats()["iterations"]
+    s.set_preconditioner("mixed")
+    s.setup_operator(Qd[0], Qd[1])
+    p = s.solve(_dev(f)).cpu().numpy()
+    st = s.stats()
+    assert st["fallbacks"] == 0 and st["lp_solves"] == 1
+    assert st["relres"] <= 1e-8 and st["lp_relres"] == st["relres"]
+    assert abs(st["iterations"] - it_plain) <= 1
+    coef = C.coefficients(Q[1].reshape(n, n), Q[0].reshape(n, n), dx, dx)
+    res = f - C.apply(coef, p.reshape(n, n)).ravel()
+    assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(f)
+    pd = spla.spsolve(C.to_sparse(coef).tocsc(), f)
+    assert np.linalg.norm(p - pd) <= 1e-6 * np.linalg.norm(pd)
+
+
 def test_elliptic_solve_error_when_capped():
     from zoomy_hip.fvm import EllipticSolveError
     n = 64

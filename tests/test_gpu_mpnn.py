@@ -160,7 +160,7 @@ def test_learned_solve_reaches_target_and_falls_back():
     import scipy.sparse.linalg as spla
     n = 128
     s, Q, dx = _chorin(n)
-    net, _ = _net(1, zero_out=True)
+    net, _ = _net(1, zero_out=False)
     s.set_preconditioner("learned", net)
     Qd = torch.as_tensor(Q, device="cuda")
     s.setup_operator(Qd[0], Qd[1])
@@ -168,19 +168,22 @@ def test_learned_solve_reaches_target_and_falls_back():
     p = s.solve(torch.as_tensor(f, device="cuda")).cpu().numpy()
     st = s.stats()
     assert st["fallbacks"] == 0 and st["relres"] <= 1e-8
-    assert st["learned_outer"] >= 2 and st["learned_inner"] > 0
+    assert st["lp_solves"] == 1 and st["lp_iterations"] > 0
     coef = C.coefficients(Q[1].reshape(n, n), Q[0].reshape(n, n), dx, dx)
     res = f - C.apply(coef, p.reshape(n, n)).ravel()
     assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(f)
     pd = spla.spsolve(C.to_sparse(coef).tocsc(), f)
     assert np.linalg.norm(p - pd) <= 1e-6 * np.linalg.norm(pd)
-    # a refinement budget too small for 1e-8: redone plainly
-    s2, _, _ = _chorin(n, max_refine=1, inner_tol=1e-2)
-    s2.set_preconditioner("learned", net)
-    s2.setup_operator(Qd[0], Qd[1])
-    p2 = s2.solve(torch.as_tensor(f, device="cuda")).cpu().numpy()
-    st2 = s2.stats()
+    # a broken network (its products overflow to inf and NaN): the fresh
+    # FP64 residual misses the target and the solve is redone plainly
+    from zoomy_hip.learned import MPPreconditioner
+    bad = MPPreconditioner(params=np.full(M.N_PARAMS, 1e30, np.float32))
+    s.set_preconditioner("learned", bad)
+    s.setup_operator(Qd[0], Qd[1])
+    p2 = s.solve(torch.as_tensor(f, device="cuda")).cpu().numpy()
+    st2 = s.stats()
     assert st2["fallbacks"] == 1 and st2["relres"] <= 1e-8
+    assert not (st2["lp_relres"] <= 1e-8)
     assert np.linalg.norm(p2 - pd) <= 1e-6 * np.linalg.norm(pd)
 
 
@@ -211,9 +214,9 @@ def test_trained_network_gain_over_the_vcycle_without_it():
     """What the shipped trained network gains, on a workload it never saw
     (512^2, the solitary wave over the seeded bed of the Chorin tests; the
     training used 256^2 grids and other beds): against the same learned
-    V-cycle with the network switched off (output weights zero, so
-    z = x0 + P Vc R (r - S x0)), the energy-norm error of one V-cycle and
-    the inner FCG iterations of a 1e-8 solve."""
+    V-cycle with the network switched off (output weights zero, so z = V r,
+    the FP32 V-cycle of the mixed-precision solve), the energy-norm error of
+    one V-cycle and the flexible-CG iterations of a 1e-8 solve."""
     from zoomy_hip.learned import MPPreconditioner, vcycle_loss
     n = 512
     s, Q, dx = _chorin(n)
@@ -235,10 +238,10 @@ def test_trained_network_gain_over_the_vcycle_without_it():
         st = s.stats()
         assert st["relres"] <= 1e-8
         # a solve that needed the plain redo did not converge on its own
-        inner[k] = st["learned_inner"] if st["fallbacks"] == 0 else 10 ** 6
+        inner[k] = st["lp_iterations"] if st["fallbacks"] == 0 else 10 ** 6
     assert inner["trained"] < 10 ** 6
     print(f"one V-cycle energy error: trained {loss['trained']:.4f}, "
-          f"without the network {loss['without']:.4f}; inner FCG iterations "
+          f"without the network {loss['without']:.4f}; FCG iterations "
           f"to 1e-8: trained {inner['trained']}, without {inner['without']}")
-    assert loss["trained"] < 0.25 * loss["without"]
+    assert loss["trained"] < 0.5 * loss["without"]
     assert inner["trained"] <= inner["without"]

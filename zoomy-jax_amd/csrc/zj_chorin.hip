@@ -912,11 +912,11 @@ __global__ __launch_bounds__(CB) void k_cast(int n, const double* __restrict__ a
     if (c < n) o[c] = (T)a[c];
 }
 
-// x64 += e32 (iterative refinement update)
-__global__ __launch_bounds__(CB) void k_add_f32(int n, const float* __restrict__ e,
-                                                double* __restrict__ x) {
+// FP32 preconditioner output back to the FP64 Krylov vectors
+__global__ __launch_bounds__(CB) void k_widen(int n, const float* __restrict__ a,
+                                              double* __restrict__ o) {
     int c = blockIdx.x * CB + threadIdx.x;
-    if (c < n) x[c] += (double)e[c];
+    if (c < n) o[c] = (double)a[c];
 }
 
 inline int nblk(long n) { return (int)((n + CB - 1) / CB); }
@@ -1099,50 +1099,30 @@ static void vcycle(Hier& H, const T* r0, T* z0, int nu, int coarse_sweeps, const
 }
 
 // Learned V-cycle on level 0 of the FP32 hierarchy:
-//   x0 = red-black sweep pair from zero
-//   x1 = x0 + N(r - S x0)
-//   x2 = x1 + P Vc R (r - S x1)        Vc: the V-cycle from level 1
-//   z  = x2 + N(r - S x2)
-// train != 0 keeps the network activations (slots 0 and 1) for
-// learned_vcycle_backward.  Needs at least two levels.
+//   x = V r                  (the plain V-cycle, FP32)
+//   z = x + N(r - S x)       (one network correction)
+// train != 0 keeps the network activations (slot 0) for
+// learned_vcycle_backward.
 static int learned_vcycle(Hier& H, zj_mpnn* net, const float* r, float* z, int nu,
                           int coarse_sweeps, const int* done, int train, hipStream_t st) {
     HLevel& L = H.levels[0];
-    HLevel& C = H.levels[1];
     const float *D = (float*)L.diag, *E = (float*)L.cE, *Nn = (float*)L.cN;
     float* t = (float*)L.r;
-    const int n = L.nx * L.ny, half = L.nx * ((L.ny + 1) >> 1);
-    k_rbgs<float><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, 0, 1, done);
-    k_rbgs<float><<<nblk(half), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, 1, 0, done);
+    const int n = L.nx * L.ny;
+    vcycle<float>(H, r, z, nu, coarse_sweeps, done, st);
     k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
-    if (int rc = mpnn_forward_f32(net, L.nx, L.ny, t, D, E, Nn, z, 1, train ? 0 : 2, done, st))
-        return rc;
-    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
-    k_restrict<float><<<nblk(C.nx * C.ny), CB, 0, st>>>(C.nx, C.ny, t, (float*)C.f, done);
-    vcycle<float>(H, (const float*)C.f, (float*)C.x, nu, coarse_sweeps, done, st, 1);
-    k_prolong<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (float*)C.x, z, done);
-    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, r, t, done);
-    return mpnn_forward_f32(net, L.nx, L.ny, t, D, E, Nn, z, 1, train ? 1 : 2, done, st);
+    return mpnn_forward_f32(net, L.nx, L.ny, t, D, E, Nn, z, 1, train ? 0 : 2, done, st);
 }
 
 // Parameter gradients of sum(g * z) for the last training learned_vcycle:
-//   a3 = N2^T g;  dx2 = g - S a3;  dx1 = dx2 - S P Vc R dx2;  N1^T dx1
-// (R = P^T and Vc is symmetric).
+// the network input r - S V r does not depend on the parameters, so this is
+// the network's own backward with dz = g.
 static int learned_vcycle_backward(Hier& H, zj_mpnn* net, const float* g, int nu,
                                    int coarse_sweeps, hipStream_t st) {
+    (void)nu;
+    (void)coarse_sweeps;
     HLevel& L = H.levels[0];
-    HLevel& C = H.levels[1];
-    const float *D = (float*)L.diag, *E = (float*)L.cE, *Nn = (float*)L.cN;
-    float *a3 = (float*)L.x, *dx2 = (float*)L.f, *v = (float*)L.r;
-    const int n = L.nx * L.ny;
-    if (int rc = mpnn_backward_f32(net, L.nx, L.ny, g, D, a3, 1, st)) return rc;
-    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, a3, g, dx2, nullptr);
-    k_restrict<float><<<nblk(C.nx * C.ny), CB, 0, st>>>(C.nx, C.ny, dx2, (float*)C.f, nullptr);
-    vcycle<float>(H, (const float*)C.f, (float*)C.x, nu, coarse_sweeps, nullptr, st, 1);
-    k_fill<float><<<nblk(n), CB, 0, st>>>(n, 0.f, v);
-    k_prolong<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, (float*)C.x, v, nullptr);
-    k_resid<float><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, D, E, Nn, v, dx2, a3, nullptr);
-    return mpnn_backward_f32(net, L.nx, L.ny, a3, D, nullptr, 0, st);
+    return mpnn_backward_f32(net, L.nx, L.ny, g, (float*)L.diag, nullptr, 0, st);
 }
 
 }  // namespace zj
@@ -1162,20 +1142,20 @@ struct zj_chorin {
     zj_solver* fv32 = nullptr;
     float *Qm32 = nullptr, *dQ32 = nullptr;
     double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
-    // FP32 vectors of the inner solve of the refinement
-    float *r32 = nullptr, *x32 = nullptr, *s32 = nullptr, *z32 = nullptr, *d32 = nullptr,
-          *q32 = nullptr, *ro32 = nullptr;
+    // the FP32 input and output of the low-precision preconditioner and the
+    // previous residual of flexible CG
+    float *r32 = nullptr, *z32 = nullptr;
+    double* r_old = nullptr;
     double* part = nullptr;  // reduction partials (2 * RED_MAX)
     double* sc = nullptr;    // CG scalars
     int* fl = nullptr;       // {done, iterations}
     int* host_fl = nullptr;  // pinned copy of fl
     double* host_sc = nullptr;
     zj_mpnn* net = nullptr;  // learned preconditioner (may be NULL)
-    // iterations of the last solve (plain) and of each refinement sweep
-    // (learned): the size of the first chunk of iterations enqueued before
-    // the host reads the convergence flag
-    int last_iters = 8;
-    int last_iters32[8] = {4, 4, 4, 4, 4, 4, 4, 4};
+    // iterations of the last solve per preconditioner (ZJ_PRECOND_*): the
+    // size of the first chunk of iterations enqueued before the host reads
+    // the convergence flag
+    int last_iters[3] = {8, 8, 8};
     zj_chorin_stats stats{};
     std::vector<void*> allocs;
 };
@@ -1199,60 +1179,74 @@ static int read_flags(zj_chorin* c, hipStream_t st) {
     return ZJ_OK;
 }
 
-// Preconditioned CG on level 0 of H (T = double: plain MG-CG; T = float with
-// `learned`: flexible CG with the learned preconditioner).  x holds the
-// initial guess when x0 != 0.  Host-synchronous once per chunk of iterations.
-template <typename T>
-static int pcg(zj_chorin* c, zj::Hier& H, const T* rhs, T* x, int x0, T* r, T* z, T* d, T* q,
-               T* r_old, double tol, int maxit, int learned, hipStream_t st, int* iters,
-               double* relres, int sweep = 0) {
+// FP64 preconditioned CG on level 0 of the FP64 hierarchy.  pmode:
+//   ZJ_PRECOND_MG       the FP64 V-cycle, CG
+//   ZJ_PRECOND_MIXED    the FP32 V-cycle of the FP32 hierarchy, flexible CG
+//   ZJ_PRECOND_LEARNED  the FP32 V-cycle plus the BF16 network correction,
+//                       flexible CG (Polak-Ribiere beta)
+// The Krylov vectors, the operator products and the residual stay FP64; the
+// low-precision preconditioners receive the residual rounded to FP32 and
+// return an FP32 correction.  x holds the initial guess when x0 != 0.
+// Host-synchronous once per chunk of iterations.
+static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, int maxit,
+               int pmode, hipStream_t st, int* iters, double* relres) {
+    zj::Hier& H = c->H64;
     zj::HLevel& L = H.levels[0];
     const int n = L.nx * L.ny;
     const int nr = nred(n);
-    const T *D = (T*)L.diag, *E = (T*)L.cE, *Nn = (T*)L.cN;
+    const double *D = (double*)L.diag, *E = (double*)L.cE, *Nn = (double*)L.cN;
+    double *r = c->r, *z = c->z, *d = c->d, *q = c->q;
+    const int flex = pmode != ZJ_PRECOND_MG;
+    double* r_old = flex ? c->r_old : nullptr;
     const double tol2 = tol * tol;
-    auto precond = [&](const T* rin, T* zout) -> int {
-        if (learned) {
-            int rc = zj::learned_vcycle(H, c->net, (const float*)rin, (float*)zout, c->prm.nu,
-                                        c->prm.coarse_sweeps, c->fl, 0, st);
-            if (rc) return rc;
-        } else {
-            zj::vcycle<T>(H, rin, zout, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
+    auto precond = [&](const double* rin, double* zout) -> int {
+        if (!flex) {
+            zj::vcycle<double>(H, rin, zout, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
+            return ZJ_OK;
         }
+        k_cast<float><<<nblk(n), CB, 0, st>>>(n, rin, c->r32);
+        if (pmode == ZJ_PRECOND_LEARNED) {
+            if (int rc = zj::learned_vcycle(c->H32, c->net, c->r32, c->z32, c->prm.nu,
+                                            c->prm.coarse_sweeps, c->fl, 0, st))
+                return rc;
+        } else {
+            zj::vcycle<float>(c->H32, c->r32, c->z32, c->prm.nu, c->prm.coarse_sweeps, c->fl,
+                              st);
+        }
+        k_widen<<<nblk(n), CB, 0, st>>>(n, c->z32, zout);
         return ZJ_OK;
     };
     auto red = [&](int op) { return Red{c->part, c->sc, c->fl, tol2, maxit, op}; };
     auto fin = [&](int op, int two = 0) { k_fin<<<1, RED_MAX, 0, st>>>(red(op), nr, two); };
-    k_cg_init<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, red(FIN_B2));
+    k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, red(FIN_B2));
     fin(FIN_B2);
     if (x0) {
-        k_dot<T><<<nr, CB, 0, st>>>(n, r, r, red(FIN_RR0), c->fl);
+        k_dot<double><<<nr, CB, 0, st>>>(n, r, r, red(FIN_RR0), c->fl);
         fin(FIN_RR0);
     }
     if (int rc = precond(r, z)) return rc;
-    k_dot<T><<<nr, CB, 0, st>>>(n, r, z, red(FIN_RZ0), c->fl);
+    k_dot<double><<<nr, CB, 0, st>>>(n, r, z, red(FIN_RZ0), c->fl);
     fin(FIN_RZ0);
-    k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
+    k_xpby<double><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
     HIPCHK(hipGetLastError());
     int launched = 0;
-    int& last = learned ? c->last_iters32[std::min(sweep, 7)] : c->last_iters;
+    int& last = c->last_iters[pmode];
     int chunk = std::max(1, std::min(maxit, last));
     while (true) {
         for (int k = 0; k < chunk; ++k) {
-            k_spmv<T><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, red(FIN_PQ), c->fl);
+            k_spmv<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, red(FIN_PQ), c->fl);
             fin(FIN_PQ);
-            k_axpy2<T><<<nr, CB, 0, st>>>(n, x, r, learned ? r_old : nullptr, d, q, red(FIN_RR),
-                                          c->fl);
+            k_axpy2<double><<<nr, CB, 0, st>>>(n, x, r, r_old, d, q, red(FIN_RR), c->fl);
             fin(FIN_RR);
             if (int rc = precond(r, z)) return rc;
-            if (learned) {
-                k_dot2<T><<<nr, CB, 0, st>>>(n, z, r, r_old, red(FIN_RZ_FLEX), c->fl);
+            if (flex) {
+                k_dot2<double><<<nr, CB, 0, st>>>(n, z, r, r_old, red(FIN_RZ_FLEX), c->fl);
                 fin(FIN_RZ_FLEX, 1);
             } else {
-                k_dot<T><<<nr, CB, 0, st>>>(n, z, r, red(FIN_RZ), c->fl);
+                k_dot<double><<<nr, CB, 0, st>>>(n, z, r, red(FIN_RZ), c->fl);
                 fin(FIN_RZ);
             }
-            k_xpby<T><<<nr, CB, 0, st>>>(n, z, d, c->sc, 0, c->fl);
+            k_xpby<double><<<nr, CB, 0, st>>>(n, z, d, c->sc, 0, c->fl);
         }
         HIPCHK(hipGetLastError());
         launched += chunk;
@@ -1287,47 +1281,32 @@ static int residual64(zj_chorin* c, const double* rhs, const double* x, double* 
     return ZJ_OK;
 }
 
-// Solve S p = rhs on the current hierarchy.  precond: plain FP64 MG-CG, or
-// FP32 flexible CG with the learned preconditioner inside FP64 iterative
-// refinement, redone plainly when the refined residual misses the target.
+// Solve S p = rhs on the current hierarchy to the relative residual tol.
+// With a low-precision preconditioner (mixed or learned) the FP64 residual
+// of the result is recomputed from scratch (rhs - S p, not the recurrence)
+// and the solve is redone with the plain FP64 MG-CG when it misses tol.
 static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStream_t st) {
     zj_chorin_stats& S = c->stats;
     const double tol = c->prm.tol;
     int its = 0;
     double rel = 0.0;
     S.solves += 1;
-    if (precond == ZJ_PRECOND_LEARNED && c->net) {
-        // outer FP64 refinement from p = 0 (r = rhs)
-        int outer = 0, inner_total = 0, rc = ZJ_OK;
-        k_fill<double><<<nblk(c->N), CB, 0, st>>>(c->N, 0.0, p);
-        const double* res = rhs;
-        while (outer < c->prm.max_refine) {
-            k_cast<float><<<nblk(c->N), CB, 0, st>>>(c->N, res, c->r32);
-            int it = 0;
-            double rel32 = 0.0;
-            rc = pcg<float>(c, c->H32, c->r32, c->x32, 0, c->s32, c->z32, c->d32, c->q32,
-                            c->ro32, c->prm.inner_tol, c->prm.maxiter, 1, st, &it, &rel32,
-                            outer);
-            inner_total += it;
-            if (rc && rc != ZJ_ESOLVE) return rc;
-            k_add_f32<<<nblk(c->N), CB, 0, st>>>(c->N, c->x32, p);
-            ++outer;
-            if ((rc = residual64(c, rhs, p, c->r, &rel, st))) return rc;
-            res = c->r;
-            if (rel <= tol) break;
-        }
-        S.learned_inner += inner_total;
-        S.learned_outer += outer;
-        S.learned_relres = std::max(S.learned_relres, rel);
-        if (rel <= tol) {
-            S.iterations += inner_total;
-            S.relres = std::max(S.relres, rel);
+    if (precond == ZJ_PRECOND_MIXED || (precond == ZJ_PRECOND_LEARNED && c->net)) {
+        int rc = pcg(c, rhs, p, 0, tol, c->prm.maxiter, precond, st, &its, &rel);
+        if (rc && rc != ZJ_ESOLVE) return rc;
+        double true_rel = 0.0;
+        if ((rc = residual64(c, rhs, p, c->r_old, &true_rel, st))) return rc;
+        S.lp_iterations += its;
+        S.lp_solves += 1;
+        if (!(true_rel <= S.lp_relres)) S.lp_relres = true_rel;
+        if (true_rel <= tol) {
+            S.iterations += its;
+            S.relres = std::max(S.relres, true_rel);
             return ZJ_OK;
         }
         S.fallbacks += 1;  // redo plainly
     }
-    int rc = pcg<double>(c, c->H64, rhs, p, 0, c->r, c->z, c->d, c->q, nullptr, tol,
-                         c->prm.maxiter, 0, st, &its, &rel);
+    int rc = pcg(c, rhs, p, 0, tol, c->prm.maxiter, ZJ_PRECOND_MG, st, &its, &rel);
     if (rc && rc != ZJ_ESOLVE) return rc;
     S.iterations += its;
     S.relres = std::max(S.relres, rel);
@@ -1338,7 +1317,7 @@ static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStr
 static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStream_t st) {
     zj::mpnn_invalidate_phi(c->net);
     zj::build_levels<double>(c->H64, h, b, c->eps, st);
-    if (c->prm.precond == ZJ_PRECOND_LEARNED) zj::build_levels<float>(c->H32, h, b, c->eps, st);
+    if (c->prm.precond != ZJ_PRECOND_MG) zj::build_levels<float>(c->H32, h, b, c->eps, st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1386,7 +1365,8 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
                      const zj_chorin_params* prm, zj_chorin** out) {
     if (!out || nx < 2 || ny < 2 || !(dx > 0) || !(dy > 0) || !prm) return ZJ_EINVAL;
     if (prm->tol <= 0 || prm->maxiter < 1 || prm->nu < 1 || prm->coarse_sweeps < 1 ||
-        prm->min_coarse < 1 || (prm->time_order != 1 && prm->time_order != 2))
+        prm->min_coarse < 1 || (prm->time_order != 1 && prm->time_order != 2) ||
+        prm->precond < ZJ_PRECOND_MG || prm->precond > ZJ_PRECOND_MIXED)
         return ZJ_EINVAL;
     zj_chorin* c = new (std::nothrow) zj_chorin();
     if (!c) return ZJ_ENOMEM;
@@ -1398,8 +1378,6 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
     c->dy = dy;
     c->eps = prm->eps_wet > 0 ? prm->eps_wet : 1e-8;
     c->prm = *prm;
-    if (c->prm.max_refine < 1) c->prm.max_refine = 4;
-    if (!(c->prm.inner_tol > 0)) c->prm.inner_tol = 1e-4;
     const size_t N = c->N;
     int rc = c->H64.alloc(nx, ny, dx, dy, prm->min_coarse, sizeof(double));
     if (!rc) rc = c->H32.alloc(nx, ny, dx, dy, prm->min_coarse, sizeof(float));
@@ -1413,12 +1391,8 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
     if (!rc) rc = dalloc(c, &c->d, N);
     if (!rc) rc = dalloc(c, &c->q, N);
     if (!rc) rc = dalloc(c, &c->r32, N);
-    if (!rc) rc = dalloc(c, &c->x32, N);
-    if (!rc) rc = dalloc(c, &c->s32, N);
     if (!rc) rc = dalloc(c, &c->z32, N);
-    if (!rc) rc = dalloc(c, &c->d32, N);
-    if (!rc) rc = dalloc(c, &c->q32, N);
-    if (!rc) rc = dalloc(c, &c->ro32, N);
+    if (!rc) rc = dalloc(c, &c->r_old, N);
     if (!rc) rc = dalloc(c, &c->part, 2 * RED_MAX);
     if (!rc) rc = dalloc(c, &c->sc, CG_NSLOT);
     if (!rc) rc = dalloc(c, &c->fl, 4);
@@ -1488,7 +1462,9 @@ int zj_chorin_set_mg_fused(zj_chorin* c, int32_t on) {
 }
 
 int zj_chorin_set_precond(zj_chorin* c, int32_t precond, zj_mpnn* net) {
-    if (!c || (precond != ZJ_PRECOND_MG && precond != ZJ_PRECOND_LEARNED)) return ZJ_EINVAL;
+    if (!c || (precond != ZJ_PRECOND_MG && precond != ZJ_PRECOND_LEARNED &&
+               precond != ZJ_PRECOND_MIXED))
+        return ZJ_EINVAL;
     if (precond == ZJ_PRECOND_LEARNED && !net) return ZJ_EINVAL;
     c->prm.precond = precond;
     c->net = net;

@@ -12,7 +12,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libzoomy_hip.so"
 
 ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP, ZJ_ESOLVE = 0, 1, 2, 3, 4, 5
-PRECOND = {"mg": 0, "learned": 1}
+PRECOND = {"mg": 0, "learned": 1, "mixed": 2}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
@@ -49,21 +49,20 @@ class Params(ctypes.Structure):
 
 
 class ChorinParams(ctypes.Structure):
-    _fields_ = [("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+    _fields_ = [("tol", ctypes.c_double),
                 ("eps_wet", ctypes.c_double), ("maxiter", ctypes.c_int32),
                 ("nu", ctypes.c_int32), ("coarse_sweeps", ctypes.c_int32),
                 ("min_coarse", ctypes.c_int32), ("precond", ctypes.c_int32),
-                ("time_order", ctypes.c_int32),
-                ("max_refine", ctypes.c_int32)]
+                ("time_order", ctypes.c_int32)]
 
 
 class ChorinStats(ctypes.Structure):
     _fields_ = [("solves", ctypes.c_int32), ("iterations", ctypes.c_int32),
                 ("fallbacks", ctypes.c_int32),
-                ("learned_inner", ctypes.c_int32),
-                ("learned_outer", ctypes.c_int32),
+                ("lp_iterations", ctypes.c_int32),
+                ("lp_solves", ctypes.c_int32),
                 ("relres", ctypes.c_double),
-                ("learned_relres", ctypes.c_double)]
+                ("lp_relres", ctypes.c_double)]
 
 
 _lib = None

@@ -18,10 +18,11 @@ The state is a ``(5, N)`` float64 tensor with rows ``(b, h, hu, hv, hw)`` on
 an ``nx x ny`` wall-bounded ``Mesh.quad`` grid (cell id ``i*ny + j``).  The
 pressure equation ``S p = -D(Q*)/dt`` has the symmetric positive definite
 5-point operator of the depth-averaged non-hydrostatic model (linear vertical
-pressure profile, bottom pressure 2p).  Preconditioners: ``"mg"`` (FP64
-geometric multigrid V-cycle inside CG) and ``"learned"`` (FP32 flexible CG
-with the BF16 message-passing network inside FP64 iterative refinement, redone
-with ``"mg"`` when the refined residual misses ``tol``).
+pressure profile, bottom pressure 2p).  Preconditioners of the FP64 CG:
+``"mg"`` (FP64 geometric multigrid V-cycle), ``"mixed"`` (the FP32 V-cycle,
+flexible CG) and ``"learned"`` (the FP32 V-cycle plus one correction by the
+BF16 message-passing network, flexible CG); a low-precision solve whose fresh
+FP64 residual misses ``tol`` is redone with ``"mg"``.
 """
 from __future__ import annotations
 
@@ -56,7 +57,6 @@ class ChorinSplitVAMSolver:
                  tol: float = 1e-8, time_order: int = 2,
                  precond: str = "mg", maxiter: int = 200, nu: int = 1,
                  coarse_sweeps: int = 2, min_coarse: int = 4,
-                 inner_tol: float = 1e-4, max_refine: int = 4,
                  flux_dtype: torch.dtype = torch.float64):
         if precond not in L.PRECOND:
             raise ValueError(f"precond must be one of {sorted(L.PRECOND)}")
@@ -69,11 +69,11 @@ class ChorinSplitVAMSolver:
         self.tol = float(tol)
         self.time_order = int(time_order)
         self.precond = precond
-        self._prm = L.ChorinParams(float(tol), float(inner_tol),
+        self._prm = L.ChorinParams(float(tol),
                                    float(self.model.wet_dry_eps), int(maxiter),
                                    int(nu), int(coarse_sweeps),
                                    int(min_coarse), L.PRECOND[precond],
-                                   int(time_order), int(max_refine))
+                                   int(time_order))
         if flux_dtype not in (torch.float64, torch.float32):
             raise ValueError("flux_dtype must be torch.float64 or float32")
         self.flux_dtype = flux_dtype
@@ -144,8 +144,11 @@ class ChorinSplitVAMSolver:
                 "zj_chorin_set_mg_fused")
 
     def set_preconditioner(self, precond: str, net=None):
-        """``"mg"``, or ``"learned"`` with a :class:`~zoomy_hip.learned.
-        MPPreconditioner` ``net``."""
+        """``"mg"`` (FP64 V-cycle), ``"mixed"`` (FP32 V-cycle inside FP64
+        flexible CG) or ``"learned"`` (FP32 V-cycle plus the BF16 network
+        correction of the :class:`~zoomy_hip.learned.MPPreconditioner`
+        ``net``, inside FP64 flexible CG).  A low-precision solve whose fresh
+        FP64 residual misses ``tol`` is redone with ``"mg"``."""
         if precond not in L.PRECOND:
             raise ValueError(f"precond must be one of {sorted(L.PRECOND)}")
         nh = None
@@ -276,8 +279,8 @@ class ChorinSplitVAMSolver:
         return z
 
     def learned_vcycle(self, r, train=False):
-        """z = M r with the learned V-cycle (FP32; the learned
-        preconditioner must be set)."""
+        """z = M r with the learned V-cycle, z = V r + N(r - S V r) in
+        FP32 (the learned preconditioner must be set)."""
         if r.dtype != torch.float32 or r.numel() != self.N or \
                 not r.is_contiguous():
             raise ValueError("r must be a contiguous float32 vector")

@@ -330,7 +330,7 @@ def main(argv=None):
                     help="of --pairs, how many come from plain solves; the "
                          "rest are error pairs")
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
