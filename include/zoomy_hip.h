@@ -214,7 +214,10 @@ int zj_chorin_setup_operator(zj_chorin *c, const double *b, const double *h, voi
 int zj_chorin_apply(zj_chorin *c, const double *x, double *y, void *stream);
 /* z = V-cycle(r) on the current operator, FP64 or FP32 hierarchy. */
 int zj_chorin_vcycle(zj_chorin *c, const void *r, void *z, int32_t dtype, void *stream);
-/* Solve S p = rhs with the selected preconditioner (GMRES of :800-805). */
+/* Solve S p = rhs with the selected preconditioner (replaces the GMRES of
+ * :800-805): FP64 CG, flexible for the mixed and learned preconditioners,
+ * whose results are accepted only when a fresh FP64 residual meets tol and
+ * are redone with the FP64 V-cycle otherwise. */
 int zj_chorin_solve(zj_chorin *c, const double *rhs, double *p, void *stream);
 /* Number of multigrid levels and their (nx, ny). */
 int zj_chorin_levels(zj_chorin *c, int32_t *n_levels, int32_t *dims_out, int32_t max_levels);
@@ -222,9 +225,9 @@ int zj_chorin_levels(zj_chorin *c, int32_t *n_levels, int32_t *dims_out, int32_t
 int zj_chorin_get_stats(zj_chorin *c, zj_chorin_stats *out);
 /* The learned V-cycle z = M r (FP32, level 0 of the current operator; needs
  * the network set by zj_chorin_set_precond):
- *   x0 = red-black sweep pair from 0;  x1 = x0 + N(r - S x0);
- *   x2 = x1 + P Vc R (r - S x1);      z = x2 + N(r - S x2).
- * Placement follows the reference's learned V-cycle smoother prior
+ *   x = V r (the FP32 V-cycle);  z = x + N(r - S x).
+ * The network is a learned smoother, the role of smooth_step in the
+ * reference's learned V-cycle prior
  * (zoomy_jax/gnn_blueprint/vcycle_structured_gnn.py:74-122, :145-200).
  * train != 0 keeps the activations for the backward below. */
 int zj_chorin_learned_vcycle(zj_chorin *c, const float *r, float *z, int32_t train,
