@@ -3,14 +3,14 @@
 The network (``zj_mpnn_*`` in the C ABI) is the reference's message-passing
 layer (``zoomy_jax/gnn_blueprint/graph_mp_multibranch.py:9-16``) made linear
 in the residual: three gated layers of hidden width 64 on the graph of 2x2
-cell blocks, BF16 MFMA products with FP32 accumulation.  It is placed as the
-fine-level smoother of a multigrid V-cycle, as in the reference's learned
-V-cycle (``vcycle_structured_gnn.py:74-122``, ``:145-200``):
+cell blocks, BF16 MFMA products with FP32 accumulation.  It is a learned
+smoother after the FP32 multigrid V-cycle V (the role of ``smooth_step`` in
+the reference's learned V-cycle, ``vcycle_structured_gnn.py:74-122``,
+``:145-200``):
 
-    x0 = red-black sweep pair from 0;   x1 = x0 + N(r - S x0)
-    x2 = x1 + P Vc R (r - S x1);        z  = x2 + N(r - S x2)
+    x = V r;    z = x + N(r - S x)
 
-and used by FP32 flexible CG inside FP64 iterative refinement.
+and the preconditioner of FP64 flexible CG.
 
 Training (``train``) follows the reference's recipe (Adam, learning rate of
 order 1e-3, target the exact solution u* = A^-1 f,
@@ -19,9 +19,9 @@ V-cycle measured in the operator's energy norm,
 
     loss = (z - p)^T S (z - p) / (p^T rhs),     z = M rhs,
 
-on (rhs, p) pairs generated by the plain solver (``solver_pairs``).  The
-gradient flows through the whole V-cycle (network backward, S, R, P and the
-coarse V-cycle, which is symmetric).
+on (rhs, p) pairs generated by the plain solver (``solver_pairs``) and error
+pairs (S e, e) (``error_pairs``).  The network's input r - S V r does not
+depend on its parameters, so the gradient is the network's own backward.
 """
 from __future__ import annotations
 
