@@ -34,7 +34,10 @@ enum { ZJ_BC_EXTRAPOLATION = 0, ZJ_BC_WALL = 1, ZJ_BC_PERIODIC = 2 };
 enum { ZJ_LIM_NONE = 0, ZJ_LIM_BARTH_JESPERSEN = 1, ZJ_LIM_MINMOD = 2,
        ZJ_LIM_VENKATAKRISHNAN = 3, ZJ_LIM_VANLEER = 4 };
 enum { ZJ_RIEMANN_RUSANOV_NC = 0, ZJ_RIEMANN_RUSANOV = 1, ZJ_RIEMANN_HLL = 2 };
-enum { ZJ_RECON_CONSERVATIVE = 0, ZJ_RECON_ETA = 1 };
+/* ZJ_RECON_XZ: conservative variables with the Xing-Zhang cell-mean
+ * positivity scaling (PositivityPreservingLSQMUSCLJAX,
+ * reconstruction_jax.py:853-984). */
+enum { ZJ_RECON_CONSERVATIVE = 0, ZJ_RECON_ETA = 1, ZJ_RECON_XZ = 2 };
 enum { ZJ_F64 = 0, ZJ_F32 = 1 };
 
 /* Host description of a mesh (topology + vertex coordinates).  All arrays
@@ -108,6 +111,16 @@ int zj_step(zj_solver *s, const double *dt, const void *Q, void *Qout,
  * fluxes: conservative).  The redo is decided on the device (no host
  * synchronisation).  Off by default. */
 int zj_solver_set_mood(zj_solver *s, int32_t on);
+/* A-priori wet/dry guards of the eta reconstruction (HyperbolicSolver
+ * positivity_method / front_theta_tol, solver_jax.py:205-217; the
+ * reconstruction at reconstruction_jax.py:1211-1244): method
+ * ZJ_POSITIVITY_ZHANG_SHU caps the depth deviation of every cell by the
+ * Xing-Zhang-Shu theta (the bed slope kept, the velocity limiters scaled by
+ * theta); front_tol in (0, 1] reconstructs at first order every cell whose
+ * theta is below it (<= 0: off).  Both are ignored by the other
+ * reconstructions, as in the reference. */
+enum { ZJ_POSITIVITY_NONE = 0, ZJ_POSITIVITY_ZHANG_SHU = 1 };
+int zj_solver_set_positivity(zj_solver *s, int32_t method, double front_tol);
 /* {steps redone, cells demoted} since MOOD was enabled (host-synchronous). */
 int zj_solver_mood_stats(zj_solver *s, int32_t *out2);
 /* post_step aux refresh (solver_jax.py:1133-1172): Qaux rows per direction

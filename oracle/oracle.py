@@ -30,7 +30,8 @@ LIB_PATH = HERE / "liboracle.so"
 LIM = {"none": 0, "barth_jespersen": 1, "minmod": 2, "venkatakrishnan": 3,
        "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
-RECON = {"conservative": 0, "eta": 1}
+RECON = {"conservative": 0, "eta": 1, "xz": 2}
+POSITIVITY = {"": 0, "zhang_shu": 1}
 
 
 class _Mesh(ctypes.Structure):
@@ -49,7 +50,8 @@ class _Params(ctypes.Structure):
                 ("eps_wet", ctypes.c_double), ("dt_max", ctypes.c_double),
                 ("manning_n", ctypes.c_double), ("order", ctypes.c_int),
                 ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
-                ("recon", ctypes.c_int)]
+                ("recon", ctypes.c_int), ("positivity", ctypes.c_int),
+                ("front_tol", ctypes.c_double)]
 
 
 _lib = None
@@ -189,9 +191,12 @@ class RefMesh:
 
 
 def params(g=9.81, cfl=0.9, eps_wet=1e-8, dt_max=0.0, manning_n=0.0,
-           order=1, limiter="minmod", riemann="rusanov", recon="conservative"):
+           order=1, limiter="minmod", riemann="rusanov", recon="conservative",
+           positivity="", front_tol=None):
     return _Params(g, cfl, eps_wet, dt_max, manning_n, int(order),
-                   LIM[limiter], RIEMANN[riemann], RECON[recon])
+                   LIM[limiter], RIEMANN[riemann], RECON[recon],
+                   POSITIVITY[positivity],
+                   -1.0 if front_tol is None else float(front_tol))
 
 
 class Oracle:

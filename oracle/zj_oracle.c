@@ -209,6 +209,33 @@ static void demote(int N, double *phi)
             for (int v = 0; v < NV; ++v) phi[(size_t)v * N + c] = 0.0;
 }
 
+/* Xing-Zhang-Shu cell-mean positivity factor (reconstruction_jax.py
+ * :840-850): with the face-midpoint depths h_f = hbar_c + s_c . (x_f - x_c)
+ * over every face of cell c, theta_c = clip(hbar_c / max(hbar_c - min_f h_f,
+ * eps), 0, 1) where min_f h_f < 0, else 1.  s: depth slope (2, N). */
+static void pp_theta(const zjo_mesh *m, const double *hbar, const double *s,
+                     double eps, double *theta)
+{
+    const int N = m->n_cells, F = m->n_faces;
+    double *hmin = malloc(sizeof(double) * N);
+    for (int c = 0; c < N; ++c) hmin[c] = INFINITY;
+    for (int f = 0; f < F; ++f) {
+        double fx = m->face_centers[f], fy = m->face_centers[F + f];
+        for (int side = 0; side < 2; ++side) {
+            int c = m->face_cells[(size_t)side * F + f];
+            if (c < 0) continue;
+            double d = s[c] * (fx - m->cell_centers[c]) +
+                       s[N + c] * (fy - m->cell_centers[N + c]);
+            hmin[c] = dmin(hmin[c], hbar[c] + d);
+        }
+    }
+    for (int c = 0; c < N; ++c) {
+        double den = dmax(hbar[c] - hmin[c], eps);
+        theta[c] = hmin[c] < 0.0 ? clip01(hbar[c] / den) : 1.0;
+    }
+    free(hmin);
+}
+
 static void face_values(const zjo_mesh *m, const double *W,
                         const double *grads, const double *phi, double *WL,
                         double *WR, const double *bW)
@@ -267,8 +294,23 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
 
     double *grads = malloc(sizeof(double) * NV * 2 * N);
     double *phi = malloc(sizeof(double) * NV * N);
-    if (p->recon == ZJO_RECON_CONSERVATIVE) {
+    if (p->recon != ZJO_RECON_ETA) {
         limited_slopes(m, p, Q, bfv, grads, phi);
+        if (p->recon == ZJO_RECON_XZ) {
+            /* PositivityPreservingLSQMUSCLJAX._xing_zhang_scale_phi
+             * (:924-937): theta of the limited depth slope scales the depth
+             * and momentum limiters, never the bed's */
+            double *s = malloc(sizeof(double) * 2 * N);
+            double *th = malloc(sizeof(double) * N);
+            for (int d = 0; d < 2; ++d)
+                for (int c = 0; c < N; ++c)
+                    s[(size_t)d * N + c] = grads[((size_t)1 * 2 + d) * N + c] * phi[N + c];
+            pp_theta(m, Q + N, s, 1e-14, th);
+            for (int c = 0; c < N; ++c)
+                for (int v = 1; v < NV; ++v) phi[(size_t)v * N + c] *= th[c];
+            free(s);
+            free(th);
+        }
         demote(N, phi);
         face_values(m, Q, grads, phi, QL, QR, bfv);
     } else {
@@ -300,6 +342,42 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
             phi[2 * (size_t)N + c] = dmin(phi[2 * (size_t)N + c], pe);
             phi[3 * (size_t)N + c] = dmin(phi[3 * (size_t)N + c], pe);
         }
+        /* front pre-detector and Zhang-Shu cap (:1211-1244), both from the
+         * tied depth slope sh = phi_eta grad eta - phi_b grad b */
+        unsigned char *front = NULL;
+        if (p->front_tol > 0.0 || p->positivity == ZJO_POS_ZHANG_SHU) {
+            double *sh = malloc(sizeof(double) * 2 * N);
+            double *th = malloc(sizeof(double) * N);
+            for (int d = 0; d < 2; ++d)
+                for (int c = 0; c < N; ++c)
+                    sh[(size_t)d * N + c] =
+                        phi[N + c] * grads[((size_t)1 * 2 + d) * N + c] -
+                        phi[c] * grads[(size_t)d * N + c];
+            pp_theta(m, Q + N, sh, 1e-14, th);
+            if (p->front_tol > 0.0) {
+                front = calloc((size_t)N, 1);
+                for (int c = 0; c < N; ++c) front[c] = th[c] < p->front_tol;
+            }
+            if (p->positivity == ZJO_POS_ZHANG_SHU) {
+                /* eta slope := phi_b grad b + theta sh with phi_eta = 1;
+                 * the velocity limiters are scaled by theta */
+                for (int c = 0; c < N; ++c) {
+                    for (int d = 0; d < 2; ++d)
+                        grads[((size_t)1 * 2 + d) * N + c] =
+                            phi[c] * grads[(size_t)d * N + c] + th[c] * sh[(size_t)d * N + c];
+                    phi[N + c] = 1.0;
+                    phi[2 * (size_t)N + c] *= th[c];
+                    phi[3 * (size_t)N + c] *= th[c];
+                }
+            }
+            free(sh);
+            free(th);
+        }
+        if (front)
+            for (int c = 0; c < N; ++c)
+                if (front[c])
+                    for (int v = 0; v < NV; ++v) phi[(size_t)v * N + c] = 0.0;
+        free(front);
         demote(N, phi);
         double *WL = malloc(sizeof(double) * NV * F);
         double *WR = malloc(sizeof(double) * NV * F);

@@ -33,7 +33,8 @@ enum { ZJO_LIM_NONE = 0, ZJO_LIM_BJ = 1, ZJO_LIM_MINMOD = 2, ZJO_LIM_VK = 3,
        ZJO_LIM_VANLEER = 4 };
 enum { ZJO_RIEMANN_RUSANOV_NC = 0, ZJO_RIEMANN_RUSANOV_POS = 1,
        ZJO_RIEMANN_HLL = 2 };
-enum { ZJO_RECON_CONSERVATIVE = 0, ZJO_RECON_ETA = 1 };
+enum { ZJO_RECON_CONSERVATIVE = 0, ZJO_RECON_ETA = 1, ZJO_RECON_XZ = 2 };
+enum { ZJO_POS_NONE = 0, ZJO_POS_ZHANG_SHU = 1 };
 
 typedef struct {
     int dim, n_cells, n_faces, n_bf, max_st;
@@ -57,6 +58,10 @@ typedef struct {
 typedef struct {
     double g, cfl, eps_wet, dt_max, manning_n;
     int order, limiter, riemann, recon;
+    /* eta reconstruction only (solver_jax.py:205-217): Zhang-Shu
+     * deviation cap, and the front pre-detector tolerance (<= 0: off) */
+    int positivity;
+    double front_tol;
 } zjo_params;
 
 double zjo_hinv(double h, double eps);

@@ -297,3 +297,137 @@ def test_mood_is_the_plain_step_when_nothing_is_troubled():
     b, bad, nd = o.step_mood(dt, Q)
     assert nd == 0 and bad == 0
     np.testing.assert_array_equal(a, b)
+
+
+# ── a-priori positivity of the order-2 reconstruction ─────────────────────
+def _cell_faces(mesh, QL, QR):
+    """(cell, face-state column) pairs: the owner's state QL[:, f] and the
+    neighbour's QR[:, f] of every interior face."""
+    from oracle.oracle import RefMesh
+    rm = RefMesh(mesh)
+    A, B = rm.face_cells
+    cells = np.concatenate([A, B[B >= 0]])
+    states = np.concatenate([QL, QR[:, B >= 0]], axis=1)
+    return cells, states
+
+
+def _xz_case():
+    from helpers import wetdry_quads
+    mesh, Q = wetdry_quads(24)
+    # a few steps of the wet/dry run so that the front has a profile
+    o = Oracle(mesh, order=2, limiter="minmod", riemann="hll", recon="eta",
+               cfl=0.45)
+    for _ in range(4):
+        Q, _, _ = o.step_mood(o.dt(Q), Q)
+    # an admissible state: depths >= 0 (a first-order redo can leave -1e-5)
+    dry = Q[1] < 0
+    Q[1:, dry] = 0.0
+    return mesh, Q
+
+
+@pytest.mark.parametrize("limiter", ["none", "venkatakrishnan"])
+def test_xz_scaling_touches_zero_and_scales_momentum(limiter):
+    """PositivityPreservingLSQMUSCLJAX (reconstruction_jax.py:853-984): in
+    every cell whose conservative reconstruction has a negative face depth,
+    the scaled deviation brings the smallest face depth to exactly zero
+    (theta = h / (h - h_min)); depth and momentum deviations are scaled by
+    the same theta, the bed's not at all; other cells are untouched."""
+    mesh, Q = _xz_case()
+    kw = dict(order=2, limiter=limiter, riemann="hll")
+    QLc, QRc, _ = Oracle(mesh, recon="conservative", **kw).reconstruct(Q)
+    QLx, QRx, _ = Oracle(mesh, recon="xz", **kw).reconstruct(Q)
+    cells, Sc = _cell_faces(mesh, QLc, QRc)
+    _, Sx = _cell_faces(mesh, QLx, QRx)
+    N = mesh.n_cells
+    hmin_c = np.full(N, np.inf)
+    hmin_x = np.full(N, np.inf)
+    np.minimum.at(hmin_c, cells, Sc[1])
+    np.minimum.at(hmin_x, cells, Sx[1])
+    neg = hmin_c < 0
+    assert neg.sum() >= 3
+    scale = Q[1].max()
+    assert hmin_x.min() >= -1e-15 * scale
+    assert np.abs(hmin_x[neg]).max() <= 1e-15 * scale
+    np.testing.assert_array_equal(Sx[0], Sc[0])
+    ok = ~neg[cells]
+    np.testing.assert_array_equal(Sx[:, ok], Sc[:, ok])
+    # theta per cell from the depth row, applied to the momentum rows
+    dc = Sc[1] - Q[1][cells]
+    dx = Sx[1] - Q[1][cells]
+    sel = neg[cells] & (np.abs(dc) > 1e-8 * scale)
+    th = dx[sel] / dc[sel]
+    assert np.all((th >= 0) & (th < 1))
+    for r in (2, 3):
+        mc = Sc[r][sel] - Q[r][cells[sel]]
+        mx = Sx[r][sel] - Q[r][cells[sel]]
+        big = np.abs(mc) > 1e-8 * max(np.abs(Q[r]).max(), 1e-300)
+        np.testing.assert_allclose(mx[big], th[big] * mc[big], rtol=1e-9,
+                                   atol=0)
+
+
+@pytest.mark.parametrize("kind", ["quad", "tri"])
+def test_zhang_shu_keeps_lake_at_rest(kind):
+    if kind == "quad":
+        m = Mesh.quad(24, 20).set_bc("wall")
+    else:
+        pts = jittered_points(15, 13, (0.0, 1.0, 0.0, 1.0), 0.3, 5)
+        m = delaunay_mesh(pts, (0.0, 1.0, 0.0, 1.0)).set_bc("wall")
+    o = Oracle(m, order=2, limiter="minmod", riemann="hll", recon="eta",
+               positivity="zhang_shu", front_tol=0.5)
+    Q, _, _ = o.march(_lake(m), n_steps=5)
+    assert np.abs(Q[0] + Q[1] - 1.0).max() < 1e-13
+    assert np.abs(Q[2:]).max() < 1e-13
+
+
+def test_zhang_shu_positivity_without_mood():
+    """The Zhang-Shu cap with CFL 1/6 (the pairing the reference documents,
+    solver_jax.py:221-226) keeps every cell-mean depth of the wet/dry run
+    non-negative up to round-off (dry cells at -1e-27) with no a-posteriori
+    step; mass is conserved.  Without the cap the same run goes negative by
+    orders of magnitude more."""
+    from helpers import wetdry_quads
+    mesh, Q0 = wetdry_quads(32)
+    worst = {}
+    for pos in ("zhang_shu", ""):
+        o = Oracle(mesh, order=2, limiter="minmod", riemann="hll",
+                   recon="eta", positivity=pos, cfl=1.0 / 6.0)
+        Q, m0, worst[pos] = Q0.copy(), o.mass(Q0), 0.0
+        for _ in range(25):
+            Q, _ = o.step(o.dt(Q), Q)
+            worst[pos] = min(worst[pos], Q[1].min())
+        if pos:
+            assert abs(o.mass(Q) - m0) < 1e-12 * m0
+    assert worst["zhang_shu"] >= -1e-20
+    assert worst[""] < -1e-3
+
+
+def test_front_detector_demotes_exactly_the_capped_cells():
+    """front_theta_tol = 1 reconstructs at first order exactly the cells in
+    which the Zhang-Shu cap acts (theta < 1): the two guards share one
+    indicator (reconstruction_jax.py:1211-1244)."""
+    mesh, Q = _xz_case()
+    kw = dict(order=2, limiter="minmod", riemann="hll", recon="eta")
+    plain = Oracle(mesh, **kw).reconstruct(Q)
+    capped = Oracle(mesh, positivity="zhang_shu", **kw).reconstruct(Q)
+    front = Oracle(mesh, front_tol=1.0, **kw).reconstruct(Q)
+    cells, Sp = _cell_faces(mesh, plain[0], plain[1])
+    _, Sz = _cell_faces(mesh, capped[0], capped[1])
+    _, Sf = _cell_faces(mesh, front[0], front[1])
+    N = mesh.n_cells
+    # cells whose faces the cap moves (theta < 1), beyond the rounding of
+    # phi_b grad b + (phi_eta grad eta - phi_b grad b) at theta = 1
+    changed = np.zeros(N, bool)
+    moved = np.abs(Sz - Sp) > 1e-12 * (np.abs(Sp) + 1)
+    changed[cells[np.any(moved, axis=0)]] = True
+    # first order: the same state on every face of the cell (flat eta, b and
+    # velocity; h = eta - b rounds the same way on each face)
+    def flat(S):
+        hi = np.full((4, N), -np.inf)
+        lo = np.full((4, N), np.inf)
+        for r in range(4):
+            np.maximum.at(hi[r], cells, S[r])
+            np.minimum.at(lo[r], cells, S[r])
+        return np.all(hi == lo, axis=0)
+    const, plain_const = flat(Sf), flat(Sp)
+    assert changed.sum() >= 3
+    np.testing.assert_array_equal(const & ~plain_const, changed & ~plain_const)

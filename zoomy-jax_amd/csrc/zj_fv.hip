@@ -51,6 +51,8 @@ struct DevMesh {
 struct Opts {
     double g, cfl, eps, dt_max, manning;
     int order, lim, rs, recon;
+    int pos;           // eta reconstruction: 1 = Zhang-Shu deviation cap
+    double front_tol;  // eta reconstruction: front pre-detector (<= 0: off)
 };
 
 struct Geo {
@@ -676,6 +678,21 @@ __device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v
     return 0.5 * area;
 }
 
+// Xing-Zhang-Shu factor of a cell with mean depth hbar and depth slope
+// (sx, sy): theta = clip(hbar / max(hbar - min_f h_f, 1e-14), 0, 1) when the
+// face-midpoint depth h_f = hbar + s . (x_f - x_c) is negative on some face,
+// else 1 (reconstruction_jax.py:840-850).
+template <typename T>
+__device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const double* fx, const double* fy,
+                                      double2 cc, int mf) {
+    T hmin = T(INFINITY);
+#pragma unroll
+    for (int k = 0; k < MAXF; ++k)
+        if (k < mf) hmin = tmin(hmin, hbar + (sx * T(fx[k] - cc.x) + sy * T(fy[k] - cc.y)));
+    const T den = tmax(hbar - hmin, T(1e-14));
+    return hmin < T(0) ? clip01(hbar / den) : T(1);
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
@@ -935,7 +952,38 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 phi[2] = tmin(phi[2], phi[1]);
                 phi[3] = tmin(phi[3], phi[1]);
             }
-            if (o1_mask && o1_mask[gi]) {  // MOOD: first order for this cell
+            // MOOD (o1_mask) or the front pre-detector: first order here
+            bool first = o1_mask && o1_mask[gi];
+            if (o.recon == RECON_XZ ||
+                (o.recon == RECON_ETA && (o.pos || o.front_tol > 0.0))) {
+                // Xing-Zhang-Shu theta of the limited depth slope
+                // (reconstruction_jax.py:840-850, :924-937, :1211-1244)
+                T sx, sy;
+                if (o.recon == RECON_XZ) {
+                    sx = phi[1] * gx[1];
+                    sy = phi[1] * gy[1];
+                } else {
+                    sx = phi[1] * gx[1] - phi[0] * gx[0];
+                    sy = phi[1] * gy[1] - phi[0] * gy[0];
+                }
+                const T th = pp_theta(q[1], sx, sy, fx, fy, cc, m.mf);
+                if (o.recon == RECON_XZ) {
+                    phi[1] *= th;
+                    phi[2] *= th;
+                    phi[3] *= th;
+                } else {
+                    if (o.front_tol > 0.0 && (double)th < o.front_tol) first = true;
+                    if (o.pos) {
+                        // eta slope := phi_b grad b + theta sh, phi_eta = 1
+                        gx[1] = phi[0] * gx[0] + th * sx;
+                        gy[1] = phi[0] * gy[0] + th * sy;
+                        phi[1] = T(1);
+                        phi[2] *= th;
+                        phi[3] *= th;
+                    }
+                }
+            }
+            if (first) {
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) phi[vv] = T(0);
             }
@@ -1292,14 +1340,14 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
     if (!m || !p || !out) return ZJ_EINVAL;
     if (p->dtype != ZJ_F64 && p->dtype != ZJ_F32) return ZJ_EINVAL;
     if (p->limiter < 0 || p->limiter > 4 || p->riemann < 0 || p->riemann > 2 ||
-        p->recon < 0 || p->recon > 1 || p->order < 1 || p->order > 2)
+        p->recon < 0 || p->recon > 2 || p->order < 1 || p->order > 2)
         return ZJ_EINVAL;
     zj_solver* s = new (std::nothrow) zj_solver();
     if (!s) return ZJ_ENOMEM;
     s->m = m;
     s->p = *p;
     s->o = Opts{p->g, p->cfl, p->eps_wet, p->dt_max, p->manning_n,
-                p->order, p->limiter, p->riemann, p->recon};
+                p->order, p->limiter, p->riemann, p->recon, 0, -1.0};
     const size_t esz = p->dtype == ZJ_F64 ? 8 : 4;
     const int N = m->d.N;
     s->mass_blocks = std::min(1024, nblocks(N));
@@ -1479,6 +1527,15 @@ int zj_solver_set_mood(zj_solver* s, int32_t on) {
         HIPCHK(hipMemset(s->mood_stats, 0, 2 * sizeof(int)));
     }
     s->mood = on ? 1 : 0;
+    return ZJ_OK;
+}
+
+int zj_solver_set_positivity(zj_solver* s, int32_t method, double front_tol) {
+    if (!s || method < ZJ_POSITIVITY_NONE || method > ZJ_POSITIVITY_ZHANG_SHU ||
+        !(front_tol <= 1.0) || front_tol != front_tol)
+        return ZJ_EINVAL;
+    s->o.pos = method;
+    s->o.front_tol = front_tol > 0.0 ? front_tol : -1.0;
     return ZJ_OK;
 }
 

@@ -16,7 +16,8 @@ PRECOND = {"mg": 0, "learned": 1, "mixed": 2}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
-RECON = {"conservative": 0, "eta": 1}
+RECON = {"conservative": 0, "eta": 1, "xz": 2}
+POSITIVITY = {"": 0, "zhang_shu": 1}
 DTYPES = {"float64": 0, "float32": 1}
 
 
@@ -96,6 +97,8 @@ _SIGS = {
                                        ctypes.c_void_p, ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]),
     "zj_solver_set_mood": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "zj_solver_set_positivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.c_double]),
     "zj_solver_mood_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zj_imex_local": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_int32, ctypes.c_void_p,

@@ -189,10 +189,25 @@ class HyperbolicSolver:
     def __init__(self, time_end: float = 1.0, cfl: float = 0.9,
                  dtype: torch.dtype = torch.float64,
                  reconstruction_variables: str = "conservative",
-                 compute_dt: Optional[Callable] = None, mood: bool = False):
+                 compute_dt: Optional[Callable] = None, mood: bool = False,
+                 positivity_method: str = "",
+                 front_theta_tol: Optional[float] = None):
+        """``reconstruction_variables``: ``"conservative"``, ``"eta"``
+        (well-balanced wet/dry recipe) or ``"xz"`` (conservative with the
+        Xing-Zhang cell-mean positivity scaling).  ``positivity_method``
+        (``""`` or ``"zhang_shu"``) and ``front_theta_tol`` (None or a value
+        in (0, 1]) act on the eta reconstruction only, as in the reference
+        (``solver_jax.py:196-226``)."""
         if reconstruction_variables not in L.RECON:
             raise ValueError(
                 f"reconstruction_variables must be one of {sorted(L.RECON)}")
+        if positivity_method not in L.POSITIVITY:
+            raise ValueError(
+                f"positivity_method must be one of {sorted(L.POSITIVITY)}")
+        if front_theta_tol is not None and not 0.0 < front_theta_tol <= 1.0:
+            raise ValueError("front_theta_tol must be None or in (0, 1]")
+        self.positivity_method = positivity_method
+        self.front_theta_tol = front_theta_tol
         self.time_end = float(time_end)
         self.cfl = float(cfl)
         self.dtype = dtype
@@ -235,6 +250,11 @@ class HyperbolicSolver:
         if self.mood:
             L.check(L.lib().zj_solver_set_mood(self._rt.handle, 1),
                     "zj_solver_set_mood")
+        if self.positivity_method or self.front_theta_tol is not None:
+            L.check(L.lib().zj_solver_set_positivity(
+                self._rt.handle, L.POSITIVITY[self.positivity_method],
+                -1.0 if self.front_theta_tol is None
+                else float(self.front_theta_tol)), "zj_solver_set_positivity")
         dev = _device()
         self.device = dev
         self._dt = torch.zeros((), dtype=torch.float64, device=dev)
