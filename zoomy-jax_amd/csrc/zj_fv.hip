@@ -1,18 +1,20 @@
 // MI355X finite-volume time step: kernels and the C ABI (include/zoomy_hip.h).
 //
-// One HIP thread owns one cell and gathers every face of that cell from the
-// per-cell slot tables (cell -> vertices, cell -> neighbour), so the update is
-// atomic-free and bit-reproducible: the face-scatter of the reference
-// (solver_jax.py:806-809, solver2/blocks.py:241-287) is turned into the
-// gather over a face-of-cell table that blocks.py:261-263 names.  Face
-// geometry (normal, length, centre) is recomputed from the vertex coordinates
-// instead of streamed from HBM.  An SSP-RK2 step is four launches:
-//
-//   slopes(Q0)      LSQ gradient + limiter bounds + limiter  -> S
-//   flux(Q0, S)     reconstruct, Riemann, cell-edge term      -> Q1
-//   slopes(Q1)                                                -> S
-//   flux(Q1, S, Q0) ... + (Q0 + Q2)/2 + source + troubled flag + dt of the
-//                   new state (block min -> one atomicMin)    -> Q
+// The cells are cut into patches of up to 256 consecutive ids (16 x 16
+// tiles of a Morton-ordered quad mesh); one workgroup stages a patch with its
+// face ring and stencil ring in LDS and runs a whole SSP-RK2 stage on it
+// (k_stage): LSQ gradients, limiter bounds and limiters of the own and
+// face-ring cells, then reconstruction, Riemann flux and cell-edge term of
+// every face of every own cell, gathered per cell so that the update is
+// atomic-free and bit-reproducible (the face-scatter of the reference,
+// solver_jax.py:806-809 and solver2/blocks.py:241-287, becomes a gather over
+// the face-of-cell table that blocks.py:261-263 names).  Face geometry is
+// recomputed from the staged vertices instead of streamed from HBM.  An
+// SSP-RK2 step is two launches (stage 2 adds the RK average, the explicit
+// source, the troubled flag and the per-wave dt minima) and one small
+// reduction.  Separate kernels: the post_step aux refresh (k_aux), the dt
+// law (k_dt), the compensated mass (k_mass_*) and the local implicit source
+// (k_imex).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -138,111 +140,66 @@ __device__ __forceinline__ void recon_face(int recon, const T* q, T hi, const T*
 }
 
 // ─────────────────────────────────────────────────────────────────────────
-// slopes: LSQ gradient with per-direction boundary ghosts, limiter bounds
-// over face neighbours and ghosts, limiter; writes S (N, 8) = phi * grad.
-// aux mode (Qaux != nullptr): conservative gradients with zero ghost deltas
-// -> derivative aux rows + hinv (post_step).
+// aux: post_step derivative rows (LSQ gradient of the conservative state
+// with zero ghost deltas; the ghost rows stay in the normal matrices) and
+// hinv.  Rows per direction d: [d q_d/dx_d, d h/dx_d], then d b/dx_d, then
+// hinv (solver_jax.py:284-337).
 // ─────────────────────────────────────────────────────────────────────────
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
-                                                  const T* __restrict__ Q,
-                                                  T* __restrict__ S,
-                                                  T* __restrict__ Qaux) {
+__global__ __launch_bounds__(BLOCK) void k_aux(DevMesh m, Opts o, const T* __restrict__ Q,
+                                               T* __restrict__ Qaux) {
     const int c = xcd_block() * BLOCK + threadIdx.x;
     if (c >= m.N) return;
     const int N = m.N;
-    // aux only (S == nullptr), slopes only (Qaux == nullptr), or both from
-    // one gather (conservative reconstruction: the aux rows are the same
-    // LSQ fit with zero ghost deltas)
-    const bool aux = Qaux != nullptr && S == nullptr;
-    const bool both = Qaux != nullptr && S != nullptr;
-    const int recon = aux ? RECON_CONS : o.recon;
     const T eps = T(o.eps);
     T q[4];
     load_q(Q, N, c, q);
     const T hic = hinv_of(q[1], eps);
-    T w[4];
-    to_w(recon, q, hic, w);
     Geo G;
     load_geo(m, c, G);
-
     double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
     double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
     T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
-    T gbx[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    T gby[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    T umax[4], umin[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) { umax[v] = w[v]; umin[v] = w[v]; }
-
     ZJ_FOR_FACES(k) {
         const int nb = m.nbr[k * N + c];
         if (nb >= 0) {
-            T qn[4], wn[4];
+            T qn[4];
             load_q(Q, N, nb, qn);
-            to_w(recon, qn, hinv_of(qn[1], eps), wn);
             const double rx = m.cen[nb] - G.cx, ry = m.cen[N + nb] - G.cy;
             Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                T dv = wn[v] - w[v];
+                T dv = qn[v] - q[v];
                 bx[v] += T(rx) * dv;
                 by[v] += T(ry) * dv;
-                umax[v] = tmax(umax[v], wn[v]);
-                umin[v] = tmin(umin[v], wn[v]);
             }
         } else {
-            const int bf = -1 - nb;
-            const int kind = m.bf_kind[bf];
-            T qg[4], wg[4];
-            if (kind == BC_PERIODIC) load_q(Q, N, m.bf_pcell[bf], qg);
-            else bc_ghost(kind, q, T(G.nx[k]), T(G.ny[k]), qg);
-            to_w(recon, qg, hinv_of(qg[1], eps), wg);
             const double rx = 2.0 * (G.fx[k] - G.cx), ry = 2.0 * (G.fy[k] - G.cy);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                T bv = (kind == BC_PERIODIC) ? wg[v] : T(2) * wg[v] - w[v];
-                umax[v] = tmax(umax[v], bv);
-                umin[v] = tmin(umin[v], bv);
-            }
             const double an[2] = {fabs(G.nx[k]), fabs(G.ny[k])};
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                if (an[d] > 0.5) {
-                    Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry;
-                    if (!aux) {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            T dv = wg[v] - w[v];
-                            gbx[d][v] += T(rx) * dv;
-                            gby[d][v] += T(ry) * dv;
-                        }
-                    }
-                }
-            }
+            for (int d = 0; d < 2; ++d)
+                if (an[d] > 0.5) { Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry; }
         }
     }
     for (int j = 0; j < m.mx; ++j) {
         const int e = m.extra[j * N + c];
         if (e < 0) break;
-        T qn[4], wn[4];
+        T qn[4];
         load_q(Q, N, e, qn);
-        to_w(recon, qn, hinv_of(qn[1], eps), wn);
         const double rx = m.cen[e] - G.cx, ry = m.cen[N + e] - G.cy;
         Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            T dv = wn[v] - w[v];
+            T dv = qn[v] - q[v];
             bx[v] += T(rx) * dv;
             by[v] += T(ry) * dv;
         }
     }
-
-    T gx[4], gy[4];
+    T ax[4], ay[4];
     if (m.dim == 1) {
         const double inv = 1.0 / (Sxx + Gxx[0]);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) { gx[v] = T(inv) * (bx[v] + gbx[0][v]); gy[v] = T(0); }
+        for (int v = 0; v < 4; ++v) { ax[v] = T(inv) * bx[v]; ay[v] = T(0); }
     } else {
         // component x: normal matrix with the x-ghost rows only (and y alike)
         const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
@@ -253,200 +210,17 @@ __global__ __launch_bounds__(BLOCK) void k_slopes(DevMesh m, Opts o,
         const T c2 = T(-bxy * dinv1), c3 = T(bxx * dinv1);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            gx[v] = c0 * (bx[v] + gbx[0][v]) + c1 * (by[v] + gby[0][v]);
-            gy[v] = c2 * (bx[v] + gbx[1][v]) + c3 * (by[v] + gby[1][v]);
+            ax[v] = c0 * bx[v] + c1 * by[v];
+            ay[v] = c2 * bx[v] + c3 * by[v];
         }
     }
-
-    if (aux || both) {
-        // rows: per direction d [d q_d/dx_d, d h/dx_d], then d b/dx_d, hinv
-        T ax[4], ay[4];
-        if (both) {  // same normal matrices, inner deltas only
-            if (m.dim == 1) {
-                const T inv = T(1.0 / (Sxx + Gxx[0]));
-#pragma unroll
-                for (int v = 0; v < 4; ++v) { ax[v] = inv * bx[v]; ay[v] = T(0); }
-            } else {
-                const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
-                const double di0 = 1.0 / (axx * ayy - axy * axy);
-                const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
-                const double di1 = 1.0 / (bxx * byy - bxy * bxy);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    ax[v] = T(ayy * di0) * bx[v] + T(-axy * di0) * by[v];
-                    ay[v] = T(-bxy * di1) * bx[v] + T(bxx * di1) * by[v];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) { ax[v] = gx[v]; ay[v] = gy[v]; }
-        }
-        int row = 0;
-        for (int d = 0; d < m.dim; ++d) {
-            Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[2] : ay[3];
-            Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[1] : ay[1];
-        }
-        for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[0] : ay[0];
-        Qaux[(size_t)row * N + c] = hic;
-        if (aux) return;
+    int row = 0;
+    for (int d = 0; d < m.dim; ++d) {
+        Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[2] : ay[3];
+        Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[1] : ay[1];
     }
-
-    T phi[4] = {1, 1, 1, 1};
-    if (o.lim != LIM_NONE) {
-        const T ev2 = T(m.dim == 1 ? G.vol * G.vol : fabs(G.vol));
-        ZJ_FOR_FACES(k) {
-            const T rx = T(G.fx[k] - G.cx), ry = T(G.fy[k] - G.cy);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                T d = gx[v] * rx + gy[v] * ry;
-                T cand = lim_cand(o.lim, d, umax[v] - w[v], umin[v] - w[v], ev2);
-                phi[v] = tmin(phi[v], cand);
-            }
-        }
-    }
-    if (recon == RECON_ETA) {
-        phi[0] = tmin(phi[0], phi[1]);
-        phi[2] = tmin(phi[2], phi[1]);
-        phi[3] = tmin(phi[3], phi[1]);
-    }
-    T* sp = S + (size_t)c * 8;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        T p = clip01(phi[v]);
-        sp[2 * v] = p * gx[v];
-        sp[2 * v + 1] = p * gy[v];
-    }
-}
-
-// ─────────────────────────────────────────────────────────────────────────
-// flux + update.  mode 0: Qout = Qin + dt L;  mode 1: Qout = (Q0 + Qin +
-// dt L)/2;  mode 2: Qout = L (flux operator).  final: explicit source with
-// the step-start aux, troubled count, dt reduction of the new state.
-// ─────────────────────────────────────────────────────────────────────────
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_flux(
-    DevMesh m, Opts o, const T* __restrict__ Qin, const T* __restrict__ S,
-    const T* Q0, T* Qout, const double* __restrict__ dtp,
-    int mode, int final_stage, int write_b, int* __restrict__ troubled,
-    unsigned long long* __restrict__ dt_red) {
-    __shared__ double lds_d[BLOCK / 64];
-    __shared__ int lds_i[BLOCK / 64];
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    const bool live = c < m.N;
-    const int N = m.N;
-    const T g = T(o.g), eps = T(o.eps);
-    int bad = 0;
-    double loc_dt = INFINITY;
-    if (live) {
-        T q[4];
-        load_q(Qin, N, c, q);
-        const T hic = hinv_of(q[1], eps);
-        T sc[8];
-        if (S) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) sc[i] = S[(size_t)c * 8 + i];
-        }
-        Geo G;
-        load_geo(m, c, G);
-        T acc[4] = {0, 0, 0, 0};
-        ZJ_FOR_FACES(k) {
-            const T nx = T(G.nx[k]), ny = T(G.ny[k]);
-            T qL[4], qR[4];
-            if (S) recon_face(o.recon, q, hic, sc, T(G.fx[k] - G.cx), T(G.fy[k] - G.cy), qL);
-            else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
-            const int nb = m.nbr[k * N + c];
-            T hiR = hic;
-            if (nb >= 0) {
-                T qn[4];
-                load_q(Qin, N, nb, qn);
-                hiR = hinv_of(qn[1], eps);
-                if (S) {
-                    T sn[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) sn[i] = S[(size_t)nb * 8 + i];
-                    recon_face(o.recon, qn, hiR, sn, T(G.fx[k] - m.cen[nb]),
-                               T(G.fy[k] - m.cen[N + nb]), qR);
-                } else {
-                    qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
-                }
-            } else {
-                const int bf = -1 - nb;
-                const int kind = m.bf_kind[bf];
-                if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qR);
-                else bc_ghost(kind, qL, nx, ny, qR);
-            }
-            T f[4];
-            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, f);
-            const T L = T(G.len[k]);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] += f[v] * L;
-            if (S) {
-                T ce[4];
-                cell_edge(q, qL, nx, ny, g, ce);
-                acc[2] += ce[2] * L;
-                acc[3] += ce[3] * L;
-            }
-        }
-        const T ivol = T(1.0 / G.vol);
-        T out[4];
-        if (mode == 2) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = -acc[v] * ivol;
-        } else {
-            const T dt = T(*dtp);
-            if (mode == 1) {
-                T q0[4];
-                load_q(Q0, N, c, q0);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) out[v] = T(0.5) * (q0[v] + (q[v] - dt * (acc[v] * ivol)));
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) out[v] = q[v] - dt * (acc[v] * ivol);
-            }
-            if (final_stage && o.manning > 0.0) {
-                // explicit source with the aux of the step-start state (ode.RK1
-                // after the hyperbolic step, solver_jax.py:1129)
-                const T h0 = mode == 1 ? Q0[N + c] : q[1];
-                const T hs = hinv_of(h0, eps);
-                const T u = out[2] * hs, vv = out[3] * hs;
-                const T sp = sqrt(u * u + vv * vv);
-                const T kk = T(o.g * o.manning * o.manning) * cbrt(hs);
-                out[2] += dt * (-kk * u * sp);
-                out[3] += dt * (-kk * vv * sp);
-            }
-        }
-        if (write_b) Qout[c] = out[0];
-        Qout[N + c] = out[1];
-        Qout[2 * N + c] = out[2];
-        Qout[3 * N + c] = out[3];
-        if (final_stage && mode != 2) {
-            bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
-                  !isfinite(out[2]) || !isfinite(out[3]);
-            if (dt_red) {
-                const T hn = hinv_of(out[1], eps);
-                const double rc = m.rin[c];
-                ZJ_FOR_FACES(k) {
-                    const int nb = m.nbr[k * N + c];
-                    const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
-                    const double a = (double)max_abs_eig(out, hn, T(G.nx[k]), T(G.ny[k]), g);
-                    double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
-                    if (l != l) l = 0.0;  // NaN speed: force the fatal dt guard
-                    loc_dt = fmin(loc_dt, l);
-                }
-            }
-        }
-    }
-    if (final_stage && mode != 2) {
-        if (troubled) {
-            int s = block_reduce(bad, [](int a, int b) { return a + b; }, lds_i);
-            if (threadIdx.x == 0 && s) atomicAdd(troubled, s);
-        }
-        if (dt_red) {
-            double s = block_reduce(loc_dt, [](double a, double b) { return fmin(a, b); }, lds_d);
-            if (threadIdx.x == 0)
-                atomicMin(dt_red, (unsigned long long)__double_as_longlong(s));
-        }
-    }
+    for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[0] : ay[0];
+    Qaux[(size_t)row * N + c] = hic;
 }
 
 // dt of a state (compute_timestep): per-cell min over faces, block min,
@@ -1169,7 +943,6 @@ struct zj_solver {
     zj_mesh* m;
     zj_params p;
     Opts o;
-    void* S;                    // (N, 8) limited slopes
     void* Q1;                   // (4, N) stage state
     unsigned long long* dt_red; // reduction scalar
     double* dt_part;            // per-wave dt minima of a stage launch
@@ -1351,8 +1124,7 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
     const size_t esz = p->dtype == ZJ_F64 ? 8 : 4;
     const int N = m->d.N;
     s->mass_blocks = std::min(1024, nblocks(N));
-    if (hipMalloc(&s->S, esz * 8 * N) != hipSuccess ||
-        hipMalloc(&s->Q1, esz * 4 * N) != hipSuccess ||
+    if (hipMalloc(&s->Q1, esz * 4 * N) != hipSuccess ||
         hipMalloc((void**)&s->dt_red, 8) != hipSuccess ||
         hipMalloc((void**)&s->dt_part, sizeof(double) * (size_t)m->pt.npatch * (SBLOCK / 64)) !=
             hipSuccess ||
@@ -1368,7 +1140,6 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
 
 int zj_solver_destroy(zj_solver* s) {
     if (!s) return ZJ_OK;
-    (void)hipFree(s->S);
     (void)hipFree(s->Q1);
     (void)hipFree(s->dt_red);
     (void)hipFree(s->dt_part);
@@ -1555,11 +1326,11 @@ int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const DevMesh& m = s->m->d;
     if (s->p.dtype == ZJ_F64)
-        hipLaunchKernelGGL(k_slopes<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const double*)Q, (double*)nullptr, (double*)Qaux);
+        hipLaunchKernelGGL(k_aux<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const double*)Q, (double*)Qaux);
     else
-        hipLaunchKernelGGL(k_slopes<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const float*)Q, (float*)nullptr, (float*)Qaux);
+        hipLaunchKernelGGL(k_aux<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
+                           (const float*)Q, (float*)Qaux);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1643,8 +1414,7 @@ static int time_kernels_impl(zj_solver* s, const T* q, const double* dt, int rep
                 else if (which == 2)
                     hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
                 else if (which == 3)
-                    hipLaunchKernelGGL(k_slopes<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q,
-                                       (T*)nullptr, aux);
+                    hipLaunchKernelGGL(k_aux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, aux);
                 else
                     launch_stage<T>(s, q, (const T*)nullptr, out, dt, aux, 0, 0, write_b, slopes,
                                     (int*)nullptr, (unsigned long long*)nullptr, st);
