@@ -442,7 +442,11 @@ static void phys_flux(const zjo_params *p, const double *q, double hinv,
  * (jax_runtime.py:780-806; update rule solver_jax.py:806-809):
  *   Rusanov: F = centred flux; Dm/Dp = (B_int dQ -/+ s_M I' dQ) / 2 with the
  *   path-conservative integral of B[q_d, b] = g h n_d over the straight path
- *   (nonconservative_flux.py:61-186).  The NC variant dissipates every row
+ *   (nonconservative_flux.py:61-186); s_M takes 1/h of each face state
+ *   itself (the KP formula), not the cell aux that the centred flux uses --
+ *   at a periodic seam the ghost is the partner cell while the aux passed is
+ *   the inner cell's (solver_jax.py:858-861), and tests/refs/bc_periodic
+ *   holds to round-off only with the state's own 1/h.  The NC variant dissipates every row
  *   (I' = I); the positive/well-balanced variant dissipates h by the jump of
  *   eta = h + b and leaves b untouched (identified from tests/refs/wb_lake).
  *   HLL: hydrostatic reconstruction (Audusse et al. 2004) of both face states,
@@ -496,7 +500,8 @@ void zjo_face_flux(const zjo_params *p, const double *qL, const double *qR,
     double FLx[NV], FRx[NV];
     phys_flux(p, qL, hinvA, n, FLx);
     phys_flux(p, qR, hinvB, n, FRx);
-    double s = dmax(max_abs_eig(p, qL, hinvA, n), max_abs_eig(p, qR, hinvB, n));
+    double s = dmax(max_abs_eig(p, qL, zjo_hinv(qL[IH], p->eps_wet), n),
+                    max_abs_eig(p, qR, zjo_hinv(qR[IH], p->eps_wet), n));
     double dq[NV];
     for (int v = 0; v < NV; ++v) {
         F[v] = 0.5 * (FLx[v] + FRx[v]);

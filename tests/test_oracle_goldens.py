@@ -22,6 +22,9 @@ from zoomy_hip.mesh import Mesh
     ("wb_lake", 100, 10.0, 1.0, "wall", lake_ic, "rusanov"),
     # test_bc_runtime.py: wall kind
     ("bc_wall", 40, 1.0, 0.2, "wall", tilted_ic, "rusanov_nc"),
+    # test_bc_runtime.py: periodic kind (the seam pairs the partner cell's
+    # state with the inner cell's aux; s_M from each face state's own 1/h)
+    ("bc_periodic", 40, 1.0, 0.2, "periodic", tilted_ic, "rusanov_nc"),
 ])
 def test_1d_goldens(golden, name, n, L, t_end, bc, ic, riemann):
     m = Mesh.line(n, 0.0, L).set_bc(bc)
@@ -31,8 +34,8 @@ def test_1d_goldens(golden, name, n, L, t_end, bc, ic, riemann):
     assert abs(t - t_end) < 1e-12
     assert np.abs(Q[:3] - ref["Q"]).max() < 1e-15
     A = o.aux(Q)
-    if name == "bc_wall":
-        # wall-boundary derivative aux rows are not reproduced (the boundary
+    if name in ("bc_wall", "bc_periodic"):
+        # boundary-cell derivative aux rows are not reproduced (the boundary
         # row weights of zoomy_core's LSQMesh are unknown); interior exact
         assert np.abs(A[:, 1:-1] - ref["Qaux"][:, 1:-1]).max() < 1e-12
         assert np.abs(A[3] - ref["Qaux"][3]).max() < 1e-12

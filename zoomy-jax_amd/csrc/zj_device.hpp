@@ -95,10 +95,13 @@ __device__ __forceinline__ T max_abs_eig(const T* q, T hinv, T nx, T ny, T g) {
 }
 
 // F + Dm for the evaluating cell (left state qL, neighbour qR, outward n).
+// hiL/hiR are the cell aux 1/h used by the fluxes; the Rusanov speed s_M
+// takes 1/h of each face state itself (what the reference's periodic-seam
+// golden pins: there the ghost is the partner cell, the aux the inner one's).
 template <typename T>
 __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
                                                T hiL, T hiR, T nx, T ny, T g,
-                                               T* out) {
+                                               T eps, T* out) {
     if (rs == RS_HLL) {
         T bs = tmax(qL[0], qR[0]);
         T hLs = tmax(T(0), qL[1] + qL[0] - bs);
@@ -135,8 +138,8 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
     T unL = (qL[2] * nx + qL[3] * ny) * hiL;
     T unR = (qR[2] * nx + qR[3] * ny) * hiR;
     T pL = T(0.5) * g * qL[1] * qL[1], pR = T(0.5) * g * qR[1] * qR[1];
-    T s = tmax(fabs(unL) + sqrt(g * tmax(qL[1], T(0))),
-               fabs(unR) + sqrt(g * tmax(qR[1], T(0))));
+    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + sqrt(g * tmax(qL[1], T(0))),
+               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + sqrt(g * tmax(qR[1], T(0))));
     T d0 = qR[0] - qL[0], d1 = qR[1] - qL[1], d2 = qR[2] - qL[2], d3 = qR[3] - qL[3];
     T Bd = g * (T(0.5) * (qL[1] + qR[1])) * d0;
     T diss0, diss1;

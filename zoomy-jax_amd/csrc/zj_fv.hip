@@ -841,7 +841,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 else bc_ghost(kind, qL, nx, ny, qR);
             }
             T f[4];
-            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, f);
+            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, T(o.eps), f);
             const T L = T(len);
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) acc[vv] += f[vv] * L;
