@@ -30,7 +30,8 @@ extern "C" {
 #define ZJ_ESOLVE 5     /* pressure solve missed its accept gate       */
 
 /* boundary kinds, limiters, Riemann solvers, reconstruction variables */
-enum { ZJ_BC_EXTRAPOLATION = 0, ZJ_BC_WALL = 1, ZJ_BC_PERIODIC = 2 };
+enum { ZJ_BC_EXTRAPOLATION = 0, ZJ_BC_WALL = 1, ZJ_BC_PERIODIC = 2,
+       ZJ_BC_DIRICHLET = 3 };
 enum { ZJ_LIM_NONE = 0, ZJ_LIM_BARTH_JESPERSEN = 1, ZJ_LIM_MINMOD = 2,
        ZJ_LIM_VENKATAKRISHNAN = 3, ZJ_LIM_VANLEER = 4 };
 enum { ZJ_RIEMANN_RUSANOV_NC = 0, ZJ_RIEMANN_RUSANOV = 1, ZJ_RIEMANN_HLL = 2 };
@@ -55,6 +56,11 @@ typedef struct {
     const int32_t *cell_vertices, *cell_nbr, *cell_extra;
     const double *verts, *cell_center, *cell_inradius;
     const int32_t *bface_cell, *bface_slot, *bface_kind, *bface_partner;
+    /* ZJ_BC_DIRICHLET faces (may be NULL when there are none): bit v of
+     * bface_mask[b] prescribes state row v (b, h, hu, hv) of face b at
+     * bface_value[v * n_bf + b]; the other rows extrapolate */
+    const int32_t *bface_mask;
+    const double *bface_value;
 } zj_mesh_desc;
 
 typedef struct zj_mesh zj_mesh;       /* device-resident mesh   */

@@ -30,8 +30,9 @@ LIB_PATH = HERE / "liboracle.so"
 LIM = {"none": 0, "barth_jespersen": 1, "minmod": 2, "venkatakrishnan": 3,
        "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
-RECON = {"conservative": 0, "eta": 1, "xz": 2}
+RECON = {"conservative": 0, "eta": 1, "xz": 2, "eta_hu": 3}
 POSITIVITY = {"": 0, "zhang_shu": 1}
+STAGE_AUX = {"refresh": 0, "frozen": 1}
 
 
 class _Mesh(ctypes.Structure):
@@ -42,7 +43,7 @@ class _Mesh(ctypes.Structure):
             "face_cells", "face_normals", "face_volumes", "face_centers",
             "cell_centers", "cell_volumes", "cell_inradius", "bface_face",
             "bface_cell", "bface_kind", "bface_partner", "stencil", "lsq_w",
-            "gslot", "lsq_wg")]
+            "gslot", "lsq_wg", "bface_mask", "bface_value")]
 
 
 class _Params(ctypes.Structure):
@@ -51,7 +52,7 @@ class _Params(ctypes.Structure):
                 ("manning_n", ctypes.c_double), ("order", ctypes.c_int),
                 ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("recon", ctypes.c_int), ("positivity", ctypes.c_int),
-                ("front_tol", ctypes.c_double)]
+                ("front_tol", ctypes.c_double), ("stage_aux", ctypes.c_int)]
 
 
 _lib = None
@@ -178,6 +179,9 @@ class RefMesh:
         self.lsq_w = np.ascontiguousarray(w)
         self.gslot = np.ascontiguousarray(gslot, np.int32)
         self.lsq_wg = np.ascontiguousarray(wg)
+        dm, dv = mesh.bface_dirichlet()
+        self.bface_mask = np.ascontiguousarray(dm, np.int32)
+        self.bface_value = np.ascontiguousarray(dv)
         self.n_faces = F
         self.c = _Mesh(mesh.dim, N, F, mesh.n_bfaces, st.shape[1],
                        *[a.ctypes.data for a in (
@@ -187,16 +191,18 @@ class RefMesh:
                            self.cell_inradius, self.bface_face,
                            self.bface_cell, self.bface_kind,
                            self.bface_partner, self.stencil, self.lsq_w,
-                           self.gslot, self.lsq_wg)])
+                           self.gslot, self.lsq_wg, self.bface_mask,
+                           self.bface_value)])
 
 
 def params(g=9.81, cfl=0.9, eps_wet=1e-8, dt_max=0.0, manning_n=0.0,
            order=1, limiter="minmod", riemann="rusanov", recon="conservative",
-           positivity="", front_tol=None):
+           positivity="", front_tol=None, stage_aux="refresh"):
     return _Params(g, cfl, eps_wet, dt_max, manning_n, int(order),
                    LIM[limiter], RIEMANN[riemann], RECON[recon],
                    POSITIVITY[positivity],
-                   -1.0 if front_tol is None else float(front_tol))
+                   -1.0 if front_tol is None else float(front_tol),
+                   STAGE_AUX[stage_aux])
 
 
 class Oracle:

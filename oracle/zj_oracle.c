@@ -45,6 +45,23 @@ void zjo_bc_ghost(int kind, const double *q, const double *n, double *out)
     }
 }
 
+#ifndef DLIFT
+#define DLIFT 0
+#endif
+/* Ghost state of boundary face b from the state q at the face: the kind's
+ * ghost, then the prescribed rows of a Dirichlet face. */
+static void ghost_at(const zjo_mesh *m, int b, const double *q,
+                     const double *n, double *out)
+{
+    zjo_bc_ghost(m->bface_kind[b], q, n, out);
+    if (m->bface_kind[b] != ZJO_BC_DIRICHLET || !m->bface_mask) return;
+    for (int v = 0; v < NV; ++v)
+        if (m->bface_mask[b] >> v & 1) {
+            double val = m->bface_value[(size_t)v * m->n_bf + b];
+            out[v] = DLIFT ? 2.0 * val - q[v] : val;
+        }
+}
+
 /* LSQ gradient (zoomy_jax/mesh/mesh.py:74-120, reconstruction_jax.py:
  * 433-522): grad = sum_j w_j (u_j - u_c) + sum_g w_g (u_bf,g - u_c).
  * ubf == NULL is the extrapolation convention (ghost delta zero). */
@@ -136,7 +153,7 @@ static void boundary_values(const zjo_mesh *m, const double *Q, double *bfv,
         if (kind == ZJO_BC_PERIODIC) {
             for (int v = 0; v < NV; ++v) o[v] = q[v];
         } else {
-            zjo_bc_ghost(kind, q, n, o);
+            ghost_at(m, b, q, n, o);
         }
         for (int v = 0; v < NV; ++v) bfv[(size_t)v * nbf + b] = o[v];
         if (bf_bound_cell) bf_bound_cell[b] = cc;
@@ -294,7 +311,7 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
 
     double *grads = malloc(sizeof(double) * NV * 2 * N);
     double *phi = malloc(sizeof(double) * NV * N);
-    if (p->recon != ZJO_RECON_ETA) {
+    if (p->recon != ZJO_RECON_ETA && p->recon != ZJO_RECON_ETA_HU) {
         limited_slopes(m, p, Q, bfv, grads, phi);
         if (p->recon == ZJO_RECON_XZ) {
             /* PositivityPreservingLSQMUSCLJAX._xing_zhang_scale_phi
@@ -315,7 +332,13 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
         face_values(m, Q, grads, phi, QL, QR, bfv);
     } else {
         /* W = (b, eta = h + b, u = hinv q, v = hinv q_y); ghost W through the
-         * same map with the ghost's own hinv. */
+         * same map with the ghost's own hinv.  ETA_HU keeps the conservative
+         * momentum rows (b, eta, hu, hv) with the 1e-3 depth threshold of the
+         * earlier EtaWellBalancedLSQMUSCLJAX (its replacement's docstring,
+         * reconstruction_jax.py:995-1008): cells below it are demoted to
+         * first order and face momentum below it is zeroed. */
+        const int vel = p->recon == ZJO_RECON_ETA;
+        const double eps_hu = 1e-3;
         double *W = malloc(sizeof(double) * NV * N);
         double *bW = malloc(sizeof(double) * NV * (nbf > 0 ? nbf : 1));
         for (int c = 0; c < N; ++c) {
@@ -323,16 +346,16 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
             double hi = zjo_hinv(h, p->eps_wet);
             W[c] = b;
             W[N + c] = h + b;
-            W[2 * (size_t)N + c] = Q[2 * (size_t)N + c] * hi;
-            W[3 * (size_t)N + c] = Q[3 * (size_t)N + c] * hi;
+            W[2 * (size_t)N + c] = Q[2 * (size_t)N + c] * (vel ? hi : 1.0);
+            W[3 * (size_t)N + c] = Q[3 * (size_t)N + c] * (vel ? hi : 1.0);
         }
         for (int k = 0; k < nbf; ++k) {
             double b = bfv[k], h = bfv[nbf + k];
             double hi = zjo_hinv(h, p->eps_wet);
             bW[k] = b;
             bW[nbf + k] = h + b;
-            bW[2 * (size_t)nbf + k] = bfv[2 * (size_t)nbf + k] * hi;
-            bW[3 * (size_t)nbf + k] = bfv[3 * (size_t)nbf + k] * hi;
+            bW[2 * (size_t)nbf + k] = bfv[2 * (size_t)nbf + k] * (vel ? hi : 1.0);
+            bW[3 * (size_t)nbf + k] = bfv[3 * (size_t)nbf + k] * (vel ? hi : 1.0);
         }
         limited_slopes(m, p, W, bW, grads, phi);
         /* tie phi_b and the velocity limiters to phi_eta (:1206-1209) */
@@ -378,6 +401,10 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
                 if (front[c])
                     for (int v = 0; v < NV; ++v) phi[(size_t)v * N + c] = 0.0;
         free(front);
+        if (!vel)
+            for (int c = 0; c < N; ++c)
+                if (Q[N + c] < eps_hu)
+                    for (int v = 0; v < NV; ++v) phi[(size_t)v * N + c] = 0.0;
         demote(N, phi);
         double *WL = malloc(sizeof(double) * NV * F);
         double *WR = malloc(sizeof(double) * NV * F);
@@ -395,8 +422,10 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
                 double h = eta - b;
                 Qx[f] = b;
                 Qx[(size_t)F + f] = h;
-                Qx[2 * (size_t)F + f] = Wx[2 * (size_t)F + f] * h;
-                Qx[3 * (size_t)F + f] = Wx[3 * (size_t)F + f] * h;
+                for (int v = 2; v < NV; ++v) {
+                    double w = Wx[(size_t)v * F + f];
+                    Qx[(size_t)v * F + f] = vel ? w * h : (h < eps_hu ? 0.0 : w);
+                }
             }
         }
         for (int b = 0; b < nbf; ++b)
@@ -536,15 +565,26 @@ static void cell_edge_source(const zjo_params *p, const double *qc,
     out[IVv] = p->g * hb * db * n[1];
 }
 
+/* The flux operator with the cell aux 1/h taken from the state H (Q itself,
+ * or the step-start state for a frozen second RK stage). */
+static void flux_operator_h(const zjo_mesh *m, const zjo_params *p,
+                            const double *Q, const double *H, double *dQ);
+
 void zjo_flux_operator(const zjo_mesh *m, const zjo_params *p, const double *Q,
                        double *dQ)
+{
+    flux_operator_h(m, p, Q, Q, dQ);
+}
+
+static void flux_operator_h(const zjo_mesh *m, const zjo_params *p,
+                            const double *Q, const double *H, double *dQ)
 {
     const int N = m->n_cells, F = m->n_faces, nbf = m->n_bf;
     double *QL = malloc(sizeof(double) * NV * F);
     double *QR = malloc(sizeof(double) * NV * F);
     double *hinv = malloc(sizeof(double) * N);
     zjo_reconstruct(m, p, Q, QL, QR, NULL);
-    for (int c = 0; c < N; ++c) hinv[c] = zjo_hinv(Q[N + c], p->eps_wet);
+    for (int c = 0; c < N; ++c) hinv[c] = zjo_hinv(H[N + c], p->eps_wet);
     memset(dQ, 0, sizeof(double) * NV * N);
     /* boundary Q_R = BC(Q_L reconstructed) or the periodic partner state */
     for (int b = 0; b < nbf; ++b) {
@@ -557,7 +597,7 @@ void zjo_flux_operator(const zjo_mesh *m, const zjo_params *p, const double *Q,
             int pc = m->bface_cell[m->bface_partner[b]];
             for (int v = 0; v < NV; ++v) o[v] = Q[(size_t)v * N + pc];
         } else {
-            zjo_bc_ghost(m->bface_kind[b], ql, n, o);
+            ghost_at(m, b, ql, n, o);
         }
         for (int v = 0; v < NV; ++v) QR[(size_t)v * F + f] = o[v];
     }
@@ -685,7 +725,8 @@ int zjo_step(const zjo_mesh *m, const zjo_params *p, double dt,
     if (p->order >= 2) {
         double *Q1 = malloc(sizeof(double) * n);
         memcpy(Q1, Qout, sizeof(double) * n);
-        zjo_flux_operator(m, p, Q1, dQ);
+        flux_operator_h(m, p, Q1,
+                        p->stage_aux == ZJO_STAGE_AUX_FROZEN ? Q : Q1, dQ);
         for (size_t i = 0; i < n; ++i) {
             double q2 = Q1[i] + dt * dQ[i];
             Qout[i] = 0.5 * (Q[i] + q2);

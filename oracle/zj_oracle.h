@@ -28,12 +28,18 @@
 extern "C" {
 #endif
 
-enum { ZJO_BC_EXTRAP = 0, ZJO_BC_WALL = 1, ZJO_BC_PERIODIC = 2 };
+enum { ZJO_BC_EXTRAP = 0, ZJO_BC_WALL = 1, ZJO_BC_PERIODIC = 2,
+       ZJO_BC_DIRICHLET = 3 };
 enum { ZJO_LIM_NONE = 0, ZJO_LIM_BJ = 1, ZJO_LIM_MINMOD = 2, ZJO_LIM_VK = 3,
        ZJO_LIM_VANLEER = 4 };
 enum { ZJO_RIEMANN_RUSANOV_NC = 0, ZJO_RIEMANN_RUSANOV_POS = 1,
        ZJO_RIEMANN_HLL = 2 };
-enum { ZJO_RECON_CONSERVATIVE = 0, ZJO_RECON_ETA = 1, ZJO_RECON_XZ = 2 };
+enum { ZJO_RECON_CONSERVATIVE = 0, ZJO_RECON_ETA = 1, ZJO_RECON_XZ = 2,
+       ZJO_RECON_ETA_HU = 3 };
+/* which 1/h the second SSP-RK2 stage's fluxes read: the stage state's own
+ * (solver_jax.py:1026-1041) or the step-start aux (the reference's recorded
+ * order-2 goldens were made with the latter) */
+enum { ZJO_STAGE_AUX_REFRESH = 0, ZJO_STAGE_AUX_FROZEN = 1 };
 enum { ZJO_POS_NONE = 0, ZJO_POS_ZHANG_SHU = 1 };
 
 typedef struct {
@@ -53,6 +59,8 @@ typedef struct {
     const double *lsq_w;         /* (N, max_st, 2) weights per component  */
     const int *gslot;            /* (N, 2) boundary faces in the stencil   */
     const double *lsq_wg;        /* (N, 2, 2) ghost weights [slot][comp]   */
+    const int *bface_mask;       /* (n_bf) Dirichlet rows (bit v), or NULL */
+    const double *bface_value;   /* (4, n_bf) Dirichlet values, or NULL    */
 } zjo_mesh;
 
 typedef struct {
@@ -62,6 +70,7 @@ typedef struct {
      * deviation cap, and the front pre-detector tolerance (<= 0: off) */
     int positivity;
     double front_tol;
+    int stage_aux;               /* ZJO_STAGE_AUX_* */
 } zjo_params;
 
 double zjo_hinv(double h, double eps);

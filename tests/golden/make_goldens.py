@@ -15,7 +15,8 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 REFS = pathlib.Path("/root/reference/tests/refs")
 NAMES = ["parity", "stoker_wet", "ritter_dry", "wb_lake", "bc_wall",
-         "swe_2d", "stoker_wet_o2_small", "ritter_dry_o2_small", "bc_periodic"]
+         "swe_2d", "stoker_wet_o2_small", "ritter_dry_o2_small", "bc_periodic",
+         "bc_inflow", "boundary_o2_small"]
 
 
 def copy_refs():

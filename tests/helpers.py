@@ -30,6 +30,25 @@ def tilted_ic(x):
     return [0.0, 0.5 + 0.05 * (x[0] - 0.5), 0.0]
 
 
+def smooth_state(x):
+    """(h, q) of the smooth Dirichlet profile (nonzero slope at both ends)."""
+    return (1.0 + 0.1 * np.sin(2.0 * np.pi * x + 0.7),
+            0.1 * np.cos(2.0 * np.pi * x))
+
+
+def smooth_dirichlet_ic(x):
+    return [0.0, *smooth_state(float(x[0]))]
+
+
+def inflow_bc():
+    """Dirichlet on h and q_0 at both ends of [0, 1], at the smooth
+    profile's own boundary values (tests/cases.py:168-183)."""
+    from zoomy_hip.mesh import Dirichlet
+    (hl, ql), (hr, qr) = smooth_state(0.0), smooth_state(1.0)
+    return {"left": [Dirichlet("h", hl), Dirichlet("q_0", ql)],
+            "right": [Dirichlet("h", hr), Dirichlet("q_0", qr)]}
+
+
 def pulse_ic(x):
     return [0.0, 0.5 + 0.05 * np.exp(-(x[0] ** 2 + x[1] ** 2) / 0.2), 0.0,
             0.0]

@@ -7,8 +7,8 @@ reference's tests/cases.py and tests/runtime/*.py.
 import numpy as np
 import pytest
 
-from helpers import (ic_rows, lake_ic, pulse_ic, ritter_ic, stoker_ic,
-                     tilted_ic)
+from helpers import (ic_rows, inflow_bc, lake_ic, pulse_ic, ritter_ic,
+                     smooth_dirichlet_ic, stoker_ic, tilted_ic)
 from oracle.oracle import Oracle
 from zoomy_hip.mesh import Mesh
 
@@ -25,22 +25,23 @@ from zoomy_hip.mesh import Mesh
     # test_bc_runtime.py: periodic kind (the seam pairs the partner cell's
     # state with the inner cell's aux; s_M from each face state's own 1/h)
     ("bc_periodic", 40, 1.0, 0.2, "periodic", tilted_ic, "rusanov_nc"),
+    # test_bc_runtime.py: inflow kind (Dirichlet h, q_0; the face value is
+    # the ghost state)
+    ("bc_inflow", 40, 1.0, 0.2, "inflow", tilted_ic, "rusanov_nc"),
 ])
 def test_1d_goldens(golden, name, n, L, t_end, bc, ic, riemann):
-    m = Mesh.line(n, 0.0, L).set_bc(bc)
+    m = Mesh.line(n, 0.0, L).set_bc(inflow_bc() if bc == "inflow" else bc)
     o = Oracle(m, order=1, riemann=riemann, cfl=0.9)
     Q, t, k = o.march(ic_rows(m, ic), t_end=t_end)
     ref = golden(name)
     assert abs(t - t_end) < 1e-12
-    assert np.abs(Q[:3] - ref["Q"]).max() < 1e-15
+    # round-off only (bc_inflow: 34 steps, |q| ~ 0.7, 3 ulp)
+    assert np.abs(Q[:3] - ref["Q"]).max() < 4e-15
     A = o.aux(Q)
-    if name in ("bc_wall", "bc_periodic"):
-        # boundary-cell derivative aux rows are not reproduced (the boundary
-        # row weights of zoomy_core's LSQMesh are unknown); interior exact
-        assert np.abs(A[:, 1:-1] - ref["Qaux"][:, 1:-1]).max() < 1e-12
-        assert np.abs(A[3] - ref["Qaux"][3]).max() < 1e-12
-    else:
-        assert np.abs(A - ref["Qaux"]).max() < 1e-12
+    # every derivative aux row, boundary cells included: a 1-D end cell's
+    # LSQ stencil is its neighbour and that neighbour's other neighbour, plus
+    # the ghost row with zero delta
+    assert np.abs(A - ref["Qaux"]).max() < 1e-12
 
 
 def test_wb_lake_is_well_balanced(golden):
@@ -69,18 +70,50 @@ def test_swe_2d_golden(golden):
     assert np.abs(o.aux(Q) - ref["Qaux"]).max() < 1e-13
 
 
-@pytest.mark.parametrize("name,ic", [("stoker_wet_o2_small", stoker_ic),
-                                     ("ritter_dry_o2_small", ritter_ic)])
-def test_o2_goldens_not_reproduced(golden, name, ic):
-    """Order-2 LSQ-MUSCL goldens (20 cells, 2 steps): the composition of
-    zoomy_core's LSQ stencil/limiter/flux at order 2 is not reproduced; the
-    oracle stays within a few percent.  Recorded so that a change that
-    closes (or widens) the gap is noticed."""
-    m = Mesh.line(20, 0.0, 10.0).set_bc("extrapolation")
-    o = Oracle(m, order=2, limiter="minmod", riemann="rusanov_nc", cfl=0.9)
-    Q, _, k = o.march(ic_rows(m, ic), n_steps=2)
+O2_CASES = [
+    # test_swe_stoker_wet.py:43-66 (plain conservative LSQ-MUSCL)
+    ("stoker_wet_o2_small", 10.0, "extrapolation", stoker_ic, {}, False),
+    # test_swe_ritter_dry.py:42-72 (wet_dry_o2: eta reconstruction + MOOD)
+    ("ritter_dry_o2_small", 10.0, "extrapolation", ritter_ic,
+     dict(recon="eta_hu"), True),
+    # test_bc_runtime.py:43-69 (smooth Dirichlet, nonzero boundary slope)
+    ("boundary_o2_small", 1.0, "inflow", smooth_dirichlet_ic, {}, False),
+]
+
+
+def _o2_march(name, L, bc, ic, kw, mood, stage_aux):
+    m = Mesh.line(20, 0.0, L).set_bc(inflow_bc() if bc == "inflow" else bc)
+    o = Oracle(m, order=2, limiter="minmod", riemann="rusanov_nc", cfl=0.9,
+               stage_aux=stage_aux, **kw)
+    Q = ic_rows(m, ic)
+    for _ in range(2):
+        dt = o.dt(Q)
+        Q = o.step_mood(dt, Q)[0] if mood else o.step(dt, Q)[0]
+    return o, Q
+
+
+@pytest.mark.parametrize("name,L,bc,ic,kw,mood", O2_CASES)
+def test_o2_goldens(golden, name, L, bc, ic, kw, mood):
+    """Order-2 goldens (20 cells, 2 adaptive steps) reproduced to round-off
+    with the second SSP-RK2 stage reading the step-start aux 1/h (the
+    goldens predate the stage-state aux refresh of solver_jax.py:1026-1041)
+    and, for the dry front, the conservative-momentum eta reconstruction
+    with its 1e-3 depth threshold (the class EmittedWLSQMUSCLJAX replaced,
+    reconstruction_jax.py:995-1008)."""
+    o, Q = _o2_march(name, L, bc, ic, kw, mood, "frozen")
+    ref = golden(name)
+    assert np.abs(Q[:3] - ref["Q"]).max() < 1e-14
+    assert np.abs(o.aux(Q) - ref["Qaux"]).max() <= 1e-12 * np.abs(
+        ref["Qaux"]).max()
+
+
+@pytest.mark.parametrize("name,L,bc,ic,kw,mood", O2_CASES)
+def test_o2_refreshed_stage_aux_differs(golden, name, L, bc, ic, kw, mood):
+    """The current source's refresh of the stage-2 aux (the default) moves
+    the order-2 goldens by O(dt) terms, conservatively: mass unchanged."""
+    _, Q = _o2_march(name, L, bc, ic, kw, mood, "refresh")
     ref = golden(name)["Q"]
     err = np.abs(Q[:3] - ref).max()
-    assert 1e-6 < err < 5e-4
-    # mass identical (both conservative)
-    assert abs(Q[1].sum() - ref[1].sum()) < 1e-15
+    assert 1e-7 < err < 5e-4
+    if bc != "inflow":
+        assert abs(Q[1].sum() - ref[1].sum()) < 1e-15

@@ -26,7 +26,7 @@ namespace zj {
 enum { LIM_NONE = 0, LIM_BJ = 1, LIM_MINMOD = 2, LIM_VK = 3, LIM_VANLEER = 4 };
 enum { RS_RUSANOV_NC = 0, RS_RUSANOV = 1, RS_HLL = 2 };
 enum { RECON_CONS = 0, RECON_ETA = 1, RECON_XZ = 2 };
-enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2 };
+enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2, BC_DIRICHLET = 3 };
 
 constexpr int MAXF = 4;      // faces per cell (quad)
 constexpr int BLOCK = 256;   // threads per block (4 wave64)
@@ -45,7 +45,8 @@ template <typename T> __device__ __forceinline__ T hinv_of(T h, T eps) {
 }
 
 // Ghost state at a boundary face from the inner state q (Wall mirrors the
-// normal momentum, Extrapolation copies).
+// normal momentum, Extrapolation and Dirichlet copy; the prescribed rows of
+// a Dirichlet face are set by the caller).
 template <typename T>
 __device__ __forceinline__ void bc_ghost(int kind, const T* q, T nx, T ny, T* o) {
     o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[3];

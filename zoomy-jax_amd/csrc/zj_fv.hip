@@ -48,7 +48,32 @@ struct DevMesh {
     const double* rin;    // (N)
     const int* bf_kind;   // (nbf)
     const int* bf_pcell;  // (nbf) periodic partner cell or -1
+    const int* bf_mask;   // (nbf) Dirichlet rows (bit v)
+    const double* bf_val; // (4, nbf) Dirichlet values
 };
+
+// Ghost state of boundary face bf seen from the face state q: the periodic
+// partner cell, or the kind's ghost with the prescribed rows of a Dirichlet
+// face set to their values (the face value is the ghost: tests/refs/
+// bc_inflow of the reference holds to round-off only with that).
+template <typename T>
+__device__ __forceinline__ void ghost_state(const DevMesh& m, const T* __restrict__ Qin,
+                                            int bf, const T* q, T nx, T ny, T* o) {
+    const int kind = m.bf_kind[bf];
+    if (kind == BC_PERIODIC) {
+        const int pc = m.bf_pcell[bf];
+        o[0] = Qin[pc]; o[1] = Qin[(size_t)m.N + pc];
+        o[2] = Qin[2 * (size_t)m.N + pc]; o[3] = Qin[3 * (size_t)m.N + pc];
+        return;
+    }
+    bc_ghost(kind, q, nx, ny, o);
+    if (kind == BC_DIRICHLET) {
+        const int mask = m.bf_mask[bf];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (mask >> v & 1) o[v] = T(m.bf_val[(size_t)v * m.nbf + bf]);
+    }
+}
 
 struct Opts {
     double g, cfl, eps, dt_max, manning;
@@ -619,8 +644,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     const int bf = -1 - m.nbr[k * N + gi];
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
-                    if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qg);
-                    else bc_ghost(kind, q, T(nx), T(ny), qg);
+                    ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
                     to_w(o.recon, qg, hinv_of(qg[1], eps), wg);
                     const double rx = 2.0 * (fx[k] - cc.x), ry = 2.0 * (fy[k] - cc.y);
 #pragma unroll
@@ -836,9 +860,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
             } else {
                 const int bf = -1 - m.nbr[k * N + gi];
-                const int kind = m.bf_kind[bf];
-                if (kind == BC_PERIODIC) load_q(Qin, N, m.bf_pcell[bf], qR);
-                else bc_ghost(kind, qL, nx, ny, qR);
+                ghost_state(m, Qin, bf, qL, nx, ny, qR);
             }
             T f[4];
             face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, T(o.eps), f);
@@ -1009,7 +1031,10 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     std::vector<int> pcell(d->n_bf > 0 ? d->n_bf : 1, -1);
     for (int b = 0; b < d->n_bf; ++b) {
         int k = d->bface_kind[b];
-        if (k < 0 || k > 2) return ZJ_EINVAL;
+        if (k < 0 || k > 3) return ZJ_EINVAL;
+        if (k == ZJ_BC_DIRICHLET && (!d->bface_mask || !d->bface_value ||
+                                     (d->bface_mask[b] & ~15)))
+            return ZJ_EINVAL;
         if (k == ZJ_BC_PERIODIC) {
             int pb = d->bface_partner ? d->bface_partner[b] : -1;
             if (pb < 0 || pb >= d->n_bf) return ZJ_EINVAL;
@@ -1031,6 +1056,20 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     rc |= dev_upload(m, d->cell_inradius, sizeof(double) * N, &p); m->d.rin = (const double*)p;
     rc |= dev_upload(m, d->n_bf ? d->bface_kind : nullptr, sizeof(int) * d->n_bf, &p); m->d.bf_kind = (const int*)p;
     rc |= dev_upload(m, pcell.data(), sizeof(int) * pcell.size(), &p); m->d.bf_pcell = (const int*)p;
+    {
+        // Dirichlet rows (zero masks when the mesh has none)
+        const int nb = d->n_bf > 0 ? d->n_bf : 1;
+        std::vector<int> mask(nb, 0);
+        std::vector<double> val((size_t)4 * nb, 0.0);
+        if (d->bface_mask && d->bface_value)
+            for (int b = 0; b < d->n_bf; ++b) {
+                mask[b] = d->bface_kind[b] == ZJ_BC_DIRICHLET ? d->bface_mask[b] : 0;
+                for (int v = 0; v < 4; ++v)
+                    val[(size_t)v * nb + b] = d->bface_value[(size_t)v * d->n_bf + b];
+            }
+        rc |= dev_upload(m, mask.data(), sizeof(int) * mask.size(), &p); m->d.bf_mask = (const int*)p;
+        rc |= dev_upload(m, val.data(), sizeof(double) * val.size(), &p); m->d.bf_val = (const double*)p;
+    }
     if (rc) {
         zj_mesh_destroy(m);
         return ZJ_ENOMEM;

@@ -38,7 +38,9 @@ class MeshDesc(ctypes.Structure):
                 ("bface_cell", ctypes.c_void_p),
                 ("bface_slot", ctypes.c_void_p),
                 ("bface_kind", ctypes.c_void_p),
-                ("bface_partner", ctypes.c_void_p)]
+                ("bface_partner", ctypes.c_void_p),
+                ("bface_mask", ctypes.c_void_p),
+                ("bface_value", ctypes.c_void_p)]
 
 
 class Params(ctypes.Structure):

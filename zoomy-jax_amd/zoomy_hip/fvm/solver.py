@@ -76,8 +76,8 @@ def _p(t):
 def mesh_desc(mesh: Mesh):
     """``zj_mesh_desc`` of a host mesh; returns (desc, arrays kept alive)."""
     N = mesh.n_cells
-    ex = mesh.extra_stencil() if mesh.dim == 2 else \
-        np.full((N, 1), -1, np.int32)
+    ex = mesh.extra_stencil()
+    dmask, dval = mesh.bface_dirichlet()
     keep = dict(
         cv=np.ascontiguousarray(mesh.cell_vertices.T, np.int32),
         nbr=np.ascontiguousarray(mesh.cell_nbr.T, np.int32),
@@ -91,12 +91,14 @@ def mesh_desc(mesh: Mesh):
         bpart=np.ascontiguousarray(
             mesh.periodic_partner if mesh.periodic_partner is not None
             else -np.ones(mesh.n_bfaces), np.int32),
+        dmask=np.ascontiguousarray(dmask, np.int32),
+        dval=np.ascontiguousarray(dval, np.float64),
     )
     d = L.MeshDesc(mesh.dim, N, len(mesh.verts), mesh.max_faces,
                    keep["ex"].shape[0], mesh.n_bfaces,
                    *[keep[n].ctypes.data for n in (
                        "cv", "nbr", "ex", "verts", "cen", "rin", "bcell",
-                       "bslot", "bkind", "bpart")])
+                       "bslot", "bkind", "bpart", "dmask", "dval")])
     return d, keep
 
 
@@ -116,29 +118,7 @@ class DeviceMesh:
     def __init__(self, mesh: Mesh):
         self.mesh = mesh
         lib = L.lib()
-        N = mesh.n_cells
-        ex = mesh.extra_stencil() if mesh.dim == 2 else \
-            np.full((N, 1), -1, np.int32)
-        self._keep = dict(
-            cv=np.ascontiguousarray(mesh.cell_vertices.T, np.int32),
-            nbr=np.ascontiguousarray(mesh.cell_nbr.T, np.int32),
-            ex=np.ascontiguousarray(ex.T, np.int32),
-            verts=np.ascontiguousarray(mesh.verts, np.float64),
-            cen=np.ascontiguousarray(mesh.cell_centers().T, np.float64),
-            rin=np.ascontiguousarray(mesh.cell_inradius(), np.float64),
-            bcell=np.ascontiguousarray(mesh.bface_cell, np.int32),
-            bslot=np.ascontiguousarray(mesh.bface_slot, np.int32),
-            bkind=np.ascontiguousarray(mesh.bface_kind(), np.int32),
-            bpart=np.ascontiguousarray(
-                mesh.periodic_partner if mesh.periodic_partner is not None
-                else -np.ones(mesh.n_bfaces), np.int32),
-        )
-        k = self._keep
-        d = L.MeshDesc(mesh.dim, N, len(mesh.verts), mesh.max_faces,
-                       k["ex"].shape[0], mesh.n_bfaces,
-                       *[k[n].ctypes.data for n in (
-                           "cv", "nbr", "ex", "verts", "cen", "rin", "bcell",
-                           "bslot", "bkind", "bpart")])
+        d, self._keep = mesh_desc(mesh)
         h = ctypes.c_void_p()
         L.check(lib.zj_mesh_create(ctypes.byref(d), ctypes.byref(h)),
                 "zj_mesh_create")

@@ -38,13 +38,35 @@ import numpy as np
 # Boundary-condition kinds understood by the kernels.  The reference's BC
 # kernels come from zoomy_core (``zoomy_jax/transformation/jax_runtime.py:
 # 837-863``); Wall mirrors the normal momentum and Extrapolation copies the
-# inner state (``zoomy_jax/fvm/reconstruction_jax.py:494-510``).
+# inner state (``zoomy_jax/fvm/reconstruction_jax.py:494-510``); Dirichlet
+# prescribes the face value of some state rows and extrapolates the others.
 BC_EXTRAPOLATION = 0
 BC_WALL = 1
 BC_PERIODIC = 2
+BC_DIRICHLET = 3
 
 _BC_NAMES = {"extrapolation": BC_EXTRAPOLATION, "wall": BC_WALL,
-             "periodic": BC_PERIODIC}
+             "periodic": BC_PERIODIC, "dirichlet": BC_DIRICHLET}
+
+# state rows by the reference's variable names (SWE: b, h, q_0, q_1)
+_ROWS = {"b": 0, "h": 1, "q_0": 2, "hu": 2, "q_1": 3, "hv": 3}
+
+
+@dataclass(frozen=True)
+class Dirichlet:
+    """Prescribed value of one state row on a boundary tag (the reference's
+    ``BC.Dirichlet(tag=..., on=..., value=...)``, tests/cases.py:168-183).
+    A tag given a list of these prescribes each listed row; the rows left
+    out are extrapolated from the inner state."""
+    on: str
+    value: float
+
+    def row(self) -> int:
+        try:
+            return _ROWS[self.on]
+        except KeyError:
+            raise ValueError(f"unknown state row {self.on!r}; "
+                             f"known: {sorted(_ROWS)}") from None
 
 
 def bc_kind(name) -> int:
@@ -69,6 +91,7 @@ class Mesh:
     tags: List[str]
     bc: Dict[str, int] = field(default_factory=dict)
     periodic_partner: Optional[np.ndarray] = None   # (n_bf,) partner bface
+    bc_values: Dict[str, Dict[int, float]] = field(default_factory=dict)
 
     # ── construction ──────────────────────────────────────────────────
     @staticmethod
@@ -242,11 +265,29 @@ class Mesh:
 
     # ── boundary conditions ───────────────────────────────────────────
     def set_bc(self, spec) -> "Mesh":
-        """``spec``: a BC name for every tag, or ``{tag: name}``."""
-        if isinstance(spec, (str, int)):
+        """``spec``: a BC name for every tag, or ``{tag: name}``, where a
+        tag may instead take a :class:`Dirichlet` or a list of them."""
+        if isinstance(spec, (str, int, Dirichlet)):
             spec = {t: spec for t in self.tags}
-        self.bc = {t: bc_kind(spec.get(t, "extrapolation"))
-                   for t in self.tags}
+        unknown = set(spec) - set(self.tags)
+        if unknown:
+            raise ValueError(f"unknown boundary tags {sorted(unknown)}; "
+                             f"known: {self.tags}")
+        self.bc, self.bc_values = {}, {}
+        for t in self.tags:
+            v = spec.get(t, "extrapolation")
+            if isinstance(v, Dirichlet):
+                v = [v]
+            if isinstance(v, (list, tuple)):
+                if not v or not all(isinstance(d, Dirichlet) for d in v):
+                    raise ValueError(f"tag {t!r}: expected Dirichlet entries")
+                self.bc[t] = BC_DIRICHLET
+                self.bc_values[t] = {d.row(): float(d.value) for d in v}
+            else:
+                self.bc[t] = bc_kind(v)
+                if self.bc[t] == BC_DIRICHLET:
+                    raise ValueError(f"tag {t!r}: a Dirichlet BC needs its "
+                                     f"values (Dirichlet(on, value))")
         if any(k == BC_PERIODIC for k in self.bc.values()):
             self._resolve_periodic()
         return self
@@ -256,6 +297,18 @@ class Mesh:
             self.set_bc("extrapolation")
         table = np.array([self.bc[t] for t in self.tags], np.int32)
         return table[self.bface_tag]
+
+    def bface_dirichlet(self):
+        """``(mask (n_bf,) int32, values (4, n_bf) float64)``: bit v of
+        mask[b] is set when row v of boundary face b is prescribed."""
+        mask = np.zeros(self.n_bfaces, np.int32)
+        val = np.zeros((4, self.n_bfaces))
+        for ti, t in enumerate(self.tags):
+            sel = self.bface_tag == ti
+            for row, x in self.bc_values.get(t, {}).items():
+                mask[sel] |= 1 << row
+                val[row, sel] = x
+        return mask, val
 
     def _resolve_periodic(self):
         """Pair left<->right and bottom<->top boundary faces by position
@@ -395,8 +448,18 @@ class Mesh:
         vptr = np.cumsum(vptr)
         vcells = cc[o]
         if self.dim == 1:
-            st = self.cell_nbr.copy()
-            st[st < 0] = -1
+            # an end cell reaches one cell further: its neighbour, then that
+            # neighbour's other neighbour (the reference's bc_wall,
+            # bc_periodic and bc_inflow derivative aux rows at the end cells
+            # and its boundary_o2_small end state need exactly this)
+            nb = self.cell_nbr
+            st = np.where(nb >= 0, nb, -1).astype(np.int64)
+            for c in np.nonzero((nb < 0).any(1))[0]:
+                own = nb[c][nb[c] >= 0]
+                if len(own) != 1:
+                    continue
+                far = [x for x in nb[own[0]] if x >= 0 and x != c]
+                st[c] = [own[0], far[0] if far else -1]
             return st.astype(np.int32)
         rows = []
         maxs = 0
@@ -521,6 +584,7 @@ class Mesh:
                  bface_slot=self.bface_slot.copy(),
                  bface_tag=self.bface_tag.copy(), tags=list(self.tags),
                  bc=dict(self.bc),
+                 bc_values={t: dict(v) for t, v in self.bc_values.items()},
                  periodic_partner=(None if self.periodic_partner is None
                                    else self.periodic_partner.copy()))
         if ex is not None:
