@@ -37,8 +37,13 @@ enum { ZJ_LIM_NONE = 0, ZJ_LIM_BARTH_JESPERSEN = 1, ZJ_LIM_MINMOD = 2,
 enum { ZJ_RIEMANN_RUSANOV_NC = 0, ZJ_RIEMANN_RUSANOV = 1, ZJ_RIEMANN_HLL = 2 };
 /* ZJ_RECON_XZ: conservative variables with the Xing-Zhang cell-mean
  * positivity scaling (PositivityPreservingLSQMUSCLJAX,
- * reconstruction_jax.py:853-984). */
-enum { ZJ_RECON_CONSERVATIVE = 0, ZJ_RECON_ETA = 1, ZJ_RECON_XZ = 2 };
+ * reconstruction_jax.py:853-984).  ZJ_RECON_ETA: (b, eta, u, v) with the
+ * face clip (EmittedWLSQMUSCLJAX, :987-1292).  ZJ_RECON_ETA_HU: (b, eta,
+ * hu, hv), cells with h < 1e-3 at first order and face momentum zeroed
+ * where h_f < 1e-3 -- the class EmittedWLSQMUSCLJAX replaced (:995-1008),
+ * with which the reference's ritter_dry_o2_small golden was recorded. */
+enum { ZJ_RECON_CONSERVATIVE = 0, ZJ_RECON_ETA = 1, ZJ_RECON_XZ = 2,
+       ZJ_RECON_ETA_HU = 3 };
 enum { ZJ_F64 = 0, ZJ_F32 = 1 };
 
 /* Host description of a mesh (topology + vertex coordinates).  All arrays
@@ -127,6 +132,14 @@ int zj_solver_set_mood(zj_solver *s, int32_t on);
  * reconstructions, as in the reference. */
 enum { ZJ_POSITIVITY_NONE = 0, ZJ_POSITIVITY_ZHANG_SHU = 1 };
 int zj_solver_set_positivity(zj_solver *s, int32_t method, double front_tol);
+/* Which 1/h the fluxes of the second SSP-RK2 stage read: REFRESH (the
+ * default) that stage's own state, as solver_jax.py:1026-1041 does;
+ * FROZEN the step-start aux, which the reference's recorded order-2 goldens
+ * (stoker_wet_o2_small, ritter_dry_o2_small, boundary_o2_small) were made
+ * with.  The reconstruction's own 1/h (eta velocity) is the stage state's
+ * either way. */
+enum { ZJ_STAGE_AUX_REFRESH = 0, ZJ_STAGE_AUX_FROZEN = 1 };
+int zj_solver_set_stage_aux(zj_solver *s, int32_t mode);
 /* {steps redone, cells demoted} since MOOD was enabled (host-synchronous). */
 int zj_solver_mood_stats(zj_solver *s, int32_t *out2);
 /* post_step aux refresh (solver_jax.py:1133-1172): Qaux rows per direction

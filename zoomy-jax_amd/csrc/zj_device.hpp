@@ -25,7 +25,7 @@ namespace zj {
 
 enum { LIM_NONE = 0, LIM_BJ = 1, LIM_MINMOD = 2, LIM_VK = 3, LIM_VANLEER = 4 };
 enum { RS_RUSANOV_NC = 0, RS_RUSANOV = 1, RS_HLL = 2 };
-enum { RECON_CONS = 0, RECON_ETA = 1, RECON_XZ = 2 };
+enum { RECON_CONS = 0, RECON_ETA = 1, RECON_XZ = 2, RECON_ETA_HU = 3 };
 enum { BC_EXTRAP = 0, BC_WALL = 1, BC_PERIODIC = 2, BC_DIRICHLET = 3 };
 
 constexpr int MAXF = 4;      // faces per cell (quad)

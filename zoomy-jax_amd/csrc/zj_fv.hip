@@ -80,7 +80,12 @@ struct Opts {
     int order, lim, rs, recon;
     int pos;           // eta reconstruction: 1 = Zhang-Shu deviation cap
     double front_tol;  // eta reconstruction: front pre-detector (<= 0: off)
+    int stage_aux;     // 1: the second RK stage's fluxes read the step-start 1/h
 };
+
+// depth below which the conservative-momentum eta reconstruction demotes a
+// cell to first order and zeroes a face momentum
+constexpr double ETA_HU_EPS = 1e-3;
 
 struct Geo {
     double cx, cy, vol;
@@ -133,7 +138,8 @@ __device__ __forceinline__ void load_q(const T* __restrict__ Q, int N, int c, T*
     q[3] = Q[3 * N + c];
 }
 
-// Reconstruction variables: conservative (b, h, hu, hv) or (b, eta, u, v).
+// Reconstruction variables: conservative (b, h, hu, hv), (b, eta, u, v),
+// or (b, eta, hu, hv).
 template <typename T>
 __device__ __forceinline__ void to_w(int recon, const T* q, T hi, T* w) {
     if (recon == RECON_ETA) {
@@ -141,6 +147,8 @@ __device__ __forceinline__ void to_w(int recon, const T* q, T hi, T* w) {
         w[1] = q[1] + q[0];
         w[2] = q[2] * hi;
         w[3] = q[3] * hi;
+    } else if (recon == RECON_ETA_HU) {
+        w[0] = q[0]; w[1] = q[1] + q[0]; w[2] = q[2]; w[3] = q[3];
     } else {
         w[0] = q[0]; w[1] = q[1]; w[2] = q[2]; w[3] = q[3];
     }
@@ -158,6 +166,15 @@ __device__ __forceinline__ void recon_face(int recon, const T* q, T hi, const T*
         eta = tmax(eta, b);  // hydrostatic face clip, reconstruction_jax.py:1269-1272
         T h = eta - b;
         out[0] = b; out[1] = h; out[2] = u * h; out[3] = v * h;
+    } else if (recon == RECON_ETA_HU) {
+        T b = q[0] + s[0] * rx + s[1] * ry;
+        T eta = (q[1] + q[0]) + s[2] * rx + s[3] * ry;
+        eta = tmax(eta, b);
+        T h = eta - b;
+        const bool dry = h < T(ETA_HU_EPS);
+        out[0] = b; out[1] = h;
+        out[2] = dry ? T(0) : q[2] + s[4] * rx + s[5] * ry;
+        out[3] = dry ? T(0) : q[3] + s[6] * rx + s[7] * ry;
     } else {
 #pragma unroll
         for (int v = 0; v < 4; ++v) out[v] = q[v] + s[2 * v] * rx + s[2 * v + 1] * ry;
@@ -745,15 +762,16 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     }
                 }
             }
-            if (o.recon == RECON_ETA) {
+            const bool eta_rows = o.recon == RECON_ETA || o.recon == RECON_ETA_HU;
+            if (eta_rows) {
                 phi[0] = tmin(phi[0], phi[1]);
                 phi[2] = tmin(phi[2], phi[1]);
                 phi[3] = tmin(phi[3], phi[1]);
             }
             // MOOD (o1_mask) or the front pre-detector: first order here
-            bool first = o1_mask && o1_mask[gi];
-            if (o.recon == RECON_XZ ||
-                (o.recon == RECON_ETA && (o.pos || o.front_tol > 0.0))) {
+            bool first = (o1_mask && o1_mask[gi]) ||
+                         (o.recon == RECON_ETA_HU && q[1] < T(ETA_HU_EPS));
+            if (o.recon == RECON_XZ || (eta_rows && (o.pos || o.front_tol > 0.0))) {
                 // Xing-Zhang-Shu theta of the limited depth slope
                 // (reconstruction_jax.py:840-850, :924-937, :1211-1244)
                 T sx, sy;
@@ -815,6 +833,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         T q[4];
         lds_q4(Qs, ML, ls, q);
         const T hic = Hs[ls];
+        // the fluxes' 1/h: this stage's state, or in a frozen second stage
+        // the step state's (the face reconstruction keeps this stage's)
+        const bool frz = !SPEC && o.stage_aux && mode == 1;
+        const T hif = frz ? hinv_of(q0[1], eps) : hic;
         T sc[8];
         if (use_slopes) {
 #pragma unroll
@@ -840,11 +862,13 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             if (use_slopes) recon_face(o.recon, q, hic, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qL);
             else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
             const uint16_t nl = nls[k];
-            T hiR = hic;
+            T hiR = hic, hfR = hif;
             if (nl != LNONE) {
                 T qn[4];
                 lds_q4(Qs, ML, nl, qn);
                 hiR = Hs[nl];
+                hfR = hiR;
+                if (frz) hfR = hinv_of(Q0[(size_t)N + (nl < nown ? c0 + nl : lids[nl])], eps);
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
@@ -863,7 +887,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 ghost_state(m, Qin, bf, qL, nx, ny, qR);
             }
             T f[4];
-            face_flux_left(o.rs, qL, qR, hic, hiR, nx, ny, g, T(o.eps), f);
+            face_flux_left(o.rs, qL, qR, hif, hfR, nx, ny, g, T(o.eps), f);
             const T L = T(len);
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) acc[vv] += f[vv] * L;
@@ -1152,14 +1176,14 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
     if (!m || !p || !out) return ZJ_EINVAL;
     if (p->dtype != ZJ_F64 && p->dtype != ZJ_F32) return ZJ_EINVAL;
     if (p->limiter < 0 || p->limiter > 4 || p->riemann < 0 || p->riemann > 2 ||
-        p->recon < 0 || p->recon > 2 || p->order < 1 || p->order > 2)
+        p->recon < 0 || p->recon > 3 || p->order < 1 || p->order > 2)
         return ZJ_EINVAL;
     zj_solver* s = new (std::nothrow) zj_solver();
     if (!s) return ZJ_ENOMEM;
     s->m = m;
     s->p = *p;
     s->o = Opts{p->g, p->cfl, p->eps_wet, p->dt_max, p->manning_n,
-                p->order, p->limiter, p->riemann, p->recon, 0, -1.0};
+                p->order, p->limiter, p->riemann, p->recon, 0, -1.0, 0};
     const size_t esz = p->dtype == ZJ_F64 ? 8 : 4;
     const int N = m->d.N;
     s->mass_blocks = std::min(1024, nblocks(N));
@@ -1207,7 +1231,7 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
-                      s->o.recon == RECON_CONS;
+                      s->o.recon == RECON_CONS && !s->o.stage_aux;
     auto kern = spec ? k_stage<T, 1> : k_stage<T, 0>;
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
@@ -1346,6 +1370,12 @@ int zj_solver_set_positivity(zj_solver* s, int32_t method, double front_tol) {
         return ZJ_EINVAL;
     s->o.pos = method;
     s->o.front_tol = front_tol > 0.0 ? front_tol : -1.0;
+    return ZJ_OK;
+}
+
+int zj_solver_set_stage_aux(zj_solver* s, int32_t mode) {
+    if (!s || mode < ZJ_STAGE_AUX_REFRESH || mode > ZJ_STAGE_AUX_FROZEN) return ZJ_EINVAL;
+    s->o.stage_aux = mode;
     return ZJ_OK;
 }
 

@@ -16,8 +16,9 @@ PRECOND = {"mg": 0, "learned": 1, "mixed": 2}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}
-RECON = {"conservative": 0, "eta": 1, "xz": 2}
+RECON = {"conservative": 0, "eta": 1, "xz": 2, "eta_hu": 3}
 POSITIVITY = {"": 0, "zhang_shu": 1}
+STAGE_AUX = {"refresh": 0, "frozen": 1}
 DTYPES = {"float64": 0, "float32": 1}
 
 
@@ -101,6 +102,7 @@ _SIGS = {
     "zj_solver_set_mood": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "zj_solver_set_positivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.c_double]),
+    "zj_solver_set_stage_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "zj_solver_mood_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zj_imex_local": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_int32, ctypes.c_void_p,

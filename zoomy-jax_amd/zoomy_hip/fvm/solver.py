@@ -171,13 +171,23 @@ class HyperbolicSolver:
                  reconstruction_variables: str = "conservative",
                  compute_dt: Optional[Callable] = None, mood: bool = False,
                  positivity_method: str = "",
-                 front_theta_tol: Optional[float] = None):
+                 front_theta_tol: Optional[float] = None,
+                 stage_aux: str = "refresh"):
         """``reconstruction_variables``: ``"conservative"``, ``"eta"``
-        (well-balanced wet/dry recipe) or ``"xz"`` (conservative with the
-        Xing-Zhang cell-mean positivity scaling).  ``positivity_method``
-        (``""`` or ``"zhang_shu"``) and ``front_theta_tol`` (None or a value
-        in (0, 1]) act on the eta reconstruction only, as in the reference
-        (``solver_jax.py:196-226``)."""
+        (well-balanced wet/dry recipe on (b, eta, u)), ``"eta_hu"`` (the
+        same on (b, eta, hu) with the 1e-3 depth threshold of the class it
+        replaced) or ``"xz"`` (conservative with the Xing-Zhang cell-mean
+        positivity scaling).  ``positivity_method`` (``""`` or
+        ``"zhang_shu"``) and ``front_theta_tol`` (None or a value in (0, 1])
+        act on the eta reconstructions only, as in the reference
+        (``solver_jax.py:196-226``).  ``stage_aux``: ``"refresh"`` (the
+        second SSP-RK2 stage's fluxes read that stage's 1/h,
+        ``solver_jax.py:1026-1041``) or ``"frozen"`` (the step-start aux,
+        with which the reference's order-2 goldens were recorded)."""
+        if stage_aux not in L.STAGE_AUX:
+            raise ValueError(
+                f"stage_aux must be one of {sorted(L.STAGE_AUX)}")
+        self.stage_aux = stage_aux
         if reconstruction_variables not in L.RECON:
             raise ValueError(
                 f"reconstruction_variables must be one of {sorted(L.RECON)}")
@@ -230,6 +240,9 @@ class HyperbolicSolver:
         if self.mood:
             L.check(L.lib().zj_solver_set_mood(self._rt.handle, 1),
                     "zj_solver_set_mood")
+        L.check(L.lib().zj_solver_set_stage_aux(
+            self._rt.handle, L.STAGE_AUX[self.stage_aux]),
+            "zj_solver_set_stage_aux")
         if self.positivity_method or self.front_theta_tol is not None:
             L.check(L.lib().zj_solver_set_positivity(
                 self._rt.handle, L.POSITIVITY[self.positivity_method],
