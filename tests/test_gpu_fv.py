@@ -99,6 +99,26 @@ def test_reference_o2_goldens(golden, name, L, bc, ic, recon, mood):
     assert np.abs(A - ref["Qaux"]).max() <= 1e-12 * np.abs(ref["Qaux"]).max()
 
 
+@pytest.mark.parametrize("name,ic,order", [
+    ("swashes_stoker_wet_o1", stoker_ic, 1),
+    ("swashes_stoker_wet_o2", stoker_ic, 2),
+    ("swashes_ritter_dry_o1", ritter_ic, 1)])
+def test_reference_swashes_goldens(golden, name, ic, order):
+    """The 400-cell SWASHES end states (t = 6) by the HIP march: the wet
+    Stoker marches to round-off, the dry Ritter front to 1e-9 absolute (its
+    last wet cells, h < 1e-7, part; the same bound as the oracle's)."""
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.line(400, 0.0, 10.0).set_bc("extrapolation")
+    model = ShallowWater(dimension=1, order=order, limiter="minmod",
+                         riemann="rusanov_nc", initial_conditions=ic)
+    s = HyperbolicSolver(time_end=6.0, cfl=0.9, stage_aux="frozen")
+    Q, _ = s.solve(mesh, model)
+    ref = golden(name)["Q"]
+    err = np.abs(Q.cpu().numpy() - ref).max()
+    assert err < (2e-9 if "ritter" in name else 1e-13)
+
+
 @pytest.mark.parametrize("recon,stage_aux,ftol", [
     ("conservative", "frozen", None), ("eta", "frozen", 0.5),
     ("eta_hu", "frozen", None), ("eta_hu", "refresh", None),

@@ -117,3 +117,30 @@ def test_o2_refreshed_stage_aux_differs(golden, name, L, bc, ic, kw, mood):
     assert 1e-7 < err < 5e-4
     if bc != "inflow":
         assert abs(Q[1].sum() - ref[1].sum()) < 1e-15
+
+
+@pytest.mark.parametrize("name,ic,order", [
+    ("swashes_stoker_wet_o1", stoker_ic, 1),
+    ("swashes_stoker_wet_o2", stoker_ic, 2),
+    ("swashes_ritter_dry_o1", ritter_ic, 1)])
+def test_swashes_goldens(golden, name, ic, order):
+    """The 400-cell end states of the SWASHES study (test_swashes_
+    convergence.py:19-47: (0, 10), t = 6, open ends, minmod).  The wet
+    Stoker marches hold to round-off (order 2 with the frozen stage-2 aux);
+    the dry Ritter front at order 1 holds to 1e-9 absolute: its last wet
+    cells, with h < 1e-7, part (the march matches ritter_dry at 100 cells
+    exactly).  The order-2 Ritter golden of the same study is reproduced by
+    none of the variants here and is not used."""
+    m = Mesh.line(400, 0.0, 10.0).set_bc("extrapolation")
+    o = Oracle(m, order=order, limiter="minmod", riemann="rusanov_nc",
+               cfl=0.9, stage_aux="frozen")
+    Q, t, k = o.march(ic_rows(m, ic), t_end=6.0)
+    ref = golden(name)
+    assert abs(t - 6.0) < 1e-12
+    err = np.abs(Q[:3] - ref["Q"]).max()
+    if "ritter" in name:
+        assert err < 2e-9
+        wet = ref["Q"][1] > 1e-4
+        assert (np.abs(Q[1] - ref["Q"][1])[wet] / ref["Q"][1][wet]).max() < 1e-6
+    else:
+        assert err < 1e-14
