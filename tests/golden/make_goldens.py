@@ -17,7 +17,8 @@ REFS = pathlib.Path("/root/reference/tests/refs")
 NAMES = ["parity", "stoker_wet", "ritter_dry", "wb_lake", "bc_wall",
          "swe_2d", "stoker_wet_o2_small", "ritter_dry_o2_small", "bc_periodic",
          "bc_inflow", "boundary_o2_small", "swashes_stoker_wet_o1",
-         "swashes_stoker_wet_o2", "swashes_ritter_dry_o1"]
+         "swashes_stoker_wet_o2", "swashes_ritter_dry_o1", "boundary_order2",
+         "wb_long"]
 
 
 def copy_refs():

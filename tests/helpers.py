@@ -30,6 +30,17 @@ def tilted_ic(x):
     return [0.0, 0.5 + 0.05 * (x[0] - 0.5), 0.0]
 
 
+def acoustic_ic(x):
+    """Mode-1 standing wave of amplitude 1e-4 on h = 1 over [0, 1]."""
+    return [0.0, 1.0 + 1e-4 * np.cos(np.pi * x[0]), 0.0]
+
+
+def bump_lake_ic(x):
+    """Lake at rest (eta = 0.3) over a Gaussian bump at x = 5."""
+    b = 0.1 * np.exp(-((x[0] - 5.0) ** 2) / 0.5)
+    return [b, 0.3 - b, 0.0]
+
+
 def smooth_state(x):
     """(h, q) of the smooth Dirichlet profile (nonzero slope at both ends)."""
     return (1.0 + 0.1 * np.sin(2.0 * np.pi * x + 0.7),

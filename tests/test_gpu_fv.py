@@ -7,7 +7,8 @@ end states are also reproduced directly on the GPU.
 import numpy as np
 import pytest
 
-from helpers import (circular_dam_state, dam1d_ic, ic_rows, inflow_bc,
+from helpers import (acoustic_ic, bump_lake_ic, circular_dam_state,
+                     dam1d_ic, ic_rows, inflow_bc,
                      lake_ic, max_rel_rows, pulse_ic, rel_l2, ritter_ic,
                      smooth_dirichlet_ic, stoker_ic, tilted_ic)
 
@@ -117,6 +118,25 @@ def test_reference_swashes_goldens(golden, name, ic, order):
     ref = golden(name)["Q"]
     err = np.abs(Q.cpu().numpy() - ref).max()
     assert err < (2e-9 if "ritter" in name else 1e-13)
+
+
+@pytest.mark.parametrize("name,n,L,t_end,ic,riemann", [
+    ("boundary_order2", 640, 1.0, 2.0 / np.sqrt(9.81), acoustic_ic,
+     "rusanov_nc"),
+    ("wb_long", 200, 10.0, 50.0, bump_lake_ic, "rusanov")])
+def test_reference_long_o2_goldens(golden, name, n, L, t_end, ic, riemann):
+    """Long order-2 marches between walls (acoustic standing wave, lake at
+    rest over a bump) by the HIP march, state and aux."""
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.line(n, 0.0, L).set_bc("wall")
+    model = ShallowWater(dimension=1, order=2, limiter="minmod",
+                         riemann=riemann, initial_conditions=ic)
+    s = HyperbolicSolver(time_end=t_end, cfl=0.9)
+    Q, Qaux = s.solve(mesh, model)
+    ref = golden(name)
+    assert np.abs(Q.cpu().numpy() - ref["Q"]).max() < 2e-14
+    assert np.abs(Qaux.cpu().numpy() - ref["Qaux"]).max() < 5e-12
 
 
 @pytest.mark.parametrize("recon,stage_aux,ftol", [

@@ -7,8 +7,9 @@ reference's tests/cases.py and tests/runtime/*.py.
 import numpy as np
 import pytest
 
-from helpers import (ic_rows, inflow_bc, lake_ic, pulse_ic, ritter_ic,
-                     smooth_dirichlet_ic, stoker_ic, tilted_ic)
+from helpers import (acoustic_ic, bump_lake_ic, ic_rows, inflow_bc, lake_ic,
+                     pulse_ic, ritter_ic, smooth_dirichlet_ic, stoker_ic,
+                     tilted_ic)
 from oracle.oracle import Oracle
 from zoomy_hip.mesh import Mesh
 
@@ -144,3 +145,26 @@ def test_swashes_goldens(golden, name, ic, order):
         assert (np.abs(Q[1] - ref["Q"][1])[wet] / ref["Q"][1][wet]).max() < 1e-6
     else:
         assert err < 1e-14
+
+
+@pytest.mark.parametrize("stage_aux", ["frozen", "refresh"])
+@pytest.mark.parametrize("name,n,L,t_end,ic,riemann", [
+    # test_boundary_order2.py: acoustic standing wave between walls, one
+    # period, 640 cells (cases.py:380-407)
+    ("boundary_order2", 640, 1.0, 2.0 / np.sqrt(9.81), acoustic_ic,
+     "rusanov_nc"),
+    # test_wb_long_march.py: lake at rest over a bump, t = 50, 200 cells,
+    # PositiveNonconservativeRusanov
+    ("wb_long", 200, 10.0, 50.0, bump_lake_ic, "rusanov"),
+])
+def test_long_o2_goldens(golden, name, n, L, t_end, ic, riemann, stage_aux):
+    """Long order-2 marches between walls (1423 and 1907 steps): state and
+    aux to round-off in both stage-aux modes (these flows are too slow for
+    the stage-2 aux choice to show)."""
+    m = Mesh.line(n, 0.0, L).set_bc("wall")
+    o = Oracle(m, order=2, limiter="minmod", riemann=riemann, cfl=0.9,
+               stage_aux=stage_aux)
+    Q, t, k = o.march(ic_rows(m, ic), t_end=t_end)
+    ref = golden(name)
+    assert np.abs(Q[:3] - ref["Q"]).max() < 2e-14
+    assert np.abs(o.aux(Q) - ref["Qaux"]).max() < 5e-12
