@@ -171,6 +171,32 @@ int zj_mass(zj_solver *s, const void *Q, double *mass_out, void *stream);
 int zj_imex_local(zj_solver *s, const double *dt, int32_t maxiter,
                   const void *Qexp, void *Qout, void *stream);
 
+/* ─────────────────── implicit diffusion (Crank-Nicolson) ───────────────────
+ * The scalar nu-Laplacian of DiffusionOperatorJAX (zoomy_jax/fvm/
+ * reconstruction_jax.py:1423-1520), which the IMEX solver applies per state
+ * row between its explicit hyperbolic stage and its implicit source
+ * (solver_imex_jax.py:396-460, 617-619): two-point fluxes
+ * nu |f| (u_b - u_a) / |x_b - x_a| / max(|n.e|, 0.1) on interior faces,
+ * none on boundary faces. */
+typedef struct zj_diffusion zj_diffusion;
+/* Face coefficients and volumes from the host mesh (explicit() of the
+ * reference's setup, :1437-1479). */
+int zj_diffusion_create(const zj_mesh_desc *desc, zj_diffusion **out);
+int zj_diffusion_destroy(zj_diffusion *d);
+/* Lu = L u, FP64 device arrays of n_cells (DiffusionOperatorJAX.explicit,
+ * :1481-1483). */
+int zj_diffusion_explicit(zj_diffusion *d, double nu, const double *u, double *Lu,
+                          void *stream);
+/* u_out solves (I - dt/2 L) u_out = (I + dt/2 L) u_star (implicit_solve,
+ * :1485-1520; the reference solves densely): Jacobi-preconditioned CG on the
+ * volume-weighted system from u_star, to a relative residual <= tol within
+ * maxiter iterations, else ZJ_ESOLVE.  Host-synchronous once per 8
+ * iterations; u_out must not alias u_star. */
+int zj_diffusion_cn(zj_diffusion *d, double nu, double dt, const double *u_star,
+                    double *u_out, double tol, int32_t maxiter, void *stream);
+/* CG iterations and relative residual of the last zj_diffusion_cn. */
+int zj_diffusion_stats(zj_diffusion *d, int32_t *iters, double *relres);
+
 /* ───────────────────────── Chorin projection ─────────────────────────────
  * Non-hydrostatic projection on an nx x ny wall-bounded quad grid whose
  * cells are numbered i*ny + j (Mesh.quad order).  State (5, N) rows

@@ -103,6 +103,17 @@ _SIGS = {
     "zj_solver_set_positivity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.c_double]),
     "zj_solver_set_stage_aux": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "zj_diffusion_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_diffusion_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zj_diffusion_explicit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                             ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "zj_diffusion_cn": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.c_int32, ctypes.c_void_p]),
+    "zj_diffusion_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]),
     "zj_solver_mood_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "zj_imex_local": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_int32, ctypes.c_void_p,
@@ -184,5 +195,5 @@ def check(rc, what=""):
         names = {ZJ_EINVAL: "invalid argument", ZJ_EHIP: "HIP runtime error",
                  ZJ_ENOMEM: "out of device memory",
                  ZJ_EUNSUP: "unsupported option",
-                 ZJ_ESOLVE: "pressure solve missed its accept gate"}
+                 ZJ_ESOLVE: "solve missed its tolerance or accept gate"}
         raise ZoomyHipError(f"{what}: {names.get(rc, rc)} (code {rc})")

@@ -1,9 +1,11 @@
 """Finite-volume time step and Chorin projection (host API over the HIP
 kernels)."""
 from .chorin import ChorinSplitVAMSolver, EllipticSolveError, elliptic_accept
+from .diffusion import DiffusionOperator
 from .solver import (DeviceMesh, HyperbolicSolver, IMEXSourceSolver,
                      ShallowWater)
 
-__all__ = ["ChorinSplitVAMSolver", "DeviceMesh", "EllipticSolveError",
+__all__ = ["ChorinSplitVAMSolver", "DeviceMesh", "DiffusionOperator",
+           "EllipticSolveError",
            "HyperbolicSolver", "IMEXSourceSolver", "ShallowWater",
            "elliptic_accept"]

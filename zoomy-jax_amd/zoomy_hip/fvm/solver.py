@@ -53,6 +53,10 @@ class ShallowWater:
     dt_max: float = 0.0
     initial_conditions: Optional[object] = None
     boundary_conditions: object = "extrapolation"
+    # {state row (index or name "h", "hu", "hv"): nu}: Crank-Nicolson
+    # diffusion of those rows in IMEXSourceSolver (the model's scalar
+    # diffusion operators, solver_imex_jax.py:396-460)
+    diffusion: Optional[dict] = None
 
     @property
     def n_state(self):
@@ -423,14 +427,19 @@ class HyperbolicSolver:
 
 
 class IMEXSourceSolver(HyperbolicSolver):
-    """Explicit hyperbolic step + local implicit source
-    (``zoomy_jax/fvm/solver_imex_jax.py``: ``implicit_maxiter = 6``, :142)."""
+    """Explicit hyperbolic step, Crank-Nicolson diffusion of the rows the
+    model lists in ``diffusion`` and the local implicit source, in the
+    reference's order (``zoomy_jax/fvm/solver_imex_jax.py:604-625``;
+    ``implicit_maxiter = 6``, :142)."""
 
     implicit_maxiter = 6
+    _ROWS = {"b": 0, "h": 1, "hu": 2, "q_0": 2, "hv": 3, "q_1": 3}
 
     def step(self, dt, time, Q, Qaux=None, parameters=None):
         Q4 = self._full(Q)
         Qe = self._hyperbolic_only(dt, Q4)   # explicit stage, no source
+        for row, op in self._diffusion:
+            Qe[row] = op.implicit_solve(Qe[row], float(dt))
         out = torch.empty_like(Q4)
         L.check(L.lib().zj_imex_local(self._rt.handle, _p(self._dt_dev(dt)),
                                       int(self.implicit_maxiter), _p(Qe),
@@ -446,6 +455,17 @@ class IMEXSourceSolver(HyperbolicSolver):
         m2.manning_n = 0.0
         self._rt_h = _Runtime(self._dmesh, m2, self.cfl, self.dtype,
                               self.reconstruction_variables)
+        self._diffusion = []
+        if model.diffusion:
+            if self.dtype != torch.float64:
+                raise ValueError("implicit diffusion runs in FP64")
+            from .diffusion import DiffusionOperator
+            for key, nu in model.diffusion.items():
+                row = self._ROWS[key] if isinstance(key, str) else int(key)
+                if not 0 <= row < model.n_state:
+                    raise ValueError(f"diffusion row {key!r} out of range")
+                self._diffusion.append(
+                    (row, DiffusionOperator(mesh, mesh.dim, float(nu))))
         return out
 
     def _hyperbolic_only(self, dt, Q4):
