@@ -260,6 +260,47 @@ def chorin_leg(precond, steps, warmup, net=None, n=CHORIN_N, L=CHORIN_L,
     return out
 
 
+DIF_BYTES = 152   # per unknown and CG iteration (see diffusion_leg)
+
+
+def diffusion_leg(mesh, Q0, reps=5):
+    """One Crank-Nicolson diffusion step of the depth field on the same
+    2048 x 2048 mesh (IMEX's implicit diffusion, DiffusionOperatorJAX), with
+    dt nu / dx^2 = 10: Jacobi-CG to a relative residual of 1e-12.
+    Algorithmic bytes per unknown and iteration: the direction update
+    fused into M p (z, p_old 16, slot tables 4 x (4 + 8) = 48, volume 8,
+    p_new and q written 16 = 88) and the update (p, q, diag, x, r read,
+    x, r, z written = 64): 152 B."""
+    import torch
+    from zoomy_hip.fvm import DiffusionOperator
+    dx = 1.0 / NX
+    dt = 1e-4
+    nu = 10.0 * dx * dx / dt
+    op = DiffusionOperator(mesh, 2, nu=nu)
+    u = torch.as_tensor(Q0[1], dtype=torch.float64, device="cuda")
+    op.implicit_solve(u, dt)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        out = op.implicit_solve(u, dt)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    it = op.stats()["iterations"]
+    gbs = DIF_BYTES * mesh.n_cells * it / (ms * 1e-3) / 1e9
+    return {"workload": "one CN step of h on the 2048x2048 dam-break mesh, "
+                        "dt nu / dx^2 = 10, CG to 1e-12",
+            "ms_per_solve": ms, "iterations": it,
+            "us_per_iteration": 1e3 * ms / max(it, 1),
+            "bytes_per_unknown_iteration": DIF_BYTES,
+            "achieved_gbs": gbs, "frac_of_hbm_peak": gbs * 1e9 / PEAK_HBM,
+            "relres": op.stats()["relres"],
+            "finite": bool(torch.isfinite(out).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +312,7 @@ def main():
     ap.add_argument("--chorin-steps", type=int, default=10)
     ap.add_argument("--no-chorin", action="store_true")
     ap.add_argument("--chorin-only", action="store_true")
+    ap.add_argument("--no-diffusion", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (10 times): "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -370,6 +412,8 @@ def main():
         out["checks"]["gpu_vs_oracle_rel_l2"] = cpu["gpu_vs_oracle"]
         out["checks"]["gpu_vs_oracle_sample"] = (
             f"256x256 sample of the same workload, {cpu['steps']} steps")
+    if not args.no_diffusion:
+        out["diffusion"] = diffusion_leg(mesh, Q0)
     if r32 is not None:
         cups32, roof32 = summarize(r32, 4)
         out["fp32"] = {"value": cups32,

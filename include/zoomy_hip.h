@@ -190,8 +190,9 @@ int zj_diffusion_explicit(zj_diffusion *d, double nu, const double *u, double *L
 /* u_out solves (I - dt/2 L) u_out = (I + dt/2 L) u_star (implicit_solve,
  * :1485-1520; the reference solves densely): Jacobi-preconditioned CG on the
  * volume-weighted system from u_star, to a relative residual <= tol within
- * maxiter iterations, else ZJ_ESOLVE.  Host-synchronous once per 8
- * iterations; u_out must not alias u_star. */
+ * maxiter iterations, else ZJ_ESOLVE.  The host reads the convergence flag
+ * of each chunk of 8 iterations while the next chunk runs, and returns with
+ * the stream idle; u_out must not alias u_star. */
 int zj_diffusion_cn(zj_diffusion *d, double nu, double dt, const double *u_star,
                     double *u_out, double tol, int32_t maxiter, void *stream);
 /* CG iterations and relative residual of the last zj_diffusion_cn. */

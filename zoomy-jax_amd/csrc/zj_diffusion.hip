@@ -12,11 +12,14 @@
 //
 // Layout: the mesh's slot tables (mf, N) -- neighbour ids and the face
 // coefficients g (0 on boundary slots) -- and per-cell volumes, all FP64.
-// Every CG pass is one HBM sweep over those tables and 3-5 vectors; the
-// scalars (alpha, beta, residuals) stay on the device, and the host reads a
-// convergence flag once per chunk of iterations.
+// A CG iteration is two grid-stride passes, each followed by a single-block
+// fold of its dot products: the direction update fused into M p (p = z +
+// beta p_old recomputed for the neighbours, double-buffered), then the x, r,
+// z update.  The scalars (alpha, beta, residuals) stay on the device, and
+// the host reads a convergence flag once per chunk of iterations.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <new>
 #include <vector>
@@ -28,6 +31,7 @@ namespace {
 constexpr int DB = 256;        // threads per block (4 wave64)
 constexpr int DFIN = 1024;     // threads of the single-block folds
 constexpr int DCHUNK = 8;      // CG iterations per host check
+constexpr int DMAXB = 2048;    // grid cap of the CG passes (grid-stride)
 
 #define DCHK(x)                                    \
     do {                                           \
@@ -63,7 +67,7 @@ __device__ __forceinline__ double gsum(const DifView& d, const double* __restric
     return acc;
 }
 
-// block sum of two values into part[2 * block]
+// block sums of (a, b) into part[2 * block]
 __device__ __forceinline__ void block_sum2(double a, double b, double* __restrict__ part) {
     __shared__ double la[DB / 64], lb[DB / 64];
     for (int off = 32; off > 0; off >>= 1) {
@@ -81,8 +85,11 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* __restric
     }
 }
 
-// fold nb block pairs; what = 0: (rz, bb) of the start, 1: pq -> alpha,
-// 2: (rz_new, rr) -> beta, convergence
+// Single-block fold of nb block pairs (fixed order: deterministic), then
+// `what`: 0 start (rz, bb), 1 alpha = rz / p.q, 2 beta and convergence.
+// (A fold in the last block of the producing kernel needs a device-scope
+// fence per block, which costs an L2 write-back on every XCD: a separate
+// launch is far cheaper.)
 __global__ __launch_bounds__(DFIN) void k_dif_fin(const double* __restrict__ part, int nb,
                                                   DifScal* __restrict__ s, int what) {
     if (s->done && what != 0) return;
@@ -106,7 +113,8 @@ __global__ __launch_bounds__(DFIN) void k_dif_fin(const double* __restrict__ par
     if (what == 0) {
         s->rz = a;
         s->bb = b;
-        s->rr = a;  // placeholder until the first update
+        s->rr = a;
+        s->beta = 0.0;
         s->done = (a == 0.0);  // u* already solves the system (G u* = 0)
         s->iters = 0;
     } else if (what == 1) {
@@ -121,14 +129,14 @@ __global__ __launch_bounds__(DFIN) void k_dif_fin(const double* __restrict__ par
     }
 }
 
-// x = u*, r = dt nu G u*, z = r / diag, p = z; partials (r.z, b.b)
+// x = u*, r = dt nu G u*, z = r / diag, p_old = z (beta = 0 for the first
+// direction); partials (r.z, b.b).  Grid-stride over the cells.
 __global__ __launch_bounds__(DB) void k_dif_init(DifView d, double c_half, const double* __restrict__ u,
                                                  double* __restrict__ x, double* __restrict__ r,
-                                                 double* __restrict__ z, double* __restrict__ p,
+                                                 double* __restrict__ z, double* __restrict__ po,
                                                  double* __restrict__ diag, double* __restrict__ part) {
-    const int c = blockIdx.x * DB + threadIdx.x;
     double rz = 0.0, bb = 0.0;
-    if (c < d.N) {
+    for (int c = blockIdx.x * DB + threadIdx.x; c < d.N; c += gridDim.x * DB) {
         double gs;
         const double gu = gsum(d, u, c, gs);
         const double vc = d.vol[c];
@@ -139,27 +147,34 @@ __global__ __launch_bounds__(DB) void k_dif_init(DifView d, double c_half, const
         x[c] = u[c];
         r[c] = rc;
         z[c] = zc;
-        p[c] = zc;
+        po[c] = zc;
         diag[c] = dg;
-        rz = rc * zc;
-        bb = bc * bc;
+        rz += rc * zc;
+        bb += bc * bc;
     }
     block_sum2(rz, bb, part);
 }
 
-// q = M p = V p - c_half G p; partials (p.q, 0)
-__global__ __launch_bounds__(DB) void k_dif_mp(DifView d, double c_half, const double* __restrict__ p,
+// the new direction p = z + beta p_old (own cell stored to pn, neighbours
+// recomputed), q = M p = V p - c_half G p; partials p.q
+__global__ __launch_bounds__(DB) void k_dif_mp(DifView d, double c_half, const double* __restrict__ z,
+                                               const double* __restrict__ po, double* __restrict__ pn,
                                                double* __restrict__ q, double* __restrict__ part,
                                                const DifScal* __restrict__ s) {
     if (s->done) return;
-    const int c = blockIdx.x * DB + threadIdx.x;
+    const double beta = s->beta;
     double pq = 0.0;
-    if (c < d.N) {
-        double gs;
-        const double gp = gsum(d, p, c, gs);
-        const double qc = d.vol[c] * p[c] - c_half * gp;
+    for (int c = blockIdx.x * DB + threadIdx.x; c < d.N; c += gridDim.x * DB) {
+        const double pc = z[c] + beta * po[c];
+        double acc = 0.0;
+        for (int k = 0; k < d.mf; ++k) {
+            const int nb = d.nbr[(size_t)k * d.N + c];
+            if (nb >= 0) acc += d.g[(size_t)k * d.N + c] * ((z[nb] + beta * po[nb]) - pc);
+        }
+        const double qc = d.vol[c] * pc - c_half * acc;
+        pn[c] = pc;
         q[c] = qc;
-        pq = p[c] * qc;
+        pq += pc * qc;
     }
     block_sum2(pq, 0.0, part);
 }
@@ -171,36 +186,27 @@ __global__ __launch_bounds__(DB) void k_dif_upd(int N, const double* __restrict_
                                                 double* __restrict__ r, double* __restrict__ z,
                                                 double* __restrict__ part, const DifScal* __restrict__ s) {
     if (s->done) return;
-    const int c = blockIdx.x * DB + threadIdx.x;
+    const double al = s->alpha;
     double rz = 0.0, rr = 0.0;
-    if (c < N) {
-        const double al = s->alpha;
+    for (int c = blockIdx.x * DB + threadIdx.x; c < N; c += gridDim.x * DB) {
         x[c] += al * p[c];
         const double rc = r[c] - al * q[c];
         const double zc = rc / diag[c];
         r[c] = rc;
         z[c] = zc;
-        rz = rc * zc;
-        rr = rc * rc;
+        rz += rc * zc;
+        rr += rc * rc;
     }
     block_sum2(rz, rr, part);
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(DB) void k_dif_p(int N, const double* __restrict__ z, double* __restrict__ p,
-                                              const DifScal* __restrict__ s) {
-    if (s->done) return;
-    const int c = blockIdx.x * DB + threadIdx.x;
-    if (c < N) p[c] = z[c] + s->beta * p[c];
 }
 
 // (L u)_c = nu / |A_c| (G u)_c
 __global__ __launch_bounds__(DB) void k_dif_explicit(DifView d, double nu, const double* __restrict__ u,
                                                      double* __restrict__ out) {
-    const int c = blockIdx.x * DB + threadIdx.x;
-    if (c >= d.N) return;
-    double gs;
-    out[c] = nu * gsum(d, u, c, gs) / d.vol[c];
+    for (int c = blockIdx.x * DB + threadIdx.x; c < d.N; c += gridDim.x * DB) {
+        double gs;
+        out[c] = nu * gsum(d, u, c, gs) / d.vol[c];
+    }
 }
 
 }  // namespace
@@ -209,9 +215,11 @@ struct zj_diffusion {
     int N = 0, mf = 0, nblk = 0;
     int* nbr = nullptr;
     double *g = nullptr, *vol = nullptr;
-    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *diag = nullptr;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *diag = nullptr;
     double* part = nullptr;
     DifScal* s = nullptr;
+    DifScal* hs = nullptr;  // 2 pinned host slots of the convergence scalars
+    hipEvent_t ev[2] = {nullptr, nullptr};
     int last_iters = 0;
     double last_relres = 0.0;
     DifView view() const { return DifView{N, mf, nbr, g, vol}; }
@@ -221,9 +229,13 @@ extern "C" {
 
 int zj_diffusion_destroy(zj_diffusion* d) {
     if (!d) return ZJ_OK;
-    void* ptrs[] = {d->nbr, d->g, d->vol, d->r, d->z, d->p, d->q, d->diag, d->part, d->s};
+    void* ptrs[] = {d->nbr, d->g, d->vol, d->r,    d->z,    d->p,
+                    d->p2,  d->q, d->diag, d->part, d->s};
     for (void* x : ptrs)
         if (x) (void)hipFree(x);
+    if (d->hs) (void)hipHostFree(d->hs);
+    for (hipEvent_t e : d->ev)
+        if (e) (void)hipEventDestroy(e);
     delete d;
     return ZJ_OK;
 }
@@ -283,15 +295,19 @@ int zj_diffusion_create(const zj_mesh_desc* m, zj_diffusion** out) {
     if (!d) return ZJ_ENOMEM;
     d->N = N;
     d->mf = mf;
-    d->nblk = (N + DB - 1) / DB;
+    d->nblk = std::min((N + DB - 1) / DB, DMAXB);
     const size_t vb = sizeof(double) * N;
     bool ok = hipMalloc((void**)&d->nbr, sizeof(int) * nbr.size()) == hipSuccess &&
               hipMalloc((void**)&d->g, sizeof(double) * g.size()) == hipSuccess &&
               hipMalloc((void**)&d->vol, vb) == hipSuccess && hipMalloc((void**)&d->r, vb) == hipSuccess &&
               hipMalloc((void**)&d->z, vb) == hipSuccess && hipMalloc((void**)&d->p, vb) == hipSuccess &&
               hipMalloc((void**)&d->q, vb) == hipSuccess && hipMalloc((void**)&d->diag, vb) == hipSuccess &&
+              hipMalloc((void**)&d->p2, vb) == hipSuccess &&
               hipMalloc((void**)&d->part, sizeof(double) * 2 * d->nblk) == hipSuccess &&
-              hipMalloc((void**)&d->s, sizeof(DifScal)) == hipSuccess;
+              hipMalloc((void**)&d->s, sizeof(DifScal)) == hipSuccess &&
+              hipHostMalloc((void**)&d->hs, 2 * sizeof(DifScal), hipHostMallocDefault) == hipSuccess &&
+              hipEventCreateWithFlags(&d->ev[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&d->ev[1], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         zj_diffusion_destroy(d);
         return ZJ_ENOMEM;
@@ -329,26 +345,42 @@ int zj_diffusion_cn(zj_diffusion* d, double nu, double dt, const double* u_star,
     hipLaunchKernelGGL(k_dif_init, dim3(d->nblk), dim3(DB), 0, st, v, c_half, u_star, u_out, d->r, d->z,
                        d->p, d->diag, d->part);
     hipLaunchKernelGGL(k_dif_fin, dim3(1), dim3(DFIN), 0, st, (const double*)d->part, d->nblk, d->s, 0);
-    DifScal h{};
-    for (;;) {
+    // p_old / p_new alternate between the two direction buffers
+    double* pb[2] = {d->p, d->p2};
+    int cur = 0;
+    auto chunk = [&]() {
         for (int i = 0; i < DCHUNK; ++i) {
-            hipLaunchKernelGGL(k_dif_mp, dim3(d->nblk), dim3(DB), 0, st, v, c_half, (const double*)d->p,
-                               d->q, d->part, (const DifScal*)d->s);
+            hipLaunchKernelGGL(k_dif_mp, dim3(d->nblk), dim3(DB), 0, st, v, c_half, (const double*)d->z,
+                               (const double*)pb[cur], pb[cur ^ 1], d->q, d->part,
+                               (const DifScal*)d->s);
             hipLaunchKernelGGL(k_dif_fin, dim3(1), dim3(DFIN), 0, st, (const double*)d->part, d->nblk,
                                d->s, 1);
-            hipLaunchKernelGGL(k_dif_upd, dim3(d->nblk), dim3(DB), 0, st, d->N, (const double*)d->p,
-                               (const double*)d->q, (const double*)d->diag, u_out, d->r, d->z, d->part,
-                               (const DifScal*)d->s);
+            hipLaunchKernelGGL(k_dif_upd, dim3(d->nblk), dim3(DB), 0, st, d->N,
+                               (const double*)pb[cur ^ 1], (const double*)d->q, (const double*)d->diag,
+                               u_out, d->r, d->z, d->part, (const DifScal*)d->s);
             hipLaunchKernelGGL(k_dif_fin, dim3(1), dim3(DFIN), 0, st, (const double*)d->part, d->nblk,
                                d->s, 2);
-            hipLaunchKernelGGL(k_dif_p, dim3(d->nblk), dim3(DB), 0, st, d->N, (const double*)d->z, d->p,
-                               (const DifScal*)d->s);
+            cur ^= 1;
         }
+    };
+    // the host reads chunk k's flag while chunk k + 1 runs (the passes after
+    // convergence return at once), so the device never waits for the host
+    chunk();
+    DCHK(hipGetLastError());
+    DCHK(hipMemcpyAsync(&d->hs[0], d->s, sizeof(DifScal), hipMemcpyDeviceToHost, st));
+    DCHK(hipEventRecord(d->ev[0], st));
+    DifScal h{};
+    for (int k = 0;; ++k) {
+        const int a = k & 1, b = a ^ 1;
+        chunk();
         DCHK(hipGetLastError());
-        DCHK(hipMemcpyAsync(&h, d->s, sizeof(DifScal), hipMemcpyDeviceToHost, st));
-        DCHK(hipStreamSynchronize(st));
+        DCHK(hipMemcpyAsync(&d->hs[b], d->s, sizeof(DifScal), hipMemcpyDeviceToHost, st));
+        DCHK(hipEventRecord(d->ev[b], st));
+        DCHK(hipEventSynchronize(d->ev[a]));
+        h = d->hs[a];
         if (h.done) break;
     }
+    DCHK(hipStreamSynchronize(st));
     d->last_iters = h.iters;
     d->last_relres = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
     if (h.bb > 0.0 && h.rr > s0.tol2 * h.bb) return ZJ_ESOLVE;
