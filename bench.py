@@ -352,16 +352,28 @@ def main():
         chorin["learned"] = chorin_leg("learned", args.chorin_steps, 2,
                                        net=MPPreconditioner.load(),
                                        flux_dtype=torch.float32)
+        # the learned V-cycle once per solve, as the initial guess of the
+        # FP64 V-cycle CG (retried from zero if it misses the tolerance),
+        # with the FP32 predictor fluxes
+        chorin["learned_guess"] = chorin_leg("learned_guess",
+                                             args.chorin_steps, 2,
+                                             net=MPPreconditioner.load(),
+                                             flux_dtype=torch.float32)
         chorin["learned_over_plain"] = (chorin["learned"]["ms_per_step"] /
                                         chorin["plain"]["ms_per_step"])
+        chorin["learned_guess_over_plain"] = (
+            chorin["learned_guess"]["ms_per_step"] /
+            chorin["plain"]["ms_per_step"])
         chorin["mixed_over_plain"] = (chorin["mixed"]["ms_per_step"] /
                                       chorin["plain"]["ms_per_step"])
         chorin["mixed"].pop("Q")
         # the learned, mixed-precision march against the plain FP64 one
         # after the same steps (pressure to 1e-8 in both, FP32 fluxes)
+        Qplain = chorin["plain"].pop("Q")
         chorin["learned_vs_plain_rel_l2"] = rel_l2_rows(
-            chorin["learned"].pop("Q"), chorin["plain"].pop("Q"),
-            rows=(1, 2, 3, 4))
+            chorin["learned"].pop("Q"), Qplain, rows=(1, 2, 3, 4))
+        chorin["learned_guess_vs_plain_rel_l2"] = rel_l2_rows(
+            chorin["learned_guess"].pop("Q"), Qplain, rows=(1, 2, 3, 4))
         if args.chorin_only:
             print(json.dumps({"chorin": chorin}))
             return

@@ -210,8 +210,12 @@ int zj_diffusion_stats(zj_diffusion *d, int32_t *iters, double *relres);
  * (mixed precision, flexible CG); the FP32 V-cycle plus one BF16
  * message-passing correction (learned, flexible CG).  The low-precision
  * forms are checked by a fresh FP64 residual and redone plainly when they
- * miss tol. */
-enum { ZJ_PRECOND_MG = 0, ZJ_PRECOND_LEARNED = 1, ZJ_PRECOND_MIXED = 2 };
+ * miss tol.  ZJ_PRECOND_LEARNED_GUESS applies the learned V-cycle once, to
+ * the right-hand side, as the initial guess of the FP64 V-cycle CG, and
+ * retries from zero when that solve misses tol (the learned initial guess
+ * of gnn_blueprint/imex_child_solver.py:253-283). */
+enum { ZJ_PRECOND_MG = 0, ZJ_PRECOND_LEARNED = 1, ZJ_PRECOND_MIXED = 2,
+       ZJ_PRECOND_LEARNED_GUESS = 3 };
 typedef struct zj_chorin zj_chorin;
 typedef struct zj_mpnn zj_mpnn;     /* learned preconditioner (zj_mpnn_*) */
 typedef struct {
@@ -249,7 +253,8 @@ int zj_chorin_set_fv32(zj_chorin *c, zj_solver *fv32);
 /* Fused multigrid kernels on/off (default on; used when nu == 1).  Both
  * give bit-identical V-cycles.  No counterpart in the reference. */
 int zj_chorin_set_mg_fused(zj_chorin *c, int32_t on);
-/* Select the preconditioner; net must be non-NULL for ZJ_PRECOND_LEARNED. */
+/* Select the preconditioner; net must be non-NULL for ZJ_PRECOND_LEARNED
+ * and ZJ_PRECOND_LEARNED_GUESS. */
 int zj_chorin_set_precond(zj_chorin *c, int32_t precond, zj_mpnn *net);
 /* chorin_cycle (solver_chorin_vam_jax.py:574-615): one step of time_order
  * stages; returns ZJ_ESOLVE when a pressure solve misses the accept gate

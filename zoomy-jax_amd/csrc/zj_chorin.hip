@@ -1155,7 +1155,7 @@ struct zj_chorin {
     // iterations of the last solve per preconditioner (ZJ_PRECOND_*): the
     // size of the first chunk of iterations enqueued before the host reads
     // the convergence flag
-    int last_iters[3] = {8, 8, 8};
+    int last_iters[4] = {8, 8, 8, 8};
     zj_chorin_stats stats{};
     std::vector<void*> allocs;
 };
@@ -1189,7 +1189,7 @@ static int read_flags(zj_chorin* c, hipStream_t st) {
 // return an FP32 correction.  x holds the initial guess when x0 != 0.
 // Host-synchronous once per chunk of iterations.
 static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, int maxit,
-               int pmode, hipStream_t st, int* iters, double* relres) {
+               int pmode, hipStream_t st, int* iters, double* relres, int slot = -1) {
     zj::Hier& H = c->H64;
     zj::HLevel& L = H.levels[0];
     const int n = L.nx * L.ny;
@@ -1230,7 +1230,7 @@ static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, i
     k_xpby<double><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
     HIPCHK(hipGetLastError());
     int launched = 0;
-    int& last = c->last_iters[pmode];
+    int& last = c->last_iters[slot < 0 ? pmode : slot];
     int chunk = std::max(1, std::min(maxit, last));
     while (true) {
         for (int k = 0; k < chunk; ++k) {
@@ -1291,6 +1291,30 @@ static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStr
     int its = 0;
     double rel = 0.0;
     S.solves += 1;
+    if (precond == ZJ_PRECOND_LEARNED_GUESS && c->net) {
+        // the learned model as the initial guess of the FP64 MG-CG, with a
+        // retry from zero when the solve misses tol (the reference's learned
+        // guess, gnn_blueprint/imex_child_solver.py:253-283): x0 = the
+        // learned V-cycle of the right-hand side, one network application
+        const int n = c->N;
+        k_cast<float><<<nblk(n), CB, 0, st>>>(n, rhs, c->r32);
+        if (int rc = zj::learned_vcycle(c->H32, c->net, c->r32, c->z32, c->prm.nu,
+                                        c->prm.coarse_sweeps, nullptr, 0, st))
+            return rc;
+        k_widen<<<nblk(n), CB, 0, st>>>(n, c->z32, p);
+        int rc = pcg(c, rhs, p, 1, tol, c->prm.maxiter, ZJ_PRECOND_MG, st, &its, &rel,
+                     ZJ_PRECOND_LEARNED_GUESS);
+        if (rc && rc != ZJ_ESOLVE) return rc;
+        S.lp_iterations += its;
+        S.lp_solves += 1;
+        if (!(rel <= S.lp_relres)) S.lp_relres = rel;
+        if (!rc && rel <= tol) {
+            S.iterations += its;
+            S.relres = std::max(S.relres, rel);
+            return ZJ_OK;
+        }
+        S.fallbacks += 1;  // retry from zero
+    }
     if (precond == ZJ_PRECOND_MIXED || (precond == ZJ_PRECOND_LEARNED && c->net)) {
         int rc = pcg(c, rhs, p, 0, tol, c->prm.maxiter, precond, st, &its, &rel);
         if (rc && rc != ZJ_ESOLVE) return rc;
@@ -1366,7 +1390,7 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
     if (!out || nx < 2 || ny < 2 || !(dx > 0) || !(dy > 0) || !prm) return ZJ_EINVAL;
     if (prm->tol <= 0 || prm->maxiter < 1 || prm->nu < 1 || prm->coarse_sweeps < 1 ||
         prm->min_coarse < 1 || (prm->time_order != 1 && prm->time_order != 2) ||
-        prm->precond < ZJ_PRECOND_MG || prm->precond > ZJ_PRECOND_MIXED)
+        prm->precond < ZJ_PRECOND_MG || prm->precond > ZJ_PRECOND_LEARNED_GUESS)
         return ZJ_EINVAL;
     zj_chorin* c = new (std::nothrow) zj_chorin();
     if (!c) return ZJ_ENOMEM;
@@ -1462,10 +1486,9 @@ int zj_chorin_set_mg_fused(zj_chorin* c, int32_t on) {
 }
 
 int zj_chorin_set_precond(zj_chorin* c, int32_t precond, zj_mpnn* net) {
-    if (!c || (precond != ZJ_PRECOND_MG && precond != ZJ_PRECOND_LEARNED &&
-               precond != ZJ_PRECOND_MIXED))
+    if (!c || precond < ZJ_PRECOND_MG || precond > ZJ_PRECOND_LEARNED_GUESS) return ZJ_EINVAL;
+    if ((precond == ZJ_PRECOND_LEARNED || precond == ZJ_PRECOND_LEARNED_GUESS) && !net)
         return ZJ_EINVAL;
-    if (precond == ZJ_PRECOND_LEARNED && !net) return ZJ_EINVAL;
     c->prm.precond = precond;
     c->net = net;
     return ZJ_OK;

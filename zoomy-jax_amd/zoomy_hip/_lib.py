@@ -12,7 +12,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libzoomy_hip.so"
 
 ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP, ZJ_ESOLVE = 0, 1, 2, 3, 4, 5
-PRECOND = {"mg": 0, "learned": 1, "mixed": 2}
+PRECOND = {"mg": 0, "learned": 1, "mixed": 2, "learned_guess": 3}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
 RIEMANN = {"rusanov_nc": 0, "rusanov": 1, "hll": 2}

@@ -145,14 +145,17 @@ class ChorinSplitVAMSolver:
 
     def set_preconditioner(self, precond: str, net=None):
         """``"mg"`` (FP64 V-cycle), ``"mixed"`` (FP32 V-cycle inside FP64
-        flexible CG) or ``"learned"`` (FP32 V-cycle plus the BF16 network
+        flexible CG), ``"learned"`` (FP32 V-cycle plus the BF16 network
         correction of the :class:`~zoomy_hip.learned.MPPreconditioner`
-        ``net``, inside FP64 flexible CG).  A low-precision solve whose fresh
-        FP64 residual misses ``tol`` is redone with ``"mg"``."""
+        ``net``, inside FP64 flexible CG) or ``"learned_guess"`` (that
+        learned V-cycle applied once to the right-hand side as the initial
+        guess of the FP64 V-cycle CG, retried from zero when it misses
+        ``tol``).  A low-precision solve whose fresh FP64 residual misses
+        ``tol`` is redone with ``"mg"``."""
         if precond not in L.PRECOND:
             raise ValueError(f"precond must be one of {sorted(L.PRECOND)}")
         nh = None
-        if precond == "learned":
+        if precond in ("learned", "learned_guess"):
             if net is None:
                 raise ValueError("the learned preconditioner needs a network")
             nh = net.handle
