@@ -301,6 +301,99 @@ def diffusion_leg(mesh, Q0, reps=5):
             "finite": bool(torch.isfinite(out).all())}
 
 
+def wetdry_tri_bed(xc, seed=3):
+    """Seeded fractal bed on [0, 10] x [0, 5]: four octaves of random-phase
+    plane waves (amplitude halving, frequency doubling), scaled to a
+    maximum of 1.5, plus a linear valley slope down toward x = 10 and the
+    centre line y = 2.5."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    x, y = xc[:, 0], xc[:, 1]
+    f = np.zeros(len(xc))
+    for o in range(4):
+        for _ in range(6):
+            th = rng.uniform(0.0, np.pi)
+            k = 2.0 ** o * 0.6 * np.array([np.cos(th), np.sin(th)])
+            f += 0.5 ** o * np.cos(k[0] * x + k[1] * y + rng.uniform(0, 6.283))
+    f = 1.5 * (f - f.min()) / (f.max() - f.min())
+    return f + 0.08 * (10.0 - x) + 0.1 * np.abs(y - 2.5)
+
+
+def wetdry_tri_leg(steps, warmup):
+    """The wet/dry triangle configuration: Delaunay of 2000 x 1000 jittered
+    points on [0, 10] x [0, 5] (~4M triangles), the bed above, reservoir
+    h = 3 in x < 2 and dry elsewhere, walls; order 2 with van Leer on the
+    eta reconstruction (hydrostatic face clip), HLL, the a-priori front
+    pre-detector (theta < 0.5: first order) and MOOD; FP32 state and
+    fluxes, the mass summed in FP64 (compensated).  (MOOD alone does not
+    keep this case positive: at CFL 0.9 the dt law 2 r / (d a) exceeds the
+    first-order positivity bound |A| / (P a) of a triangle 1.8-fold, so the
+    first-order redo itself goes negative.)  Run in the RCM cell order the configuration names and
+    in the Morton order of the cell centres (the order the patch kernel is
+    laid out for: RCM's level sets are long thin strips, so a patch of
+    consecutive cells drags in 8-12x its own size of ring cells, against
+    2.5x for Morton).  Returns cell-updates/s and the checks per order."""
+    import numpy as np
+    import torch
+    from zoomy_hip import _lib
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.mesh import delaunay_mesh, jittered_points
+    t_setup = time.perf_counter()
+    pts = jittered_points(2000, 1000, (0.0, 10.0, 0.0, 5.0), 0.3, 11)
+    base = delaunay_mesh(pts, (0.0, 10.0, 0.0, 5.0))
+    ip, ix = base.adjacency_csr()
+    rcm = np.zeros(base.n_cells, np.int32)
+    _lib.check(_lib.lib().zj_rcm(base.n_cells, ip.ctypes.data,
+                                 ix.ctypes.data, rcm.ctypes.data), "zj_rcm")
+    delaunay_s = time.perf_counter() - t_setup
+    out = {"workload": "Delaunay of 2000x1000 jittered points on [0,10]x"
+                       "[0,5], fractal bed + valley slope, reservoir h=3 in "
+                       "x<2, dry elsewhere, walls; van Leer on the eta "
+                       "reconstruction, HLL, front pre-detector 0.5 + MOOD, "
+                       "FP32 state, FP64 mass",
+           "cells": base.n_cells, "steps": steps,
+           "delaunay_and_rcm_s": delaunay_s}
+    for name, perm in (("rcm", rcm), ("morton", None)):
+        t1 = time.perf_counter()
+        mesh = base.reorder(base.morton_permutation() if perm is None
+                            else perm).set_bc("wall")
+        xc = mesh.cell_centers()
+        Q0 = np.zeros((4, mesh.n_cells))
+        Q0[0] = wetdry_tri_bed(xc)
+        Q0[1] = np.where(xc[:, 0] < 2.0, 3.0, 0.0)
+        model = ShallowWater(dimension=2, order=2, limiter="vanleer",
+                             riemann="hll", boundary_conditions="wall")
+        s = HyperbolicSolver(time_end=float("inf"), cfl=0.9,
+                             dtype=torch.float32,
+                             reconstruction_variables="eta", mood=True,
+                             front_theta_tol=0.5)
+        s.setup_simulation(mesh, model)
+        setup_s = time.perf_counter() - t1
+        Q = torch.as_tensor(Q0, dtype=torch.float32,
+                            device="cuda").contiguous()
+        m0 = float(s.compute_mass(Q))
+        s.march_device(Q, warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, t_dev, _ = s.march_device(Q, steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        m1 = float(s.compute_mass(Q))
+        redone, demoted = s.mood_stats()
+        out[name] = {
+            "value": mesh.n_cells * steps / el, "unit": "cell-updates/s",
+            "ms_per_step": el * 1e3 / steps, "setup_s": setup_s,
+            "checks": {"sim_time": float(t_dev),
+                       "mass_rel_drift": abs(m1 - m0) / m0,
+                       "min_h": float(Q[1].min()),
+                       "finite": bool(torch.isfinite(Q).all()),
+                       "mood_steps_redone": redone,
+                       "mood_cells_demoted": demoted}}
+        print(json.dumps({"wetdry_tri_" + name: out[name]}), flush=True)
+        del s, Q
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +406,9 @@ def main():
     ap.add_argument("--no-chorin", action="store_true")
     ap.add_argument("--chorin-only", action="store_true")
     ap.add_argument("--no-diffusion", action="store_true")
+    ap.add_argument("--wetdry-tri", action="store_true",
+                    help="only the wet/dry triangle configuration (~4M "
+                         "triangles; about a minute of host-side mesh set-up)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 1 GiB (10 times): "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters")
@@ -335,6 +431,10 @@ def main():
     import torch
     from zoomy_hip.mesh import Mesh
 
+    if args.wetdry_tri:
+        print(json.dumps({"wetdry_tri": wetdry_tri_leg(args.steps,
+                                                       args.warmup)}))
+        return
     chorin = None
     if not args.no_chorin:
         chorin = {"workload": f"{CHORIN_N}x{CHORIN_N} solitary wave "
