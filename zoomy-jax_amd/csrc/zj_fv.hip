@@ -317,7 +317,8 @@ __global__ __launch_bounds__(1024) void k_reduce_part(const double* __restrict__
     if (gate && *gate == 0) return;
     __shared__ double lds[1024 / 64];
     double v = INFINITY;
-    for (int i = threadIdx.x; i < n; i += 1024) v = fmin(v, part[i]);
+    for (int i = blockIdx.x * 1024 + threadIdx.x; i < n; i += gridDim.x * 1024)
+        v = fmin(v, part[i]);
     auto mn = [](double a, double b) { return fmin(a, b); };
     v = block_reduce<double, decltype(mn), 1024>(v, mn, lds);
     if (threadIdx.x == 0) atomicMin(dt_red, (unsigned long long)__double_as_longlong(v));
@@ -1221,6 +1222,11 @@ int zj_solver_destroy(zj_solver* s) {
 
 namespace {
 
+// blocks of the dt fold: one per 4096 partials, at most 64 (each folds its
+// share and takes the minimum with one atomicMin -- order-independent, so
+// the result does not depend on the block count)
+static inline int reduce_blocks(int n) { return std::max(1, std::min(64, (n + 4095) / 4096)); }
+
 template <typename T>
 void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double* dt, T* aux,
                   int mode, int final_stage, int write_b, int use_slopes, int* troubled,
@@ -1238,7 +1244,8 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
                        o1_mask, bad_out, gate);
     if (dt_red)
-        hipLaunchKernelGGL(k_reduce_part, dim3(1), dim3(1024), 0, st, (const double*)s->dt_part,
+        hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(M->pt.npatch * (SBLOCK / 64))),
+                           dim3(1024), 0, st, (const double*)s->dt_part,
                            M->pt.npatch * (SBLOCK / 64), dt_red, gate);
 }
 
