@@ -310,6 +310,18 @@ __global__ void k_dt_finalize(unsigned long long* red, double* dt, const double*
 
 __global__ void k_t_advance(double* t, const double* dt) { *t += *dt; }
 
+// The march's per-step head: t += the previous step's dt (advance != 0),
+// then dt of this step as k_dt_finalize (one launch instead of two).
+__global__ void k_dt_next(unsigned long long* red, double* dt, double* t, double t_end,
+                          double dt_max, int advance) {
+    if (advance) *t += *dt;
+    double d = __longlong_as_double((long long)*red);
+    if (dt_max > 0.0) d = fmin(d, dt_max);
+    if (t_end >= 0.0) d = fmin(d, t_end - *t);
+    *dt = d;
+    *red = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
 // Folds the per-wave dt minima of a stage launch into the reduction scalar.
 __global__ __launch_bounds__(1024) void k_reduce_part(const double* __restrict__ part, int n,
                                                       unsigned long long* __restrict__ dt_red,
@@ -1251,11 +1263,15 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
 
 template <typename T>
 int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled,
-              unsigned long long* dt_red, hipStream_t st, T* aux_of_Q = nullptr) {
+              unsigned long long* dt_red, hipStream_t st, T* aux_of_Q = nullptr,
+              bool copy_b = true) {
     const DevMesh& m = s->m->d;
     T* Q1 = (T*)s->Q1;
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
-    if (!write_b) HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
+    // solvers that never change b leave the stage buffer's b row alone: it
+    // is copied once per call (once per march: copy_b on the first step)
+    if (!write_b && copy_b)
+        HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
     if (!write_b && Qout != Q)
         HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
     if (s->o.order < 2) {
@@ -1429,20 +1445,21 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     // step is order 2; the state after the last step gets its own pass.
     const bool fuse = Qaux && s->o.order >= 2 && s->o.recon == RECON_CONS;
     for (int k = 0; k < n_steps; ++k) {
-        hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev,
-                           (const double*)t_dev, t_end, s->p.dt_max);
+        hipLaunchKernelGGL(k_dt_next, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev, t_dev, t_end,
+                           s->p.dt_max, k > 0 ? 1 : 0);
         int rc = s->p.dtype == ZJ_F64
                      ? step_impl(s, dt_dev, (const double*)Q, (double*)Q, troubled_out, s->dt_red, st,
-                                 fuse && k > 0 ? (double*)Qaux : (double*)nullptr)
+                                 fuse && k > 0 ? (double*)Qaux : (double*)nullptr, k == 0)
                      : step_impl(s, dt_dev, (const float*)Q, (float*)Q, troubled_out, s->dt_red, st,
-                                 fuse && k > 0 ? (float*)Qaux : (float*)nullptr);
+                                 fuse && k > 0 ? (float*)Qaux : (float*)nullptr, k == 0);
         if (rc) return rc;
         if (Qaux && !fuse) {
             rc = zj_update_aux(s, Q, Qaux, stream);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_t_advance, dim3(1), dim3(1), 0, st, t_dev, (const double*)dt_dev);
     }
+    if (n_steps > 0)
+        hipLaunchKernelGGL(k_t_advance, dim3(1), dim3(1), 0, st, t_dev, (const double*)dt_dev);
     if (Qaux && fuse && n_steps > 0) {
         int rc = zj_update_aux(s, Q, Qaux, stream);
         if (rc) return rc;
