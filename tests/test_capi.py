@@ -27,3 +27,36 @@ def test_library_exports_declared_symbols():
 
 def test_version_string():
     assert _lib.lib().zj_version().startswith(b"zoomy_hip")
+
+
+def test_argument_checks_before_any_device_work():
+    """Entry points reject bad arguments with ZJ_EINVAL before they touch
+    the device (these calls run without a GPU)."""
+    import ctypes
+    L = _lib.lib()
+    null = ctypes.c_void_p(None)
+    assert L.zj_diffusion_create(null, null) == _lib.ZJ_EINVAL
+    assert L.zj_diffusion_explicit(null, 1.0, null, null, null) == _lib.ZJ_EINVAL
+    assert L.zj_diffusion_cn(null, 1.0, 0.1, null, null, 1e-12, 10,
+                             null) == _lib.ZJ_EINVAL
+    assert L.zj_diffusion_stats(null, null, null) == _lib.ZJ_EINVAL
+    assert L.zj_diffusion_destroy(null) == _lib.ZJ_OK
+    assert L.zj_solver_set_stage_aux(null, 0) == _lib.ZJ_EINVAL
+    assert L.zj_mesh_create(None, null) == _lib.ZJ_EINVAL
+
+
+def test_dirichlet_descriptor_validation():
+    """A Dirichlet face needs its tables, and only state rows 0-3 may be
+    prescribed (host-side checks of zj_mesh_create / zj_patch_check)."""
+    from zoomy_hip.fvm.solver import mesh_desc
+    from zoomy_hip.mesh import Dirichlet, Mesh
+    m = Mesh.line(8, 0.0, 1.0).set_bc({"left": Dirichlet("h", 1.0)})
+    mask, val = m.bface_dirichlet()
+    assert mask.tolist() == [2, 0] and val[1, 0] == 1.0
+    import pytest
+    with pytest.raises(ValueError):
+        Mesh.line(8).set_bc({"left": Dirichlet("u", 1.0)})
+    with pytest.raises(ValueError):
+        Mesh.line(8).set_bc({"left": "dirichlet"})
+    with pytest.raises(ValueError):
+        Mesh.line(8).set_bc({"middle": "wall"})
