@@ -265,11 +265,12 @@ __global__ __launch_bounds__(BLOCK) void k_aux(DevMesh m, Opts o, const T* __res
     Qaux[(size_t)row * N + c] = hic;
 }
 
-// dt of a state (compute_timestep): per-cell min over faces, block min,
-// atomicMin on the bit pattern (non-negative doubles order as integers).
+// dt of a state (compute_timestep): per-cell min over faces, block min to
+// part[block]; k_reduce_part folds the partials (a single address taking
+// one atomic per block serialises them: 16k blocks cost ~0.2 ms).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __restrict__ Q,
-                                              unsigned long long* __restrict__ dt_red) {
+                                              double* __restrict__ part) {
     __shared__ double lds[BLOCK / 64];
     const int c = xcd_block() * BLOCK + threadIdx.x;
     double loc = INFINITY;
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __rest
         }
     }
     double s = block_reduce(loc, [](double a, double b) { return fmin(a, b); }, lds);
-    if (threadIdx.x == 0) atomicMin(dt_red, (unsigned long long)__double_as_longlong(s));
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 __global__ void k_dt_init(unsigned long long* red, int* troubled) {
@@ -1239,6 +1240,17 @@ namespace {
 // the result does not depend on the block count)
 static inline int reduce_blocks(int n) { return std::max(1, std::min(64, (n + 4095) / 4096)); }
 
+// dt of Q into s->dt_red (already reset): block minima, then the fold.
+// dt_part holds npatch * 4 >= N / 64 slots, more than the N / 256 blocks.
+template <typename T>
+void launch_dt(zj_solver* s, const T* Q, hipStream_t st) {
+    const DevMesh& m = s->m->d;
+    const int nb = (m.N + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, s->dt_part);
+    hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(nb)), dim3(1024), 0, st,
+                       (const double*)s->dt_part, nb, s->dt_red, (const int*)nullptr);
+}
+
 template <typename T>
 void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double* dt, T* aux,
                   int mode, int final_stage, int write_b, int use_slopes, int* troubled,
@@ -1339,11 +1351,9 @@ int zj_compute_dt(zj_solver* s, const void* Q, double t_remaining, double* dt_ou
     const DevMesh& m = s->m->d;
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, (int*)nullptr);
     if (s->p.dtype == ZJ_F64)
-        hipLaunchKernelGGL(k_dt<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const double*)Q, s->dt_red);
+        launch_dt<double>(s, (const double*)Q, st);
     else
-        hipLaunchKernelGGL(k_dt<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const float*)Q, s->dt_red);
+        launch_dt<float>(s, (const float*)Q, st);
     // t_remaining is applied through a host scalar: finalize with t = 0 and
     // t_end = t_remaining.
     if (t_remaining >= 0.0) {
@@ -1434,11 +1444,9 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     const DevMesh& m = s->m->d;
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, troubled_out);
     if (s->p.dtype == ZJ_F64)
-        hipLaunchKernelGGL(k_dt<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const double*)Q, s->dt_red);
+        launch_dt<double>(s, (const double*)Q, st);
     else
-        hipLaunchKernelGGL(k_dt<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const float*)Q, s->dt_red);
+        launch_dt<float>(s, (const float*)Q, st);
     HIPCHK(hipGetLastError());
     // post_step aux refresh: fused into the first slope pass of the next
     // step (same gather) when the reconstruction is conservative and the
@@ -1505,7 +1513,7 @@ static int time_kernels_impl(zj_solver* s, const T* q, const double* dt, int rep
                     launch_stage<T>(s, q, q, out, dt, (T*)nullptr, 1, 1, write_b, slopes,
                                     s->troubled, s->dt_red, st);
                 else if (which == 2)
-                    hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, s->dt_red);
+                    launch_dt<T>(s, q, st);
                 else if (which == 3)
                     hipLaunchKernelGGL(k_aux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, aux);
                 else
