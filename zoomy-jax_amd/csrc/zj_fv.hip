@@ -197,8 +197,17 @@ __global__ __launch_bounds__(BLOCK) void k_aux(DevMesh m, Opts o, const T* __res
     T q[4];
     load_q(Q, N, c, q);
     const T hic = hinv_of(q[1], eps);
+    // the face geometry enters only through the ghost rows: interior cells
+    // read their centre alone
+    bool bnd = false;
+    ZJ_FOR_FACES(k) bnd |= m.nbr[k * N + c] < 0;
     Geo G;
-    load_geo(m, c, G);
+    if (bnd) {
+        load_geo(m, c, G);
+    } else {
+        G.cx = m.cen[c];
+        G.cy = m.cen[N + c];
+    }
     double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
     double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
     T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
