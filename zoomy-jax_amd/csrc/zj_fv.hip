@@ -360,12 +360,32 @@ __global__ void k_mood_finish(const int* __restrict__ cnt, int* __restrict__ sta
 }
 
 // Compensated (Neumaier) mass: per-thread partials, per-block (sum, comp)
-// pairs, then a single ordered pass -- deterministic for a fixed grid.
+// pairs folded by a fixed tree, then one more tree over the blocks --
+// deterministic for a fixed grid.
 __device__ __forceinline__ void neumaier(double& s, double& comp, double x) {
     double t = s + x;
     if (fabs(s) >= fabs(x)) comp += (s - t) + x;
     else comp += (x - t) + s;
     s = t;
+}
+
+__device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v);
+
+// Fixed-order pairwise fold of NT compensated (sum, compensation) pairs in
+// LDS into lds[0] (deterministic: the tree does not depend on timing).
+template <int NT>
+__device__ __forceinline__ void fold_pairs(double2* lds) {
+#pragma unroll 1
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < w) {
+            const double2 a = lds[threadIdx.x], b = lds[threadIdx.x + w];
+            double S = a.x, C = a.y;
+            neumaier(S, C, b.x);
+            lds[threadIdx.x] = make_double2(S, C + b.y);
+        }
+    }
+    __syncthreads();
 }
 
 template <typename T>
@@ -374,29 +394,22 @@ __global__ __launch_bounds__(BLOCK) void k_mass_partial(DevMesh m, const T* __re
     __shared__ double2 lds[BLOCK];
     double s = 0.0, comp = 0.0;
     for (int c = blockIdx.x * BLOCK + threadIdx.x; c < m.N; c += gridDim.x * BLOCK) {
-        Geo G;
-        load_geo(m, c, G);
-        neumaier(s, comp, (double)Q[m.N + c] * G.vol);
+        double2 v[MAXF];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k)
+            v[k] = k < m.mf ? m.verts[m.cv[k * m.N + c]] : make_double2(0.0, 0.0);
+        neumaier(s, comp, (double)Q[m.N + c] * cell_volume(m, v));
     }
     lds[threadIdx.x] = make_double2(s, comp);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double S = 0.0, C = 0.0;
-        for (int i = 0; i < BLOCK; ++i) {
-            neumaier(S, C, lds[i].x);
-            C += lds[i].y;
-        }
-        part[blockIdx.x] = make_double2(S, C);
-    }
+    fold_pairs<BLOCK>(lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
 }
 
-__global__ void k_mass_final(const double2* part, int n, double* out) {
-    double S = 0.0, C = 0.0;
-    for (int i = 0; i < n; ++i) {
-        neumaier(S, C, part[i].x);
-        C += part[i].y;
-    }
-    *out = S + C;
+__global__ __launch_bounds__(1024) void k_mass_final(const double2* part, int n, double* out) {
+    __shared__ double2 lds[1024];
+    lds[threadIdx.x] = threadIdx.x < n ? part[threadIdx.x] : make_double2(0.0, 0.0);
+    fold_pairs<1024>(lds);
+    if (threadIdx.x == 0) *out = lds[0].x + lds[0].y;
 }
 
 // Local implicit Manning stage (solver_imex_jax.py:268-342).
@@ -1567,7 +1580,7 @@ int zj_mass(zj_solver* s, const void* Q, double* mass_out, void* stream) {
     else
         hipLaunchKernelGGL(k_mass_partial<float>, dim3(s->mass_blocks), dim3(BLOCK), 0, st, m,
                            (const float*)Q, s->mass_part);
-    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1), 0, st, (const double2*)s->mass_part,
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1024), 0, st, (const double2*)s->mass_part,
                        s->mass_blocks, mass_out);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
