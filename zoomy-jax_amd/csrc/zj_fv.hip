@@ -652,7 +652,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
     if (use_slopes) {
-        for (int ls = tid; ls < nslope; ls += SBLOCK) {
+        // mode 3 (post_step aux rows only): the own cells' gradients, no
+        // limiter, no slopes, no face pass
+        const int nsl = mode == 3 ? nown : nslope;
+        for (int ls = tid; ls < nsl; ls += SBLOCK) {
             const int gi = ls < nown ? c0 + ls : lids[ls];
             uint16_t nls[MAXF];
 #pragma unroll
@@ -771,6 +774,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[0] : ay[0];
                 aux_out[(size_t)row * N + gi] = hic;
             }
+            if (mode == 3) continue;
             T phi[4] = {1, 1, 1, 1};
             if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
@@ -850,6 +854,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
         __syncthreads();
     }
+    if (mode == 3) return;
 
     // ── phase 3: fluxes of the own cells ─────────────────────────────
     int bad = 0;
@@ -1449,12 +1454,25 @@ int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {
     if (!s || !Q || !Qaux) return ZJ_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const DevMesh& m = s->m->d;
-    if (s->p.dtype == ZJ_F64)
+    const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
+    if (s->o.recon == RECON_CONS) {
+        // the patch kernel's gradient pass alone (mode 3): staged neighbours
+        // instead of per-cell gathers
+        if (s->p.dtype == ZJ_F64)
+            launch_stage<double>(s, (const double*)Q, (const double*)nullptr, (double*)nullptr,
+                                 (const double*)nullptr, (double*)Qaux, 3, 0, write_b, 1,
+                                 (int*)nullptr, (unsigned long long*)nullptr, st);
+        else
+            launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)nullptr,
+                                (const double*)nullptr, (float*)Qaux, 3, 0, write_b, 1,
+                                (int*)nullptr, (unsigned long long*)nullptr, st);
+    } else if (s->p.dtype == ZJ_F64) {
         hipLaunchKernelGGL(k_aux<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
                            (const double*)Q, (double*)Qaux);
-    else
+    } else {
         hipLaunchKernelGGL(k_aux<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
                            (const float*)Q, (float*)Qaux);
+    }
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1537,7 +1555,7 @@ static int time_kernels_impl(zj_solver* s, const T* q, const double* dt, int rep
                 else if (which == 2)
                     launch_dt<T>(s, q, st);
                 else if (which == 3)
-                    hipLaunchKernelGGL(k_aux<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, q, aux);
+                    (void)zj_update_aux(s, q, aux, st);
                 else
                     launch_stage<T>(s, q, (const T*)nullptr, out, dt, aux, 0, 0, write_b, slopes,
                                     (int*)nullptr, (unsigned long long*)nullptr, st);
