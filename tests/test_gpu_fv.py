@@ -139,6 +139,35 @@ def test_reference_long_o2_goldens(golden, name, n, L, t_end, ic, riemann):
     assert np.abs(Qaux.cpu().numpy() - ref["Qaux"]).max() < 5e-12
 
 
+@pytest.mark.parametrize("recon", ["conservative", "xz", "eta", "eta_hu"])
+@pytest.mark.parametrize("kind", ["quad", "tri", "line"])
+def test_post_step_aux_every_reconstruction(recon, kind):
+    """post_step's aux rows (the patch kernel's gradient pass for the
+    conservative reconstructions, the per-cell kernel for the eta ones)
+    against the oracle."""
+    from zoomy_hip.mesh import Mesh
+    if kind == "quad":
+        mesh = Mesh.quad(37, 29).set_bc({"left": "wall", "right": "extrapolation",
+                                         "bottom": "periodic", "top": "periodic"})
+    elif kind == "tri":
+        mesh, _ = _wetdry_tri()
+    else:
+        mesh = Mesh.line(90, 0.0, 3.0).set_bc("wall")
+    Q0 = circular_dam_state(mesh, seed=13) if kind != "line" else \
+        np.stack([0.1 * np.sin(mesh.cell_centers()[:, 0]),
+                  1.0 + 0.1 * np.cos(3 * mesh.cell_centers()[:, 0]),
+                  0.05 * np.ones(mesh.n_cells), np.zeros(mesh.n_cells)])
+    rng = np.random.default_rng(7)
+    Q0[2] = rng.normal(0.0, 0.1, mesh.n_cells)
+    if kind != "line":
+        Q0[3] = rng.normal(0.0, 0.1, mesh.n_cells)
+    o = _oracle(mesh, order=2, riemann="hll", recon=recon)
+    s = _solver(mesh, order=2, riemann="hll", recon=recon)
+    A = s.update_qaux(_to_dev(Q0), None).cpu().numpy()
+    Ao = o.aux(Q0)
+    assert np.abs(A - Ao).max() <= 1e-12 * np.abs(Ao).max()
+
+
 @pytest.mark.parametrize("recon,riemann", [
     ("conservative", "hll"), ("conservative", "rusanov_nc"),
     ("eta", "rusanov"), ("xz", "hll")])

@@ -1320,7 +1320,9 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
         HIPCHK(hipMemcpyAsync(Qout, Q1, sizeof(T) * 4 * m.N, hipMemcpyDeviceToDevice, st));
         return ZJ_OK;
     }
-    const bool fuse_aux = aux_of_Q && s->o.recon == RECON_CONS;
+    // the aux rows are gradients of the conservative state: the slope pass
+    // gives them when it reconstructs conservative variables (cons, xz)
+    const bool fuse_aux = aux_of_Q && (s->o.recon == RECON_CONS || s->o.recon == RECON_XZ);
     launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, fuse_aux ? aux_of_Q : (T*)nullptr, 0, 0,
                     write_b, 1, (int*)nullptr, (unsigned long long*)nullptr, st);
     if (!s->mood) {
@@ -1455,7 +1457,7 @@ int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const DevMesh& m = s->m->d;
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
-    if (s->o.recon == RECON_CONS) {
+    if (s->o.recon == RECON_CONS || s->o.recon == RECON_XZ) {
         // the patch kernel's gradient pass alone (mode 3): staged neighbours
         // instead of per-cell gathers
         if (s->p.dtype == ZJ_F64)
@@ -1491,7 +1493,8 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     // post_step aux refresh: fused into the first slope pass of the next
     // step (same gather) when the reconstruction is conservative and the
     // step is order 2; the state after the last step gets its own pass.
-    const bool fuse = Qaux && s->o.order >= 2 && s->o.recon == RECON_CONS;
+    const bool fuse = Qaux && s->o.order >= 2 &&
+                      (s->o.recon == RECON_CONS || s->o.recon == RECON_XZ);
     for (int k = 0; k < n_steps; ++k) {
         hipLaunchKernelGGL(k_dt_next, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev, t_dev, t_end,
                            s->p.dt_max, k > 0 ? 1 : 0);
