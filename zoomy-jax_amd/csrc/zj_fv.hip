@@ -562,8 +562,9 @@ __device__ __forceinline__ void lds_q4(const typename Vec2<T>::type* Qs, int ML,
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
 // HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
-// the options.  Both instantiations evaluate the same formulas.
-template <typename T, int SPEC>
+// the options.  Both instantiations evaluate the same formulas.  AUXONLY = 1
+// is the post_step aux refresh alone: staging and the own cells' gradients.
+template <typename T, int SPEC, int AUXONLY = 0>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_stage(
     DevMesh m_in, DevPatch pt, Opts o_in, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
@@ -652,9 +653,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
     if (use_slopes) {
-        // mode 3 (post_step aux rows only): the own cells' gradients, no
-        // limiter, no slopes, no face pass
-        const int nsl = mode == 3 ? nown : nslope;
+        // AUXONLY (post_step aux rows only, launched as mode 3): the own
+        // cells' gradients, no limiter, no slopes, no face pass
+        const int nsl = AUXONLY ? nown : nslope;
         for (int ls = tid; ls < nsl; ls += SBLOCK) {
             const int gi = ls < nown ? c0 + ls : lids[ls];
             uint16_t nls[MAXF];
@@ -774,7 +775,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[0] : ay[0];
                 aux_out[(size_t)row * N + gi] = hic;
             }
-            if (mode == 3) continue;
+            if (AUXONLY) continue;
             T phi[4] = {1, 1, 1, 1};
             if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
@@ -854,7 +855,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
         __syncthreads();
     }
-    if (mode == 3) return;
+    if (AUXONLY) return;
 
     // ── phase 3: fluxes of the own cells ─────────────────────────────
     int bad = 0;
@@ -1289,7 +1290,7 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                       s->o.recon == RECON_CONS && !s->o.stage_aux;
-    auto kern = spec ? k_stage<T, 1> : k_stage<T, 0>;
+    auto kern = mode == 3 ? k_stage<T, 0, 1> : (spec ? k_stage<T, 1> : k_stage<T, 0>);
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
