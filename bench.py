@@ -424,9 +424,8 @@ def main():
                           "launches": 10}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the multi-GPU halo exchange is "
-              "out of scope for this build (see DESIGN.md); use --gpus 1",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: the multi-GPU halo exchange "
+              "is not implemented; use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     from zoomy_hip.mesh import Mesh
