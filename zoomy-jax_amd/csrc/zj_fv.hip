@@ -1290,7 +1290,8 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                       s->o.recon == RECON_CONS && !s->o.stage_aux;
-    auto kern = mode == 3 ? k_stage<T, 0, 1> : (spec ? k_stage<T, 1> : k_stage<T, 0>);
+    auto kern = mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
+                          : (spec ? k_stage<T, 1> : k_stage<T, 0>);
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
