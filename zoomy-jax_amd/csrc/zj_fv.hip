@@ -274,11 +274,31 @@ __global__ __launch_bounds__(BLOCK) void k_aux(DevMesh m, Opts o, const T* __res
     Qaux[(size_t)row * N + c] = hic;
 }
 
+// Per (slot, cell): the outward unit normal and r_f = min(r_cell, r_nbr)
+// (r_cell on boundary slots) -- what the dt law reads, precomputed once per
+// mesh so that k_dt streams them instead of gathering vertices.
+__global__ __launch_bounds__(BLOCK) void k_slot_dt_geo(DevMesh m, double2* __restrict__ sn,
+                                                       double* __restrict__ sr) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= m.N) return;
+    const int N = m.N;
+    Geo G;
+    load_geo(m, c, G);
+    const double rc = m.rin[c];
+    ZJ_FOR_FACES(k) {
+        const int nb = m.nbr[k * N + c];
+        sn[(size_t)k * N + c] = make_double2(G.nx[k], G.ny[k]);
+        sr[(size_t)k * N + c] = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
+    }
+}
+
 // dt of a state (compute_timestep): per-cell min over faces, block min to
 // part[block]; k_reduce_part folds the partials (a single address taking
 // one atomic per block serialises them: 16k blocks cost ~0.2 ms).
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __restrict__ Q,
+                                              const double2* __restrict__ sn,
+                                              const double* __restrict__ sr,
                                               double* __restrict__ part) {
     __shared__ double lds[BLOCK / 64];
     const int c = xcd_block() * BLOCK + threadIdx.x;
@@ -288,13 +308,10 @@ __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __rest
         T q[4];
         load_q(Q, N, c, q);
         const T hn = hinv_of(q[1], T(o.eps));
-        Geo G;
-        load_geo(m, c, G);
-        const double rc = m.rin[c];
         ZJ_FOR_FACES(k) {
-            const int nb = m.nbr[k * N + c];
-            const double rf = nb >= 0 ? fmin(rc, m.rin[nb]) : rc;
-            const double a = (double)max_abs_eig(q, hn, T(G.nx[k]), T(G.ny[k]), T(o.g));
+            const double2 n = sn[(size_t)k * N + c];
+            const double rf = sr[(size_t)k * N + c];
+            const double a = (double)max_abs_eig(q, hn, T(n.x), T(n.y), T(o.g));
             double l = o.cfl * 2.0 * rf / ((double)m.dim * a);
             if (l != l) l = 0.0;
             loc = fmin(loc, l);
@@ -1017,6 +1034,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 struct zj_mesh {
     DevMesh d;
     DevPatch pt;
+    double2* slot_n = nullptr;  // (mf, N) outward normals (dt law)
+    double* slot_r = nullptr;   // (mf, N) min(r_cell, r_nbr)
     int n_templates = 0;
     std::vector<void*> allocs;
 };
@@ -1203,6 +1222,23 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         }
         m->n_templates = ph.ntpl;
     }
+    {
+        void* q = nullptr;
+        rc = dev_upload(m, nullptr, sizeof(double2) * (size_t)mf * N, &q);
+        m->slot_n = (double2*)q;
+        rc |= dev_upload(m, nullptr, sizeof(double) * (size_t)mf * N, &q);
+        m->slot_r = (double*)q;
+        if (rc) {
+            zj_mesh_destroy(m);
+            return ZJ_ENOMEM;
+        }
+        hipLaunchKernelGGL(k_slot_dt_geo, dim3(nblocks(N)), dim3(BLOCK), 0, 0, m->d, m->slot_n,
+                           m->slot_r);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            zj_mesh_destroy(m);
+            return ZJ_EHIP;
+        }
+    }
     *out = m;
     return ZJ_OK;
 }
@@ -1274,7 +1310,8 @@ template <typename T>
 void launch_dt(zj_solver* s, const T* Q, hipStream_t st) {
     const DevMesh& m = s->m->d;
     const int nb = (m.N + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q, s->dt_part);
+    hipLaunchKernelGGL(k_dt<T>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, Q,
+                       (const double2*)s->m->slot_n, (const double*)s->m->slot_r, s->dt_part);
     hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(nb)), dim3(1024), 0, st,
                        (const double*)s->dt_part, nb, s->dt_red, (const int*)nullptr);
 }
