@@ -119,3 +119,126 @@ def wetdry_quads(n=48):
     Q[0] = 0.3 * np.sin(xc[:, 0]) ** 2 + 0.05 * xc[:, 1]
     Q[1] = np.where(xc[:, 0] < 3.0, np.maximum(3.0 - Q[0], 0.0), 0.0)
     return mesh, Q
+
+
+def swe_exact_riemann(hL, uL, hR, uR, g=G):
+    """Exact solution of the shallow-water Riemann problem with both sides
+    wet (Toro, "Shock-capturing methods for free-surface shallow flows",
+    2001, ch. 5): the star depth from the bisection of
+    f_L(h) + f_R(h) + u_R - u_L = 0 (shock branch (h - h_k) sqrt(g (h + h_k)
+    / (2 h h_k)), rarefaction branch 2 (sqrt(g h) - c_k)).  Returns a sampler
+    xi = x / t -> (h, u); ``sample.span`` = the slowest and fastest wave
+    speeds, ``sample.us`` = the star (contact) speed."""
+    cL, cR = np.sqrt(g * hL), np.sqrt(g * hR)
+
+    def fk(h, hk, ck):
+        if h > hk:
+            return (h - hk) * np.sqrt(0.5 * g * (h + hk) / (h * hk))
+        return 2.0 * (np.sqrt(g * h) - ck)
+
+    def f(h):
+        return fk(h, hL, cL) + fk(h, hR, cR) + uR - uL
+
+    a, b = 1e-12, 10.0 * max(hL, hR) + (abs(uL) + abs(uR)) ** 2 / g
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if f(a) * f(m) <= 0:
+            b = m
+        else:
+            a = m
+    hs = 0.5 * (a + b)
+    cs = np.sqrt(g * hs)
+    us = 0.5 * (uL + uR) + 0.5 * (fk(hs, hR, cR) - fk(hs, hL, cL))
+    sl_shock = uL - cL * np.sqrt(0.5 * hs * (hs + hL)) / hL
+    sr_shock = uR + cR * np.sqrt(0.5 * hs * (hs + hR)) / hR
+
+    def sample(xi):
+        xi = np.asarray(xi, float)
+        h = np.empty_like(xi)
+        u = np.empty_like(xi)
+        left = xi <= us
+        if hs > hL:                           # left shock
+            m1 = left & (xi < sl_shock)
+            m2 = left & (xi >= sl_shock)
+            h[m1], u[m1] = hL, uL
+            h[m2], u[m2] = hs, us
+        else:                                 # left rarefaction
+            head, tail = uL - cL, us - cs
+            m1 = left & (xi < head)
+            m3 = left & (xi > tail)
+            m2 = left & ~m1 & ~m3
+            h[m1], u[m1] = hL, uL
+            h[m3], u[m3] = hs, us
+            u[m2] = (uL + 2.0 * cL + 2.0 * xi[m2]) / 3.0
+            h[m2] = ((uL + 2.0 * cL - xi[m2]) / 3.0) ** 2 / g
+        right = ~left
+        if hs > hR:                           # right shock
+            m1 = right & (xi > sr_shock)
+            m2 = right & (xi <= sr_shock)
+            h[m1], u[m1] = hR, uR
+            h[m2], u[m2] = hs, us
+        else:                                 # right rarefaction
+            head, tail = uR + cR, us + cs
+            m1 = right & (xi > head)
+            m3 = right & (xi < tail)
+            m2 = right & ~m1 & ~m3
+            h[m1], u[m1] = hR, uR
+            h[m3], u[m3] = hs, us
+            u[m2] = (uR - 2.0 * cR + 2.0 * xi[m2]) / 3.0
+            h[m2] = ((-uR + 2.0 * cR + xi[m2]) / 3.0) ** 2 / g
+        return h, u
+
+    sample.span = (sl_shock if hs > hL else uL - cL,
+                   sr_shock if hs > hR else uR + cR)
+    sample.us = us
+    return sample
+
+
+def ritter_exact(x, t, hL, x0, g=G):
+    """Ritter's dry dam break (the reference's tests/test_swe_dambreak_jax.py
+    :106-125): depth at points x, time t."""
+    cL = np.sqrt(g * hL)
+    x = np.asarray(x, float)
+    h = np.zeros_like(x)
+    h[x < x0 - cL * t] = hL
+    fan = (x >= x0 - cL * t) & (x <= x0 + 2.0 * cL * t)
+    h[fan] = (2.0 * cL - (x[fan] - x0) / t) ** 2 / (9.0 * g)
+    return h
+
+
+def fractal_valley_bed(xc, seed=3):
+    """Seeded fractal bed on [0, 10] x [0, 5] of the wet/dry triangle
+    workload: four octaves of random-phase plane waves (amplitude halving,
+    frequency doubling), scaled to a maximum of 1.5, plus a linear valley
+    slope toward x = 10 and the centre line y = 2.5."""
+    rng = np.random.default_rng(seed)
+    x, y = xc[:, 0], xc[:, 1]
+    f = np.zeros(len(xc))
+    for o in range(4):
+        for _ in range(6):
+            th = rng.uniform(0.0, np.pi)
+            k = 2.0 ** o * 0.6 * np.array([np.cos(th), np.sin(th)])
+            f += 0.5 ** o * np.cos(k[0] * x + k[1] * y + rng.uniform(0, 6.283))
+    f = 1.5 * (f - f.min()) / (f.max() - f.min())
+    return f + 0.08 * (10.0 - x) + 0.1 * np.abs(y - 2.5)
+
+
+def rcm_wetdry_triangles(nx=100, ny=50, seed=3):
+    """The wet/dry triangle workload at a reduced size: Delaunay of nx x ny
+    jittered points on [0, 10] x [0, 5] (about 2 nx ny triangles), cells in
+    RCM order, walls; the fractal valley bed, reservoir h = 3 in x < 2 and
+    dry elsewhere."""
+    from zoomy_hip import _lib
+    from zoomy_hip.mesh import delaunay_mesh, jittered_points
+    base = delaunay_mesh(jittered_points(nx, ny, (0.0, 10.0, 0.0, 5.0), 0.3,
+                                         seed), (0.0, 10.0, 0.0, 5.0))
+    ip, ix = base.adjacency_csr()
+    perm = np.zeros(base.n_cells, np.int32)
+    _lib.check(_lib.lib().zj_rcm(base.n_cells, ip.ctypes.data, ix.ctypes.data,
+                                 perm.ctypes.data), "zj_rcm")
+    mesh = base.reorder(perm).set_bc("wall")
+    xc = mesh.cell_centers()
+    Q = np.zeros((4, mesh.n_cells))
+    Q[0] = fractal_valley_bed(xc, seed)
+    Q[1] = np.where(xc[:, 0] < 2.0, 3.0, 0.0)
+    return mesh, Q

@@ -1063,6 +1063,10 @@ struct zj_solver {
     uint8_t* mood_mask = nullptr;   // per-cell first-order flags
     int* mood_cnt = nullptr;        // troubled cells of the candidate
     int* mood_stats = nullptr;      // {steps redone, cells demoted}
+    // (4, N) second-stage output of an in-place step with the frozen stage
+    // aux: that stage reads the neighbours' step-start depth from Q0, which
+    // other workgroups of the same launch would be overwriting
+    void* Qfrz = nullptr;
 };
 
 namespace zj {
@@ -1291,6 +1295,7 @@ int zj_solver_destroy(zj_solver* s) {
     if (s->mood_mask) (void)hipFree(s->mood_mask);
     if (s->mood_cnt) (void)hipFree(s->mood_cnt);
     if (s->mood_stats) (void)hipFree(s->mood_stats);
+    if (s->Qfrz) (void)hipFree(s->Qfrz);
     delete s;
     return ZJ_OK;
 }
@@ -1365,8 +1370,17 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
     launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, fuse_aux ? aux_of_Q : (T*)nullptr, 0, 0,
                     write_b, 1, (int*)nullptr, (unsigned long long*)nullptr, st);
     if (!s->mood) {
-        launch_stage<T>(s, (const T*)Q1, Q, Qout, dt, (T*)nullptr, 1, 1, write_b, 1, troubled,
+        // with the frozen stage aux, stage 2 gathers the neighbours' step-start
+        // depth from Q: an in-place step writes it to scratch first
+        const bool stage_scratch = s->o.stage_aux && (const void*)Qout == (const void*)Q;
+        T* Q2 = stage_scratch ? (T*)s->Qfrz : Qout;
+        if (stage_scratch && !Q2) return ZJ_EINVAL;
+        launch_stage<T>(s, (const T*)Q1, Q, Q2, dt, (T*)nullptr, 1, 1, write_b, 1, troubled,
                         dt_red, st);
+        if (stage_scratch)
+            HIPCHK(hipMemcpyAsync(Qout + (write_b ? 0 : m.N), Q2 + (write_b ? 0 : m.N),
+                                  sizeof(T) * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice,
+                                  st));
         HIPCHK(hipGetLastError());
         return ZJ_OK;
     }
@@ -1454,12 +1468,20 @@ int zj_solver_set_mood(zj_solver* s, int32_t on) {
     if (on && !s->Qc) {
         const size_t N = s->m->d.N;
         const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
-        if (hipMalloc(&s->Qc, esz * 4 * N) != hipSuccess ||
-            hipMalloc((void**)&s->mood_mask, N) != hipSuccess ||
-            hipMalloc((void**)&s->mood_cnt, sizeof(int)) != hipSuccess ||
-            hipMalloc((void**)&s->mood_stats, 2 * sizeof(int)) != hipSuccess)
+        // all four buffers or none: a partial set is freed again
+        void *qc = nullptr, *mask = nullptr, *cnt = nullptr, *stats = nullptr;
+        if (hipMalloc(&qc, esz * 4 * N) != hipSuccess || hipMalloc(&mask, N) != hipSuccess ||
+            hipMalloc(&cnt, sizeof(int)) != hipSuccess ||
+            hipMalloc(&stats, 2 * sizeof(int)) != hipSuccess ||
+            hipMemset(stats, 0, 2 * sizeof(int)) != hipSuccess) {
+            for (void* q : {qc, mask, cnt, stats})
+                if (q) (void)hipFree(q);
             return ZJ_ENOMEM;
-        HIPCHK(hipMemset(s->mood_stats, 0, 2 * sizeof(int)));
+        }
+        s->Qc = qc;
+        s->mood_mask = (uint8_t*)mask;
+        s->mood_cnt = (int*)cnt;
+        s->mood_stats = (int*)stats;
     }
     s->mood = on ? 1 : 0;
     return ZJ_OK;
@@ -1476,6 +1498,13 @@ int zj_solver_set_positivity(zj_solver* s, int32_t method, double front_tol) {
 
 int zj_solver_set_stage_aux(zj_solver* s, int32_t mode) {
     if (!s || mode < ZJ_STAGE_AUX_REFRESH || mode > ZJ_STAGE_AUX_FROZEN) return ZJ_EINVAL;
+    if (mode == ZJ_STAGE_AUX_FROZEN && !s->Qfrz) {
+        const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+        if (hipMalloc(&s->Qfrz, esz * 4 * (size_t)s->m->d.N) != hipSuccess) {
+            s->Qfrz = nullptr;
+            return ZJ_ENOMEM;
+        }
+    }
     s->o.stage_aux = mode;
     return ZJ_OK;
 }

@@ -239,19 +239,7 @@ class HyperbolicSolver:
         self.model = model
         self.n_state = model.n_state
         self._dmesh = DeviceMesh(mesh)
-        self._rt = _Runtime(self._dmesh, model, self.cfl, self.dtype,
-                            self.reconstruction_variables)
-        if self.mood:
-            L.check(L.lib().zj_solver_set_mood(self._rt.handle, 1),
-                    "zj_solver_set_mood")
-        L.check(L.lib().zj_solver_set_stage_aux(
-            self._rt.handle, L.STAGE_AUX[self.stage_aux]),
-            "zj_solver_set_stage_aux")
-        if self.positivity_method or self.front_theta_tol is not None:
-            L.check(L.lib().zj_solver_set_positivity(
-                self._rt.handle, L.POSITIVITY[self.positivity_method],
-                -1.0 if self.front_theta_tol is None
-                else float(self.front_theta_tol)), "zj_solver_set_positivity")
+        self._rt = self._make_runtime(model)
         dev = _device()
         self.device = dev
         self._dt = torch.zeros((), dtype=torch.float64, device=dev)
@@ -264,6 +252,25 @@ class HyperbolicSolver:
                            dtype=self.dtype, device=dev)
         Qaux = self.update_qaux(Q, Qaux)
         return self._view(Q), Qaux
+
+    def _make_runtime(self, model):
+        """A solver context with this solver's numerics options (MOOD,
+        stage aux, positivity guards) applied."""
+        rt = _Runtime(self._dmesh, model, self.cfl, self.dtype,
+                      self.reconstruction_variables)
+        lib = L.lib()
+        if self.mood:
+            L.check(lib.zj_solver_set_mood(rt.handle, 1),
+                    "zj_solver_set_mood")
+        L.check(lib.zj_solver_set_stage_aux(
+            rt.handle, L.STAGE_AUX[self.stage_aux]),
+            "zj_solver_set_stage_aux")
+        if self.positivity_method or self.front_theta_tol is not None:
+            L.check(lib.zj_solver_set_positivity(
+                rt.handle, L.POSITIVITY[self.positivity_method],
+                -1.0 if self.front_theta_tol is None
+                else float(self.front_theta_tol)), "zj_solver_set_positivity")
+        return rt
 
     @property
     def runtime(self):
@@ -346,12 +353,28 @@ class HyperbolicSolver:
         (``zoomy_jax/fvm/solver2/blocks.py:290-301``); synchronises."""
         return int(self._troubled[0].item())
 
+    def _check_qaux(self, Qaux, n_cells):
+        """The kernels write (3 dim + 1, N) words of the solver's dtype
+        through Qaux's pointer: anything else is refused."""
+        shape = (3 * self.mesh.dim + 1, n_cells)
+        if not torch.is_tensor(Qaux):
+            raise ValueError("Qaux must be a torch tensor on the solver's "
+                             "device")
+        if tuple(Qaux.shape) != shape or Qaux.dtype != self.dtype or \
+                Qaux.device != self.device or not Qaux.is_contiguous():
+            raise ValueError(
+                f"Qaux must be a contiguous {self.dtype} tensor of shape "
+                f"{shape} on {self.device}; got {Qaux.dtype} "
+                f"{tuple(Qaux.shape)} on {Qaux.device}"
+                f"{'' if Qaux.is_contiguous() else ' (non-contiguous)'}")
+        return Qaux
+
     def update_qaux(self, Q, Qaux):
         Q4 = self._full(Q)
-        if Qaux is None or Qaux.shape != (3 * self.mesh.dim + 1,
-                                          Q4.shape[1]):
+        if Qaux is None:
             Qaux = torch.zeros((3 * self.mesh.dim + 1, Q4.shape[1]),
                                dtype=self.dtype, device=self.device)
+        self._check_qaux(Qaux, Q4.shape[1])
         L.check(L.lib().zj_update_aux(self._rt.handle, _p(Q4), _p(Qaux),
                                       _stream()), "zj_update_aux")
         return Qaux
@@ -395,6 +418,8 @@ class HyperbolicSolver:
         aux refresh into ``Qaux`` when given).  Q is updated in place;
         returns ``(Q, t_device, dt_device)``."""
         Q4 = self._full(Q)
+        if Qaux is not None:
+            self._check_qaux(Qaux, Q4.shape[1])
         self._t.fill_(float(t0))
         te = self.time_end if time_end is None else time_end
         L.check(L.lib().zj_march(self._rt.handle, _p(Q4), _p(Qaux),
@@ -453,8 +478,10 @@ class IMEXSourceSolver(HyperbolicSolver):
         import copy
         m2 = copy.copy(model)
         m2.manning_n = 0.0
-        self._rt_h = _Runtime(self._dmesh, m2, self.cfl, self.dtype,
-                              self.reconstruction_variables)
+        # the explicit stage keeps MOOD, the positivity guards and the stage
+        # aux, as the reference's IMEX explicit stage does
+        # (solver_imex_jax.py:604-611)
+        self._rt_h = self._make_runtime(m2)
         self._diffusion = []
         if model.diffusion:
             if self.dtype != torch.float64:
