@@ -109,6 +109,16 @@ int zj_flux_operator(zj_solver *s, const void *Q, void *dQ, void *stream);
 int zj_compute_dt(zj_solver *s, const void *Q, double t_remaining,
                   double *dt_out, void *stream);
 
+/* zj_compute_dt without device allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  Same kernels and result as zj_compute_dt,
+ * with its block partials and reduction word in a static device array of
+ * the library (512 KiB per GPU, kept until the process ends) instead of the
+ * solver's scratch: meshes of more than 16,777,216 cells are refused
+ * (ZJ_EINVAL).  Calls that overlap on one GPU share that array and race. */
+int zj_compute_dt_noalloc(zj_solver *s, const void *Q, double t_remaining,
+                          double *dt_out, void *stream);
+
 /* One step with device dt (step, solver_jax.py:1095-1131): Qout = step(Q).
  * troubled_out (device int32, may be NULL) receives the number of cells with
  * h < 0 or a non-finite component (blocks.py:290-301). */

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -336,6 +337,15 @@ __global__ void k_dt_finalize(unsigned long long* red, double* dt, const double*
 }
 
 __global__ void k_t_advance(double* t, const double* dt) { *t += *dt; }
+
+// dt of compute_timestep capped by a host remainder (t_rem < 0: no cap).
+__global__ void k_dt_finalize_rem(unsigned long long* red, double* dt, double t_rem,
+                                  double dt_max) {
+    double d = __longlong_as_double((long long)*red);
+    if (dt_max > 0.0) d = fmin(d, dt_max);
+    if (t_rem >= 0.0) d = fmin(d, t_rem);
+    *dt = d;
+}
 
 // The march's per-step head: t += the previous step's dt (advance != 0),
 // then dt of this step as k_dt_finalize (one launch instead of two).
@@ -1691,3 +1701,76 @@ int zj_imex_local(zj_solver* s, const double* dt, int32_t maxiter, const void* Q
 }
 
 }  // extern "C"
+
+// ─────────────────────────────────────────────────────────────────────────
+// Allocation-free dt: the kernels of zj_compute_dt with their per-block
+// partials and reduction word in a file-scope device array instead of the
+// solver's scratch.  Partials of up to DT_NOALLOC_MAX_BLOCKS blocks of BLOCK
+// cells (16,777,216 cells), then the 8-byte reduction word: 512 KiB + 256 B
+// that every GPU the library runs on holds until the process ends.  Every
+// call on a device shares that device's array; calls that overlap on one
+// device race.
+// ─────────────────────────────────────────────────────────────────────────
+namespace {
+constexpr int DT_NOALLOC_MAX_BLOCKS = 65536;
+constexpr int DT_NOALLOC_WORDS = DT_NOALLOC_MAX_BLOCKS + 32;
+}  // namespace
+
+static __device__ __align__(256) double scratch[DT_NOALLOC_WORDS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[MAX_DEVICES];  // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the array's name is only a host-side
+// stand-in: never pass it to a kernel or a copy).
+int scratch_on_current_device(double** out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return ZJ_EUNSUP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ZJ_EHIP;
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return ZJ_OK;
+}
+}  // namespace
+
+extern "C" int zj_compute_dt_noalloc(zj_solver* s, const void* Q, double t_remaining,
+                                     double* dt_out, void* stream) {
+    if (!s || !Q || !dt_out) return ZJ_EINVAL;
+    const DevMesh& m = s->m->d;
+    const int nb = nblocks(m.N);
+    if (nb > DT_NOALLOC_MAX_BLOCKS) {
+        fprintf(stderr,
+                "zoomy_hip: zj_compute_dt_noalloc: %d cells exceed the %d cells its static "
+                "scratch holds; use zj_compute_dt\n",
+                m.N, DT_NOALLOC_MAX_BLOCKS * BLOCK);
+        return ZJ_EINVAL;
+    }
+    double* part = nullptr;
+    const int rc = scratch_on_current_device(&part);
+    if (rc) return rc;
+    unsigned long long* red = (unsigned long long*)(part + DT_NOALLOC_MAX_BLOCKS);
+    hipStream_t st = (hipStream_t)stream;
+    // the reduction word keeps the last call's value: reset it first
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, red, (int*)nullptr);
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_dt<double>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const double*)Q,
+                           (const double2*)s->m->slot_n, (const double*)s->m->slot_r, part);
+    else
+        hipLaunchKernelGGL(k_dt<float>, dim3(nb), dim3(BLOCK), 0, st, m, s->o, (const float*)Q,
+                           (const double2*)s->m->slot_n, (const double*)s->m->slot_r, part);
+    hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(nb)), dim3(1024), 0, st,
+                       (const double*)part, nb, red, (const int*)nullptr);
+    hipLaunchKernelGGL(k_dt_finalize_rem, dim3(1), dim3(1), 0, st, red, dt_out, t_remaining,
+                       s->p.dt_max);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}

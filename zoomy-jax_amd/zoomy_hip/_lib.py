@@ -88,6 +88,9 @@ _SIGS = {
     "zj_compute_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_double, ctypes.c_void_p,
                                      ctypes.c_void_p]),
+    "zj_compute_dt_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_double, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "zj_step": (ctypes.c_int, [ctypes.c_void_p] * 6),
     "zj_update_aux": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "zj_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
