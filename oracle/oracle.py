@@ -43,7 +43,8 @@ class _Mesh(ctypes.Structure):
             "face_cells", "face_normals", "face_volumes", "face_centers",
             "cell_centers", "cell_volumes", "cell_inradius", "bface_face",
             "bface_cell", "bface_kind", "bface_partner", "stencil", "lsq_w",
-            "gslot", "lsq_wg", "bface_mask", "bface_value")]
+            "gslot", "lsq_wg", "bface_mask", "bface_value")] + [
+        ("max_cf", ctypes.c_int), ("cell_faces", ctypes.c_void_p)]
 
 
 class _Params(ctypes.Structure):
@@ -68,38 +69,60 @@ def build(force: bool = False) -> pathlib.Path:
     return LIB_PATH
 
 
-def lib():
-    global _lib
+LIB_OMP_PATH = HERE / "liboracle_omp.so"
+_lib_omp = None
+
+
+def lib(omp: bool = False):
+    """The serial checker, or (``omp=True``) the same source built with
+    OpenMP -- the multi-threaded CPU baseline, bit-identical results."""
+    global _lib, _lib_omp
+    if omp:
+        if _lib_omp is None:
+            if not LIB_OMP_PATH.exists():
+                build(force=True)
+            _lib_omp = _bind(ctypes.CDLL(str(LIB_OMP_PATH)))
+        return _lib_omp
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = ctypes.CDLL(str(LIB_PATH))
-        P = ctypes.POINTER
-        d = ctypes.c_double
-        vp = ctypes.c_void_p
-        L.zjo_hinv.restype = d
-        L.zjo_hinv.argtypes = [d, d]
-        L.zjo_compute_dt.restype = d
-        L.zjo_compute_dt.argtypes = [P(_Mesh), P(_Params), vp, d]
-        L.zjo_step.restype = ctypes.c_int
-        L.zjo_step.argtypes = [P(_Mesh), P(_Params), d, vp, vp]
-        L.zjo_step_mood.restype = ctypes.c_int
-        L.zjo_step_mood.argtypes = [P(_Mesh), P(_Params), d, vp, vp,
-                                    P(ctypes.c_int)]
-        L.zjo_flux_operator.argtypes = [P(_Mesh), P(_Params), vp, vp]
-        L.zjo_reconstruct.argtypes = [P(_Mesh), P(_Params), vp, vp, vp, vp]
-        L.zjo_update_aux.argtypes = [P(_Mesh), P(_Params), vp, vp]
-        L.zjo_mass.restype = d
-        L.zjo_mass.argtypes = [P(_Mesh), vp]
-        L.zjo_lsq_grad.argtypes = [P(_Mesh), vp, vp, vp]
-        L.zjo_imex_local.argtypes = [P(_Mesh), P(_Params), d, ctypes.c_int,
-                                     vp, vp]
-        L.zjo_march.restype = ctypes.c_int
-        L.zjo_march.argtypes = [P(_Mesh), P(_Params), vp, P(d), d,
-                                ctypes.c_int]
-        L.zjo_face_flux.argtypes = [P(_Params), vp, vp, d, d, vp, vp, vp, vp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """Threads of the OpenMP build (returns the count in effect)."""
+    return int(lib(omp=True).zjo_set_threads(int(n)))
+
+
+def _bind(L):
+    P = ctypes.POINTER
+    d = ctypes.c_double
+    vp = ctypes.c_void_p
+    L.zjo_hinv.restype = d
+    L.zjo_hinv.argtypes = [d, d]
+    L.zjo_compute_dt.restype = d
+    L.zjo_compute_dt.argtypes = [P(_Mesh), P(_Params), vp, d]
+    L.zjo_step.restype = ctypes.c_int
+    L.zjo_step.argtypes = [P(_Mesh), P(_Params), d, vp, vp]
+    L.zjo_step_mood.restype = ctypes.c_int
+    L.zjo_step_mood.argtypes = [P(_Mesh), P(_Params), d, vp, vp,
+                                P(ctypes.c_int)]
+    L.zjo_flux_operator.argtypes = [P(_Mesh), P(_Params), vp, vp]
+    L.zjo_reconstruct.argtypes = [P(_Mesh), P(_Params), vp, vp, vp, vp]
+    L.zjo_update_aux.argtypes = [P(_Mesh), P(_Params), vp, vp]
+    L.zjo_mass.restype = d
+    L.zjo_mass.argtypes = [P(_Mesh), vp]
+    L.zjo_lsq_grad.argtypes = [P(_Mesh), vp, vp, vp]
+    L.zjo_imex_local.argtypes = [P(_Mesh), P(_Params), d, ctypes.c_int,
+                                 vp, vp]
+    L.zjo_march.restype = ctypes.c_int
+    L.zjo_march.argtypes = [P(_Mesh), P(_Params), vp, P(d), d,
+                            ctypes.c_int]
+    L.zjo_face_flux.argtypes = [P(_Params), vp, vp, d, d, vp, vp, vp, vp]
+    L.zjo_set_threads.restype = ctypes.c_int
+    L.zjo_set_threads.argtypes = [ctypes.c_int]
+    return L
 
 
 def _ptr(a):
@@ -182,6 +205,11 @@ class RefMesh:
         dm, dv = mesh.bface_dirichlet()
         self.bface_mask = np.ascontiguousarray(dm, np.int32)
         self.bface_value = np.ascontiguousarray(dv)
+        # faces of each cell in increasing id, -1 last
+        cf = mesh.cell_faces().astype(np.int64)
+        cf = np.sort(np.where(cf < 0, np.iinfo(np.int64).max, cf), axis=1)
+        self.cell_faces = np.ascontiguousarray(
+            np.where(cf == np.iinfo(np.int64).max, -1, cf), np.int32)
         self.n_faces = F
         self.c = _Mesh(mesh.dim, N, F, mesh.n_bfaces, st.shape[1],
                        *[a.ctypes.data for a in (
@@ -192,7 +220,8 @@ class RefMesh:
                            self.bface_cell, self.bface_kind,
                            self.bface_partner, self.stencil, self.lsq_w,
                            self.gslot, self.lsq_wg, self.bface_mask,
-                           self.bface_value)])
+                           self.bface_value)],
+                       self.cell_faces.shape[1], self.cell_faces.ctypes.data)
 
 
 def params(g=9.81, cfl=0.9, eps_wet=1e-8, dt_max=0.0, manning_n=0.0,
@@ -208,10 +237,10 @@ def params(g=9.81, cfl=0.9, eps_wet=1e-8, dt_max=0.0, manning_n=0.0,
 class Oracle:
     """Stateless wrapper; ``Q`` is a (4, N) float64 array (b, h, hu, hv)."""
 
-    def __init__(self, mesh, **kw):
+    def __init__(self, mesh, omp: bool = False, **kw):
         self.rm = mesh if isinstance(mesh, RefMesh) else RefMesh(mesh)
         self.p = params(**kw)
-        self.L = lib()
+        self.L = lib(omp)
 
     @property
     def n(self):

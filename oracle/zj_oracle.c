@@ -12,6 +12,42 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef ZJO_OMP
+#include <omp.h>
+#endif
+
+/* The same source builds the serial checker (liboracle.so) and, with
+ * -fopenmp -DZJO_OMP, the multi-threaded CPU baseline (liboracle_omp.so).
+ * Only loops whose result does not depend on the thread count run in
+ * parallel: per-cell and per-face loops writing their own entries, min/max
+ * scatters (exact in any order) and integer counts; every floating-point
+ * sum keeps its serial order, so both builds give identical bits. */
+#ifdef ZJO_OMP
+#define ZJO_PRAGMA(x) _Pragma(#x)
+#define ZJO_PAR_FOR ZJO_PRAGMA(omp parallel for schedule(static))
+#define ZJO_PAR_FOR_MAXMIN(a, b, n) \
+    ZJO_PRAGMA(omp parallel for schedule(static) reduction(max : a[:n]) reduction(min : b[:n]))
+#define ZJO_PAR_FOR_MIN(a, n) ZJO_PRAGMA(omp parallel for schedule(static) reduction(min : a[:n]))
+#define ZJO_PAR_FOR_MIN_SCALAR(x) ZJO_PRAGMA(omp parallel for schedule(static) reduction(min : x))
+#define ZJO_PAR_FOR_SUM_INT(x) ZJO_PRAGMA(omp parallel for schedule(static) reduction(+ : x))
+#else
+#define ZJO_PAR_FOR
+#define ZJO_PAR_FOR_MAXMIN(a, b, n)
+#define ZJO_PAR_FOR_MIN(a, n)
+#define ZJO_PAR_FOR_MIN_SCALAR(x)
+#define ZJO_PAR_FOR_SUM_INT(x)
+#endif
+
+int zjo_set_threads(int n)
+{
+#ifdef ZJO_OMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 #define NV 4 /* b, h, hu, hv */
 #define IB 0
@@ -68,6 +104,7 @@ void zjo_lsq_grad(const zjo_mesh *m, const double *u, const double *ubf,
                   double *grad)
 {
     const int N = m->n_cells, S = m->max_st;
+    ZJO_PAR_FOR
     for (int c = 0; c < N; ++c) {
         double gx = 0.0, gy = 0.0, uc = u[c];
         for (int j = 0; j < S; ++j) {
@@ -174,7 +211,24 @@ static void limited_slopes(const zjo_mesh *m, const zjo_params *p,
         double *gr = grads + (size_t)v * 2 * N;
         double *ph = phi + (size_t)v * N;
         zjo_lsq_grad(m, u, bW + (size_t)v * nbf, gr);
+        ZJO_PAR_FOR
         for (int c = 0; c < N; ++c) { umax[c] = u[c]; umin[c] = u[c]; }
+#ifdef ZJO_OMP
+        if (m->cell_faces) {
+            /* the same maxima and minima gathered per cell */
+            ZJO_PAR_FOR
+            for (int c = 0; c < N; ++c)
+                for (int k = 0; k < m->max_cf; ++k) {
+                    int f = m->cell_faces[(size_t)c * m->max_cf + k];
+                    if (f < 0) continue;
+                    int A = m->face_cells[f], B = m->face_cells[F + f];
+                    if (B < 0) continue;
+                    int o = A == c ? B : A;
+                    umax[c] = dmax(umax[c], u[o]); umin[c] = dmin(umin[c], u[o]);
+                }
+        } else
+#endif
+        ZJO_PAR_FOR_MAXMIN(umax, umin, N)
         for (int f = 0; f < F; ++f) {
             int A = m->face_cells[f], B = m->face_cells[F + f];
             if (B < 0) continue;
@@ -190,8 +244,30 @@ static void limited_slopes(const zjo_mesh *m, const zjo_params *p,
                             : 2.0 * bW[(size_t)v * nbf + b] - u[c];
             umax[c] = dmax(umax[c], gv); umin[c] = dmin(umin[c], gv);
         }
+        ZJO_PAR_FOR
         for (int c = 0; c < N; ++c) ph[c] = 1.0;
         if (p->limiter == ZJO_LIM_NONE) continue;
+#ifdef ZJO_OMP
+        if (m->cell_faces) {
+            /* the same face candidates, minimised per cell */
+            ZJO_PAR_FOR
+            for (int c = 0; c < N; ++c)
+                for (int k = 0; k < m->max_cf; ++k) {
+                    int f = m->cell_faces[(size_t)c * m->max_cf + k];
+                    if (f < 0) continue;
+                    double rx = m->face_centers[f] - m->cell_centers[c];
+                    double ry = m->face_centers[F + f] - m->cell_centers[N + c];
+                    double d = gr[c] * rx + gr[N + c] * ry;
+                    double ev2 = p->limiter == ZJO_LIM_VK
+                                     ? pow(m->cell_volumes[c], 2.0 / (m->dim > 1 ? m->dim : 1))
+                                     : 0.0; /* read by VK only */
+                    double cand = lim_cand(p->limiter, d, umax[c] - u[c],
+                                           umin[c] - u[c], ev2);
+                    if (cand < ph[c]) ph[c] = cand;
+                }
+        } else
+#endif
+        ZJO_PAR_FOR_MIN(ph, N)
         for (int f = 0; f < F; ++f) {
             int cells[2] = {m->face_cells[f], m->face_cells[F + f]};
             for (int side = 0; side < 2; ++side) {
@@ -200,12 +276,15 @@ static void limited_slopes(const zjo_mesh *m, const zjo_params *p,
                 double rx = m->face_centers[f] - m->cell_centers[c];
                 double ry = m->face_centers[F + f] - m->cell_centers[N + c];
                 double d = gr[c] * rx + gr[N + c] * ry;
-                double ev2 = pow(m->cell_volumes[c], 2.0 / (m->dim > 1 ? m->dim : 1));
+                double ev2 = p->limiter == ZJO_LIM_VK
+                                     ? pow(m->cell_volumes[c], 2.0 / (m->dim > 1 ? m->dim : 1))
+                                     : 0.0; /* read by VK only */
                 double cand = lim_cand(p->limiter, d, umax[c] - u[c],
                                        umin[c] - u[c], ev2);
                 if (cand < ph[c]) ph[c] = cand;
             }
         }
+        ZJO_PAR_FOR
         for (int c = 0; c < N; ++c) ph[c] = clip01(ph[c]);
     }
     free(umax);
@@ -235,6 +314,7 @@ static void pp_theta(const zjo_mesh *m, const double *hbar, const double *s,
     const int N = m->n_cells, F = m->n_faces;
     double *hmin = malloc(sizeof(double) * N);
     for (int c = 0; c < N; ++c) hmin[c] = INFINITY;
+    ZJO_PAR_FOR_MIN(hmin, N)
     for (int f = 0; f < F; ++f) {
         double fx = m->face_centers[f], fy = m->face_centers[F + f];
         for (int side = 0; side < 2; ++side) {
@@ -257,6 +337,7 @@ static void face_values(const zjo_mesh *m, const double *W,
                         double *WR, const double *bW)
 {
     const int N = m->n_cells, F = m->n_faces, nbf = m->n_bf;
+    ZJO_PAR_FOR
     for (int f = 0; f < F; ++f) {
         int A = m->face_cells[f], B = m->face_cells[F + f];
         double fx = m->face_centers[f], fy = m->face_centers[F + f];
@@ -583,6 +664,7 @@ static void flux_operator_h(const zjo_mesh *m, const zjo_params *p,
     double *QR = malloc(sizeof(double) * NV * F);
     double *hinv = malloc(sizeof(double) * N);
     zjo_reconstruct(m, p, Q, QL, QR, NULL);
+    ZJO_PAR_FOR
     for (int c = 0; c < N; ++c) hinv[c] = zjo_hinv(H[N + c], p->eps_wet);
     memset(dQ, 0, sizeof(double) * NV * N);
     /* boundary Q_R = BC(Q_L reconstructed) or the periodic partner state */
@@ -601,57 +683,87 @@ static void flux_operator_h(const zjo_mesh *m, const zjo_params *p,
         for (int v = 0; v < NV; ++v) QR[(size_t)v * F + f] = o[v];
     }
     int use_ces = p->order >= 2;
-    /* interior faces: A side, then B side (solver_jax.py:806-809) */
+    /* per-face terms (parallel), then the scatter in face order (serial):
+     * interior faces A side, then B side (solver_jax.py:806-809), each face's
+     * flux before its cell-edge terms; boundary faces to the owner only
+     * (solver_jax.py:827-883) */
+    double *tA = malloc(sizeof(double) * NV * F);   /* (F + Dm) |f| / |A|   */
+    double *tB = malloc(sizeof(double) * NV * F);   /* (-F + Dp) |f| / |B|  */
+    double *eA = use_ces ? malloc(sizeof(double) * NV * F) : NULL;
+    double *eB = use_ces ? malloc(sizeof(double) * NV * F) : NULL;
+    ZJO_PAR_FOR
     for (int f = 0; f < F; ++f) {
         int A = m->face_cells[f], B = m->face_cells[F + f];
-        if (B < 0) continue;
         double n[2] = {m->face_normals[f], m->face_normals[F + f]};
         double ql[NV], qr[NV], Fx[NV], Dp[NV], Dm[NV];
         for (int v = 0; v < NV; ++v) {
             ql[v] = QL[(size_t)v * F + f];
             qr[v] = QR[(size_t)v * F + f];
         }
-        zjo_face_flux(p, ql, qr, hinv[A], hinv[B], n, Fx, Dp, Dm);
+        zjo_face_flux(p, ql, qr, hinv[A], B >= 0 ? hinv[B] : hinv[A], n, Fx, Dp, Dm);
         double fa = m->face_volumes[f] / m->cell_volumes[A];
-        double fb = m->face_volumes[f] / m->cell_volumes[B];
+        double fb = B >= 0 ? m->face_volumes[f] / m->cell_volumes[B] : 0.0;
         for (int v = 0; v < NV; ++v) {
-            dQ[(size_t)v * N + A] -= (Fx[v] + Dm[v]) * fa;
-            dQ[(size_t)v * N + B] -= (-Fx[v] + Dp[v]) * fb;
+            tA[(size_t)v * F + f] = (Fx[v] + Dm[v]) * fa;
+            tB[(size_t)v * F + f] = (-Fx[v] + Dp[v]) * fb;
         }
         if (use_ces) {
             double qa[NV], qb[NV], sa[NV], sb[NV], nm[2] = {-n[0], -n[1]};
-            for (int v = 0; v < NV; ++v) {
-                qa[v] = Q[(size_t)v * N + A];
-                qb[v] = Q[(size_t)v * N + B];
-            }
+            for (int v = 0; v < NV; ++v) qa[v] = Q[(size_t)v * N + A];
             cell_edge_source(p, qa, ql, n, sa);
-            cell_edge_source(p, qb, qr, nm, sb);
-            for (int v = 0; v < NV; ++v) {
-                dQ[(size_t)v * N + A] -= sa[v] * fa;
-                dQ[(size_t)v * N + B] -= sb[v] * fb;
+            for (int v = 0; v < NV; ++v) eA[(size_t)v * F + f] = sa[v] * fa;
+            if (B >= 0) {
+                for (int v = 0; v < NV; ++v) qb[v] = Q[(size_t)v * N + B];
+                cell_edge_source(p, qb, qr, nm, sb);
+                for (int v = 0; v < NV; ++v) eB[(size_t)v * F + f] = sb[v] * fb;
             }
         }
     }
-    /* boundary faces: owner only (solver_jax.py:827-883) */
+#ifdef ZJO_OMP
+    if (m->cell_faces) {
+        /* per cell over its faces in increasing id (interior faces precede
+         * boundary faces): the serial scatter's order of operations */
+        ZJO_PAR_FOR
+        for (int c = 0; c < N; ++c)
+            for (int k = 0; k < m->max_cf; ++k) {
+                int f = m->cell_faces[(size_t)c * m->max_cf + k];
+                if (f < 0) continue;
+                const double *t = m->face_cells[f] == c ? tA : tB;
+                const double *e = m->face_cells[f] == c ? eA : eB;
+                for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + c] -= t[(size_t)v * F + f];
+                if (use_ces)
+                    for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + c] -= e[(size_t)v * F + f];
+            }
+        goto scattered;
+    }
+#endif
+    for (int f = 0; f < F; ++f) {
+        int A = m->face_cells[f], B = m->face_cells[F + f];
+        if (B < 0) continue;
+        for (int v = 0; v < NV; ++v) {
+            dQ[(size_t)v * N + A] -= tA[(size_t)v * F + f];
+            dQ[(size_t)v * N + B] -= tB[(size_t)v * F + f];
+        }
+        if (use_ces)
+            for (int v = 0; v < NV; ++v) {
+                dQ[(size_t)v * N + A] -= eA[(size_t)v * F + f];
+                dQ[(size_t)v * N + B] -= eB[(size_t)v * F + f];
+            }
+    }
     for (int b = 0; b < nbf; ++b) {
         int f = m->bface_face[b];
         int A = m->face_cells[f];
-        double n[2] = {m->face_normals[f], m->face_normals[F + f]};
-        double ql[NV], qr[NV], Fx[NV], Dp[NV], Dm[NV];
-        for (int v = 0; v < NV; ++v) {
-            ql[v] = QL[(size_t)v * F + f];
-            qr[v] = QR[(size_t)v * F + f];
-        }
-        zjo_face_flux(p, ql, qr, hinv[A], hinv[A], n, Fx, Dp, Dm);
-        double fa = m->face_volumes[f] / m->cell_volumes[A];
-        for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + A] -= (Fx[v] + Dm[v]) * fa;
-        if (use_ces) {
-            double qa[NV], sa[NV];
-            for (int v = 0; v < NV; ++v) qa[v] = Q[(size_t)v * N + A];
-            cell_edge_source(p, qa, ql, n, sa);
-            for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + A] -= sa[v] * fa;
-        }
+        for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + A] -= tA[(size_t)v * F + f];
+        if (use_ces)
+            for (int v = 0; v < NV; ++v) dQ[(size_t)v * N + A] -= eA[(size_t)v * F + f];
     }
+#ifdef ZJO_OMP
+scattered:
+#endif
+    free(tA);
+    free(tB);
+    free(eA);
+    free(eB);
     free(QL);
     free(QR);
     free(hinv);
@@ -667,6 +779,7 @@ double zjo_compute_dt(const zjo_mesh *m, const zjo_params *p, const double *Q,
 {
     const int N = m->n_cells, F = m->n_faces;
     double dt = INFINITY;
+    ZJO_PAR_FOR_MIN_SCALAR(dt)
     for (int f = 0; f < F; ++f) {
         int A = m->face_cells[f], B = m->face_cells[F + f];
         if (B < 0) B = A;
@@ -680,7 +793,7 @@ double zjo_compute_dt(const zjo_mesh *m, const zjo_params *p, const double *Q,
                         max_abs_eig(p, qb, zjo_hinv(qb[IH], p->eps_wet), n));
         double r = dmin(m->cell_inradius[A], m->cell_inradius[B]);
         double loc = p->cfl * 2.0 * r / ((double)m->dim * a);
-        if (loc < dt) dt = loc;
+        dt = dmin(loc, dt);
     }
     if (p->dt_max > 0.0) dt = dmin(dt, p->dt_max);
     if (t_remaining >= 0.0) dt = dmin(dt, t_remaining);
@@ -702,6 +815,7 @@ void zjo_manning_source(const zjo_params *p, const double *q, double hinv,
 static int count_troubled(int N, const double *Q)
 {
     int bad = 0;
+    ZJO_PAR_FOR_SUM_INT(bad)
     for (int c = 0; c < N; ++c) {
         int t = Q[(size_t)N + c] < 0.0;
         for (int v = 0; v < NV; ++v) t |= !isfinite(Q[(size_t)v * N + c]);
@@ -720,12 +834,14 @@ int zjo_step(const zjo_mesh *m, const zjo_params *p, double dt,
     size_t n = (size_t)NV * N;
     double *dQ = malloc(sizeof(double) * n);
     zjo_flux_operator(m, p, Q, dQ);
+    ZJO_PAR_FOR
     for (size_t i = 0; i < n; ++i) Qout[i] = Q[i] + dt * dQ[i];
     if (p->order >= 2) {
         double *Q1 = malloc(sizeof(double) * n);
         memcpy(Q1, Qout, sizeof(double) * n);
         flux_operator_h(m, p, Q1,
                         p->stage_aux == ZJO_STAGE_AUX_FROZEN ? Q : Q1, dQ);
+        ZJO_PAR_FOR
         for (size_t i = 0; i < n; ++i) {
             double q2 = Q1[i] + dt * dQ[i];
             Qout[i] = 0.5 * (Q[i] + q2);
@@ -733,6 +849,7 @@ int zjo_step(const zjo_mesh *m, const zjo_params *p, double dt,
         free(Q1);
     }
     if (p->manning_n > 0.0) {
+        ZJO_PAR_FOR
         for (int c = 0; c < N; ++c) {
             double q[NV], S[NV];
             for (int v = 0; v < NV; ++v) q[v] = Qout[(size_t)v * N + c];
@@ -785,6 +902,7 @@ void zjo_update_aux(const zjo_mesh *m, const zjo_params *p, const double *Q,
     zjo_lsq_grad(m, Q + (size_t)IB * N, NULL, gr);
     for (int d = 0; d < D; ++d)
         memcpy(Qaux + (size_t)row++ * N, gr + (size_t)d * N, sizeof(double) * N);
+    ZJO_PAR_FOR
     for (int c = 0; c < N; ++c)
         Qaux[(size_t)row * N + c] = zjo_hinv(Q[(size_t)N + c], p->eps_wet);
     free(gr);

@@ -61,6 +61,10 @@ typedef struct {
     const double *lsq_wg;        /* (N, 2, 2) ghost weights [slot][comp]   */
     const int *bface_mask;       /* (n_bf) Dirichlet rows (bit v), or NULL */
     const double *bface_value;   /* (4, n_bf) Dirichlet values, or NULL    */
+    /* (N, max_cf) faces of each cell, -1 padded, or NULL: the OpenMP build
+     * turns the face scatters of min/max into per-cell gathers with it */
+    int max_cf;
+    const int *cell_faces;
 } zjo_mesh;
 
 typedef struct {
@@ -125,6 +129,10 @@ void zjo_imex_local(const zjo_mesh *m, const zjo_params *p, double dt,
 
 /* March n_steps adaptive steps (or to t_end when n_steps < 0).  Returns the
  * number of steps taken; *t_io is advanced. */
+/* Threads of the OpenMP build (n <= 0: leave as is); returns the count in
+ * effect (1 for the serial build). */
+int zjo_set_threads(int n);
+
 int zjo_march(const zjo_mesh *m, const zjo_params *p, double *Q, double *t_io,
               double t_end, int n_steps);
 

@@ -585,3 +585,28 @@ def test_front_detector_demotes_exactly_the_capped_cells():
     const, plain_const = flat(Sf), flat(Sp)
     assert changed.sum() >= 3
     np.testing.assert_array_equal(const & ~plain_const, changed & ~plain_const)
+
+
+@pytest.mark.parametrize("recon,kw", [
+    ("conservative", dict(limiter="minmod", riemann="hll")),
+    ("eta", dict(limiter="vanleer", riemann="hll", positivity="zhang_shu",
+                 front_tol=0.5)),
+    ("conservative", dict(limiter="venkatakrishnan", riemann="rusanov_nc",
+                          manning_n=0.03))])
+def test_openmp_build_is_bit_identical(recon, kw):
+    """The OpenMP build of the oracle (the CPU baseline) runs only
+    order-independent loops in parallel: its march equals the serial
+    checker's bit for bit at any thread count."""
+    from helpers import wetdry_quads
+    from oracle import oracle as OO
+    mesh, Q0 = wetdry_quads(40)
+    Q0[2:] += 0.01
+    o1 = Oracle(mesh, order=2, recon=recon, cfl=0.2, **kw)
+    Qs, ts, _ = o1.march(Q0, n_steps=5)
+    for threads in (1, 3, 8):
+        assert OO.set_threads(threads) >= 1
+        op = Oracle(mesh, omp=True, order=2, recon=recon, cfl=0.2, **kw)
+        Qp, tp, _ = op.march(Q0, n_steps=5)
+        assert tp == ts
+        np.testing.assert_array_equal(Qp, Qs)
+        np.testing.assert_array_equal(op.aux(Qp), o1.aux(Qs))
