@@ -29,6 +29,12 @@ PEAK_HBM = 8.0e12           # MI355X HBM3E, bytes/s
 NX = NY = 2048
 
 
+def log(msg):
+    """Progress of the legs, on stderr (stdout carries the one JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr,
+          flush=True)
+
+
 def algorithmic_bytes(w):
     """Algorithmic bytes per cell, quad mesh (k = 4 face neighbours, F/N = 2):
     slope pass 28w + 16 B, face pass 30w + 16 B (+ 3w re-read of the step
@@ -94,26 +100,53 @@ def rel_l2_rows(A, B, rows=(1, 2, 3)):
                             max(np.linalg.norm(B[r]), 1e-300)) for r in rows}
 
 
-def cpu_leg(budget_s=12.0):
-    """The FP64 C oracle (scalar, 1 thread) on a bounded sample: a 256 x 256
-    mesh of the same workload, as many steps as fit in ~budget_s."""
+def cpu_model():
+    """CPU model string of the host (first 'model name' of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_leg(runs=5):
+    """The CPU baseline: the FP64 C oracle built with OpenMP (the same source
+    as the serial checker, bit-identical results) on a bounded sample of the
+    same workload -- the 256 x 256 mesh, a fixed number of steps per run
+    (dt reduction, both RK stages, aux refresh) -- on every core of this
+    process's affinity set and on one thread; one warm-up run, then the
+    median of `runs` timed runs each."""
     import numpy as np
-    from oracle.oracle import Oracle
+    from oracle import oracle as OO
     from zoomy_hip.mesh import Mesh
     m = Mesh.quad(256, 256).set_bc("wall")
-    Q = build_state(m)
-    o = Oracle(m, order=2, limiter="minmod", riemann="hll", cfl=0.9)
-    t0 = time.perf_counter()
-    k = 0
-    Q0 = Q
-    while True:
-        dt = o.dt(Q)
-        Q, _ = o.step(dt, Q)
-        o.aux(Q)
-        k += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
+    Q0 = build_state(m)
+    cores = len(os.sched_getaffinity(0))
+    out = {"cores": cores, "cpu_model": cpu_model(), "cells": m.n_cells}
+    Qref = None
+    for threads, steps in ((cores, 40), (1, 8)):
+        OO.set_threads(threads)
+        o = OO.Oracle(m, omp=True, order=2, limiter="minmod", riemann="hll",
+                      cfl=0.9)
+        times = []
+        for r in range(runs + 1):
+            Q = Q0
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                dt = o.dt(Q)
+                Q, _ = o.step(dt, Q)
+                o.aux(Q)
+            if r > 0:
+                times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        out[f"threads_{threads}" if threads == 1 else "threads_all"] = {
+            "threads": threads, "steps": steps, "runs_s": times,
+            "median_s": med, "value": steps * m.n_cells / med}
+        if threads == cores:
+            Qref, k = Q, steps
     # the FP64 device march over the same sample, checked against the oracle
     import torch
     from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
@@ -123,10 +156,10 @@ def cpu_leg(budget_s=12.0):
     s.setup_simulation(m, model)
     Qd = torch.as_tensor(Q0, dtype=torch.float64, device="cuda").contiguous()
     s.march_device(Qd, k)
-    err = rel_l2_rows(Qd.cpu().numpy(), Q)
+    out["gpu_vs_oracle"] = rel_l2_rows(Qd.cpu().numpy(), Qref)
+    out["steps_checked"] = k
     del s
-    return dict(value=k * m.n_cells / el, steps=k, cells=m.n_cells,
-                elapsed=el, gpu_vs_oracle=err)
+    return out
 
 
 def measured_traffic(w, kernel):
@@ -319,6 +352,11 @@ def wetdry_tri_bed(xc, seed=3):
     return f + 0.08 * (10.0 - x) + 0.1 * np.abs(y - 2.5)
 
 
+# algorithmic bytes per cell-update on a triangle mesh (k = 3 face
+# neighbours, F/N = 1.5): 104 words + 48 index bytes, FP32 words
+WETDRY_BYTES_F32 = 104 * 4 + 48
+
+
 def wetdry_tri_leg(steps, warmup):
     """The wet/dry triangle configuration: Delaunay of 2000 x 1000 jittered
     points on [0, 10] x [0, 5] (~4M triangles), the bed above, reservoir
@@ -353,7 +391,9 @@ def wetdry_tri_leg(steps, warmup):
                        "FP32 state, FP64 mass",
            "cells": base.n_cells, "steps": steps,
            "delaunay_and_rcm_s": delaunay_s}
+    log(f"triangulated {base.n_cells} cells in {delaunay_s:.1f} s")
     for name, perm in (("rcm", rcm), ("morton", None)):
+        log(f"wet/dry march in {name} order")
         t1 = time.perf_counter()
         mesh = base.reorder(base.morton_permutation() if perm is None
                             else perm).set_bc("wall")
@@ -389,7 +429,14 @@ def wetdry_tri_leg(steps, warmup):
                        "finite": bool(torch.isfinite(Q).all()),
                        "mood_steps_redone": redone,
                        "mood_cells_demoted": demoted}}
-        print(json.dumps({"wetdry_tri_" + name: out[name]}), flush=True)
+        B = WETDRY_BYTES_F32
+        out[name]["roofline"] = {
+            "bound": "hbm", "alg_bytes_per_cell_update": B,
+            "achieved": out[name]["value"] * B / 1e9, "unit": "GB/s",
+            "peak": PEAK_HBM / 1e9,
+            "frac": out[name]["value"] * B / PEAK_HBM}
+        print(json.dumps({"wetdry_tri_" + name: out[name]}), file=sys.stderr,
+              flush=True)
         del s, Q
     return out
 
@@ -406,6 +453,9 @@ def main():
     ap.add_argument("--no-chorin", action="store_true")
     ap.add_argument("--chorin-only", action="store_true")
     ap.add_argument("--no-diffusion", action="store_true")
+    ap.add_argument("--no-wetdry", action="store_true",
+                    help="skip the wet/dry triangle leg (~4M triangles, "
+                         "about a minute of host-side mesh set-up)")
     ap.add_argument("--wetdry-tri", action="store_true",
                     help="only the wet/dry triangle configuration (~4M "
                          "triangles; about a minute of host-side mesh set-up)")
@@ -436,6 +486,7 @@ def main():
         return
     chorin = None
     if not args.no_chorin:
+        log("chorin legs (plain, mixed, learned, learned_guess)")
         chorin = {"workload": f"{CHORIN_N}x{CHORIN_N} solitary wave "
                               "(amplitude 0.2, depth 1) over a seeded "
                               "sinusoidal bed, fixed dt, SSP-RK2 projection, "
@@ -482,6 +533,7 @@ def main():
     mesh = mesh.reorder(mesh.morton_permutation()).set_bc("wall")
     Q0 = build_state(mesh)
     N = mesh.n_cells
+    log("FP64 and FP32 marches on the 2048x2048 quad mesh")
     r64 = None if args.no_fp64 else gpu_leg(mesh, Q0, torch.float64,
                                              args.steps, args.warmup)
     r32 = None if args.no_fp32 else gpu_leg(mesh, Q0, torch.float32,
@@ -491,6 +543,8 @@ def main():
         print(json.dumps({"fp32_only": True, "value": cups32,
                           "roofline": roof32, "kernel_ms": r32["ms"]}))
         return
+    if not args.no_cpu:
+        log("CPU baseline (OpenMP oracle)")
     cpu = None if args.no_cpu else cpu_leg()
 
     def summarize(r, w):
@@ -522,9 +576,14 @@ def main():
     if cpu is not None:
         out["checks"]["gpu_vs_oracle_rel_l2"] = cpu["gpu_vs_oracle"]
         out["checks"]["gpu_vs_oracle_sample"] = (
-            f"256x256 sample of the same workload, {cpu['steps']} steps")
+            f"256x256 sample of the same workload, {cpu['steps_checked']} "
+            f"steps")
     if not args.no_diffusion:
+        log("diffusion leg")
         out["diffusion"] = diffusion_leg(mesh, Q0)
+    if not args.no_wetdry:
+        log("wet/dry triangle leg (Delaunay of 2M points, RCM and Morton)")
+        out["wetdry_tri"] = wetdry_tri_leg(args.steps, args.warmup)
     if r32 is not None:
         cups32, roof32 = summarize(r32, 4)
         out["fp32"] = {"value": cups32,
@@ -538,11 +597,19 @@ def main():
     if chorin is not None:
         out["chorin"] = chorin
     if cpu is not None:
+        ca, c1 = cpu["threads_all"], cpu["threads_1"]
         out["cpu_baseline"] = {
-            "value": cpu["value"], "unit": "cell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"FP64 C oracle, same workload on a 256x256 mesh, "
-                      f"{cpu['steps']} steps in {cpu['elapsed']:.1f} s"}
+            "value": ca["value"], "unit": "cell-updates/s",
+            "cores": cpu["cores"], "kind": "port",
+            "cpu_model": cpu["cpu_model"],
+            "value_1_thread": c1["value"],
+            "sample": f"FP64 C oracle (OpenMP build, bit-identical to the "
+                      f"serial checker), same workload on a 256x256 mesh: "
+                      f"median of 5 runs of {ca['steps']} steps on "
+                      f"{cpu['cores']} threads ({ca['median_s']:.2f} s) and "
+                      f"of {c1['steps']} steps on 1 thread "
+                      f"({c1['median_s']:.2f} s), each after a warm-up run",
+            "runs_s": {"all": ca["runs_s"], "one": c1["runs_s"]}}
     print(json.dumps(out))
 
 

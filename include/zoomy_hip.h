@@ -164,6 +164,72 @@ int zj_march(zj_solver *s, void *Q, void *Qaux, int32_t n_steps,
              double t_end, double *t_dev, double *dt_dev,
              int32_t *troubled_out, void *stream);
 
+/* flux_operator with the per-cell force_o1 demotion mask
+ * (solver_jax.py:724-785): cells with force_o1[c] != 0 reconstruct at
+ * first order; force_o1 (device uint8, N) may be NULL. */
+int zj_flux_operator_o1(zj_solver *s, const void *Q, const uint8_t *force_o1,
+                        void *dQ, void *stream);
+
+/* ───────────── the block contract (zoomy_jax/fvm/solver2/blocks.py) ─────────────
+ * Separate kernels on stored per-face arrays, in the face order of the
+ * reference's mesh (face_cells owner / neighbour, -1 on boundary faces).
+ * Arrays are device arrays of the solver's dtype unless noted. */
+/* The reference-layout face list: face f joins cell face_cells[f] (its slot
+ * face_slot[f]) and face_cells[n_faces + f] (slot face_slot[n_faces + f]),
+ * -1 for a boundary face; every cell slot must be named exactly once
+ * (host arrays, checked; ZJ_EINVAL otherwise).  Needed by the calls below
+ * (MeshJAX.face_cells, zoomy_jax/mesh/mesh.py:123-169). */
+int zj_mesh_set_faces(zj_mesh *m, int32_t n_faces, const int32_t *face_cells,
+                      const int32_t *face_slot);
+/* dt_pass (blocks.py:36-61) and get_compute_max_abs_eigenvalue
+ * (solver_jax.py:434-457): per face the lower and upper bounds lo, hi (F) of
+ * the eigenvalues {0, u.n - c, u.n, u.n + c} of both cell states (the owner
+ * on both sides of a boundary face); max |lambda| = max(|lo|, |hi|).
+ * hinv_aux (N, may be NULL: from Q) is the aux 1/h row. */
+int zj_face_speeds(zj_solver *s, const void *Q, const void *hinv_aux, void *lo,
+                   void *hi, void *stream);
+/* reduce_dt (blocks.py:64-95): *dt_out = min_f CFL 2 r_f / (d (2k+1)
+ * max(|lo_f|, |hi_f|)), wave-free faces dropping out, capped by dt_max
+ * (> 0) and clamp (>= 0: the remaining time or the coupling window);
+ * inradius_f (F, FP64) = min(r_A, r_B). */
+int zj_reduce_dt(zj_solver *s, const void *lo, const void *hi,
+                 const double *inradius_f, double cfl, int32_t dimension,
+                 int32_t degree, double dt_max, double clamp, double *dt_out,
+                 void *stream);
+/* halo_bc (blocks.py:129-157): bf_values (4, n_bf) = the BC ghost of each
+ * boundary face's cell state. */
+int zj_bc_values(zj_solver *s, const void *Q, void *bf_values, void *stream);
+/* The reconstruction object (reconstruction_jax.py:398-429; eta:
+ * :1307-1366): face states QL, QR (4, F); Q_R of a boundary face is the
+ * ghost of the cell state (the halo_bc values).  force_o1 (uint8, N, may be
+ * NULL) demotes cells to first order; order1 != 0 gives the constant
+ * reconstruction (MOOD redo).  lim_grad (4, dim, N, may be NULL) receives
+ * the limited cell gradient in conservative variables (reconstruct_with_grad;
+ * the eta reconstruction's W slopes pushed through the inverse map). */
+int zj_reconstruct(zj_solver *s, const void *Q, const uint8_t *force_o1,
+                   int32_t order1, void *QL, void *QR, void *lim_grad,
+                   void *stream);
+/* flux_pass (blocks.py:185-238): F, Dp, Dm (4, F) per face; a boundary
+ * face's Q_R is recomputed as BC(Q_L) (the partner cell at a periodic seam);
+ * hinv_aux (N, may be NULL: from Q) is the aux 1/h the fluxes read. */
+int zj_flux_pass(zj_solver *s, const void *Q, const void *hinv_aux,
+                 const void *QL, const void *QR, void *Fface, void *Dp,
+                 void *Dm, void *stream);
+/* gather_update (blocks.py:241-301): Qcand = Q0 + a dt (-div(F + D) +
+ * cell term + source) gathered per cell over its faces; the cell term is
+ * cell_term (4, N) or, when that is NULL and lim_grad (4, dim, N) is given,
+ * -B(Q0) . lim_grad (the interior nonconservative integral, march.py
+ * :144-149); source != 0 adds the explicit Manning friction of Q0.
+ * troubled (uint8, N, may be NULL) flags h < 0 or a non-finite component;
+ * *count (device int32, may be NULL) is incremented by their number.
+ * dt: device scalar. */
+int zj_gather_update(zj_solver *s, const void *Q0, const void *Fface,
+                     const void *Dp, const void *Dm, const double *dt,
+                     double a_stage, const void *cell_term,
+                     const void *lim_grad, int32_t source,
+                     const void *hinv_aux, void *Qcand, uint8_t *troubled,
+                     int32_t *count, void *stream);
+
 /* Profiling: average duration (ms) of each kernel of a step over `reps`
  * launches, timed with HIP events on `stream`: ms_out[5] = {stage 1, stage 2
  * (+ RK average, source, troubled flag, dt of the new state), standalone dt,

@@ -511,12 +511,34 @@ void zjo_reconstruct(const zjo_mesh *m, const zjo_params *p, const double *Q,
         for (int b = 0; b < nbf; ++b)
             for (int v = 0; v < NV; ++v)
                 QR[(size_t)v * F + m->bface_face[b]] = bfv[(size_t)v * nbf + b];
+        if (lim_grad) {
+            /* reconstruct_with_grad (:1313-1366): the limited W slopes
+             * pushed to conservative rows through the inverse map at the
+             * cell, dh = deta - db, d(u h) = h du + u dh (eta); the
+             * conservative momentum rows of eta_hu stay as they are */
+            for (int d = 0; d < 2; ++d)
+                for (int c = 0; c < N; ++c) {
+                    double s[NV];
+                    for (int v = 0; v < NV; ++v)
+                        s[v] = phi[(size_t)v * N + c] * grads[((size_t)v * 2 + d) * N + c];
+                    double dh = s[1] - s[0];
+                    if (vel) {
+                        double hc = W[N + c] - W[c];
+                        for (int v = 2; v < NV; ++v) s[v] = hc * s[v] + W[(size_t)v * N + c] * dh;
+                    }
+                    s[1] = dh;
+                    for (int v = 0; v < NV; ++v) lim_grad[((size_t)v * 2 + d) * N + c] = s[v];
+                }
+            free(grads);
+            free(phi);
+            grads = phi = NULL;
+        }
         free(WL);
         free(WR);
         free(W);
         free(bW);
     }
-    if (lim_grad) {
+    if (lim_grad && grads) {
         for (int v = 0; v < NV; ++v)
             for (int d = 0; d < 2; ++d)
                 for (int c = 0; c < N; ++c)

@@ -43,6 +43,22 @@ def test_argument_checks_before_any_device_work():
     assert L.zj_diffusion_destroy(null) == _lib.ZJ_OK
     assert L.zj_solver_set_stage_aux(null, 0) == _lib.ZJ_EINVAL
     assert L.zj_mesh_create(None, null) == _lib.ZJ_EINVAL
+    # the block contract and the per-face entry points
+    assert L.zj_mesh_set_faces(null, 4, null, null) == _lib.ZJ_EINVAL
+    assert L.zj_face_speeds(null, null, null, null, null,
+                            null) == _lib.ZJ_EINVAL
+    assert L.zj_reduce_dt(null, null, null, null, 0.9, 2, 0, -1.0, -1.0,
+                          null, null) == _lib.ZJ_EINVAL
+    assert L.zj_bc_values(null, null, null, null) == _lib.ZJ_EINVAL
+    assert L.zj_reconstruct(null, null, null, 0, null, null, null,
+                            null) == _lib.ZJ_EINVAL
+    assert L.zj_flux_pass(null, null, null, null, null, null, null, null,
+                          null) == _lib.ZJ_EINVAL
+    assert L.zj_gather_update(null, null, null, null, null, null, 1.0, null,
+                              null, 0, null, null, null, null,
+                              null) == _lib.ZJ_EINVAL
+    assert L.zj_flux_operator_o1(null, null, null, null,
+                                 null) == _lib.ZJ_EINVAL
 
 
 def test_dirichlet_descriptor_validation():

@@ -157,6 +157,70 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
     out[3] = T(0.5) * ((qL[3] * unL + pL * ny) + (qR[3] * unR + pR * ny)) + T(0.5) * (Bd * ny - s * d3);
 }
 
+// The same face numerics as face_flux_left, kept apart as the reference's
+// stored triple (blocks.py:185-238): conservative flux F and the two
+// fluctuations Dp (B side) and Dm (A side), with F + Dm the A-side update
+// and -F + Dp the B-side one.  hiL / hiR: the cell aux 1/h of A and B.
+template <typename T>
+__device__ __forceinline__ void face_split(int rs, const T* qL, const T* qR, T hiL, T hiR,
+                                           T nx, T ny, T g, T eps, T* F, T* Dp, T* Dm) {
+    if (rs == RS_HLL) {
+        T bs = tmax(qL[0], qR[0]);
+        T hLs = tmax(T(0), qL[1] + qL[0] - bs);
+        T hRs = tmax(T(0), qR[1] + qR[0] - bs);
+        T uL = qL[2] * hiL, vL = qL[3] * hiL;
+        T uR = qR[2] * hiR, vR = qR[3] * hiR;
+        T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+        T cL = sqrt(g * hLs), cR = sqrt(g * hRs);
+        T sL = tmin(unL - cL, unR - cR);
+        T sR = tmax(unL + cL, unR + cR);
+        T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
+        T FL[3] = {hLs * unL, hLs * uL * unL + pL * nx, hLs * vL * unL + pL * ny};
+        T FR[3] = {hRs * unR, hRs * uR * unR + pR * nx, hRs * vR * unR + pR * ny};
+        T UL[3] = {hLs, hLs * uL, hLs * vL};
+        T UR[3] = {hRs, hRs * uR, hRs * vR};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (sL >= T(0)) F[k + 1] = FL[k];
+            else if (sR <= T(0)) F[k + 1] = FR[k];
+            else F[k + 1] = (sR * FL[k] - sL * FR[k] + sL * sR * (UR[k] - UL[k])) / (sR - sL);
+        }
+        F[0] = T(0);
+        // hydrostatic pressure corrections g/2 (h^2 - h*^2) n
+        T cA = T(0.5) * g * (qL[1] * qL[1] - hLs * hLs);
+        T cB = T(0.5) * g * (qR[1] * qR[1] - hRs * hRs);
+        Dm[0] = Dm[1] = T(0);
+        Dm[2] = cA * nx;
+        Dm[3] = cA * ny;
+        Dp[0] = Dp[1] = T(0);
+        Dp[2] = -cB * nx;
+        Dp[3] = -cB * ny;
+        return;
+    }
+    T unL = (qL[2] * nx + qL[3] * ny) * hiL;
+    T unR = (qR[2] * nx + qR[3] * ny) * hiR;
+    T pL = T(0.5) * g * qL[1] * qL[1], pR = T(0.5) * g * qR[1] * qR[1];
+    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + sqrt(g * tmax(qL[1], T(0))),
+               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + sqrt(g * tmax(qR[1], T(0))));
+    T d[4] = {qR[0] - qL[0], qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
+    T Bd = g * (T(0.5) * (qL[1] + qR[1])) * d[0];
+    F[0] = T(0);
+    F[1] = T(0.5) * ((qL[2] * nx + qL[3] * ny) + (qR[2] * nx + qR[3] * ny));
+    F[2] = T(0.5) * ((qL[2] * unL + pL * nx) + (qR[2] * unR + pR * nx));
+    F[3] = T(0.5) * ((qL[3] * unL + pL * ny) + (qR[3] * unR + pR * ny));
+    T diss[4] = {s * d[0], s * d[1], s * d[2], s * d[3]};
+    if (rs == RS_RUSANOV) {
+        diss[0] = T(0);
+        diss[1] = s * (d[1] + d[0]);
+    }
+    const T B[4] = {T(0), T(0), Bd * nx, Bd * ny};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        Dm[v] = T(0.5) * (B[v] - diss[v]);
+        Dp[v] = T(0.5) * (B[v] + diss[v]);
+    }
+}
+
 // Cell-edge nonconservative integral g (h_c + h_e)/2 (b_e - b_c) n (q rows).
 template <typename T>
 __device__ __forceinline__ void cell_edge(const T* qc, const T* qe, T nx, T ny, T g,

@@ -586,18 +586,32 @@ __device__ __forceinline__ void lds_q4(const typename Vec2<T>::type* Qs, int ML,
     q[0] = a.x; q[1] = a.y; q[2] = b.x; q[3] = b.y;
 }
 
+// Outputs of the reconstruction-only instantiation (AUXONLY = 2): face
+// states (4, F) in the reference's face order, written through the per-slot
+// table cf = 2 f + side (side 1: the cell is the face's neighbour B), and the
+// limited cell gradients (4, dim, N) in conservative variables (or NULL).
+template <typename T>
+struct ReconOut {
+    T* QL;
+    T* QR;
+    T* lim;
+    const int* cf;
+    int F;
+};
+
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
 // HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
 // the options.  Both instantiations evaluate the same formulas.  AUXONLY = 1
-// is the post_step aux refresh alone: staging and the own cells' gradients.
+// is the post_step aux refresh alone: staging and the own cells' gradients;
+// AUXONLY = 2 the reconstruction object alone (face states, no fluxes).
 template <typename T, int SPEC, int AUXONLY = 0>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_stage(
     DevMesh m_in, DevPatch pt, Opts o_in, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part,
     const uint8_t* __restrict__ o1_mask, uint8_t* __restrict__ bad_out,
-    const int* __restrict__ gate) {
+    const int* __restrict__ gate, ReconOut<T> ro) {
     // gate: a MOOD redo launch runs only when the candidate had troubled cells
     if (gate && *gate == 0) return;
     DevMesh m = m_in;
@@ -682,7 +696,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     if (use_slopes) {
         // AUXONLY (post_step aux rows only, launched as mode 3): the own
         // cells' gradients, no limiter, no slopes, no face pass
-        const int nsl = AUXONLY ? nown : nslope;
+        const int nsl = AUXONLY == 1 ? nown : nslope;
         for (int ls = tid; ls < nsl; ls += SBLOCK) {
             const int gi = ls < nown ? c0 + ls : lids[ls];
             uint16_t nls[MAXF];
@@ -802,7 +816,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     aux_out[(size_t)(row++) * N + gi] = d == 0 ? ax[0] : ay[0];
                 aux_out[(size_t)row * N + gi] = hic;
             }
-            if (AUXONLY) continue;
+            if (AUXONLY == 1) continue;
             T phi[4] = {1, 1, 1, 1};
             if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
@@ -879,10 +893,84 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 sv.y = pv * gy[vv];
                 Ss[vv * MS + ls] = sv;
             }
+            if (AUXONLY == 2 && ro.lim && ls < nown) {
+                // limited gradient pushed back to conservative rows, the
+                // JVP of the inverse map at the cell (reconstruction_jax.py
+                // :1313-1366): dh = deta - db, d(hu) = h du + u dh
+                T sx[4], sy[4];
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) {
+                    const T pv = clip01(phi[vv]);
+                    sx[vv] = pv * gx[vv];
+                    sy[vv] = pv * gy[vv];
+                }
+                if (eta_rows) {
+                    const T dhx = sx[1] - sx[0], dhy = sy[1] - sy[0];
+                    if (o.recon == RECON_ETA) {
+                        const T hc = w[1] - w[0];
+#pragma unroll
+                        for (int vv = 2; vv < 4; ++vv) {
+                            sx[vv] = hc * sx[vv] + w[vv] * dhx;
+                            sy[vv] = hc * sy[vv] + w[vv] * dhy;
+                        }
+                    }
+                    sx[1] = dhx;
+                    sy[1] = dhy;
+                }
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) {
+                    ro.lim[(size_t)(vv * m.dim) * N + gi] = sx[vv];
+                    if (m.dim == 2) ro.lim[(size_t)(vv * 2 + 1) * N + gi] = sy[vv];
+                }
+            }
         }
         __syncthreads();
     }
-    if (AUXONLY) return;
+    if (AUXONLY == 1) return;
+    if (AUXONLY == 2) {
+        // the reconstruction object: each own cell's face values into the
+        // face arrays; a boundary face's Q_R is the ghost of the cell state
+        // (the bf values of halo_bc, blocks.py:129-157)
+        if (tid < nown) {
+            const int gi = c0 + tid;
+            T q[4];
+            lds_q4(Qs, ML, tid, q);
+            const T hic = Hs[tid];
+            T sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (use_slopes) {
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) {
+                    const T2 sv = Ss[vv * MS + tid];
+                    sc[2 * vv] = sv.x;
+                    sc[2 * vv + 1] = sv.y;
+                }
+            }
+            double2 v[MAXF];
+            double fx[MAXF], fy[MAXF];
+            face_centres(m, Vs, tv, MS, tid, v, fx, fy);
+            const double2 cc = Cs[tid];
+            ZJ_FOR_FACES(k) {
+                T qf[4] = {q[0], q[1], q[2], q[3]};
+                if (use_slopes)
+                    recon_face(o.recon, q, hic, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qf);
+                const int f2 = ro.cf[(size_t)k * N + gi];
+                const int f = f2 >> 1;
+                T* dst = (f2 & 1) ? ro.QR : ro.QL;
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) dst[(size_t)vv * ro.F + f] = qf[vv];
+                const int nb = m.nbr[(size_t)k * N + gi];
+                if (nb < 0) {
+                    double nxd, nyd, len;
+                    face_normal(m, v, k, nxd, nyd, len);
+                    T qg[4];
+                    ghost_state(m, Qin, -1 - nb, q, T(nxd), T(nyd), qg);
+#pragma unroll
+                    for (int vv = 0; vv < 4; ++vv) ro.QR[(size_t)vv * ro.F + f] = qg[vv];
+                }
+            }
+        }
+        return;
+    }
 
     // ── phase 3: fluxes of the own cells ─────────────────────────────
     int bad = 0;
@@ -1048,6 +1136,14 @@ struct zj_mesh {
     double* slot_r = nullptr;   // (mf, N) min(r_cell, r_nbr)
     int n_templates = 0;
     std::vector<void*> allocs;
+    int* bf_cell = nullptr;     // (nbf) inner cell of each boundary face
+    int* bf_slot = nullptr;     // (nbf) its slot in that cell
+    // reference-layout face list (zj_mesh_set_faces), for the per-face
+    // entry points
+    int nfaces = 0;
+    int* fc = nullptr;          // (2, F) owner A, neighbour B (-1: boundary)
+    int* fs = nullptr;          // (2, F) slot of the face in A, in B
+    int* cell_face = nullptr;   // (mf, N) 2 f + side of each cell slot
 };
 
 static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
@@ -1129,6 +1225,11 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         if (d->cell_extra[i] >= N) return ZJ_EINVAL;
     std::vector<int> pcell(d->n_bf > 0 ? d->n_bf : 1, -1);
     for (int b = 0; b < d->n_bf; ++b) {
+        // boundary face b sits in slot bface_slot[b] of cell bface_cell[b]
+        const int bc = d->bface_cell[b], bs = d->bface_slot[b];
+        if (bc < 0 || bc >= N || bs < 0 || bs >= mf ||
+            d->cell_nbr[(size_t)bs * N + bc] != -1 - b)
+            return ZJ_EINVAL;
         int k = d->bface_kind[b];
         if (k < 0 || k > 3) return ZJ_EINVAL;
         if (k == ZJ_BC_DIRICHLET && (!d->bface_mask || !d->bface_value ||
@@ -1167,6 +1268,10 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
                     val[(size_t)v * nb + b] = d->bface_value[(size_t)v * d->n_bf + b];
             }
         rc |= dev_upload(m, mask.data(), sizeof(int) * mask.size(), &p); m->d.bf_mask = (const int*)p;
+        rc |= dev_upload(m, d->n_bf ? d->bface_cell : nullptr, sizeof(int) * d->n_bf, &p);
+        m->bf_cell = (int*)p;
+        rc |= dev_upload(m, d->n_bf ? d->bface_slot : nullptr, sizeof(int) * d->n_bf, &p);
+        m->bf_slot = (int*)p;
         rc |= dev_upload(m, val.data(), sizeof(double) * val.size(), &p); m->d.bf_val = (const double*)p;
     }
     if (rc) {
@@ -1260,6 +1365,8 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
 int zj_mesh_destroy(zj_mesh* m) {
     if (!m) return ZJ_OK;
     for (void* p : m->allocs) (void)hipFree(p);
+    for (void* p : {(void*)m->fc, (void*)m->fs, (void*)m->cell_face})
+        if (p) (void)hipFree(p);
     delete m;
     return ZJ_OK;
 }
@@ -1336,18 +1443,19 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   int mode, int final_stage, int write_b, int use_slopes, int* troubled,
                   unsigned long long* dt_red, hipStream_t st,
                   const uint8_t* o1_mask = nullptr, uint8_t* bad_out = nullptr,
-                  const int* gate = nullptr) {
+                  const int* gate = nullptr, ReconOut<T> ro = ReconOut<T>{}) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                       s->o.recon == RECON_CONS && !s->o.stage_aux;
-    auto kern = mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
-                          : (spec ? k_stage<T, 1> : k_stage<T, 0>);
+    auto kern = mode == 4   ? k_stage<T, 0, 2>
+                : mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
+                            : (spec ? k_stage<T, 1> : k_stage<T, 0>);
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
-                       o1_mask, bad_out, gate);
+                       o1_mask, bad_out, gate, ro);
     if (dt_red)
         hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(M->pt.npatch * (SBLOCK / 64))),
                            dim3(1024), 0, st, (const double*)s->dt_part,
@@ -1774,3 +1882,402 @@ extern "C" int zj_compute_dt_noalloc(zj_solver* s, const void* Q, double t_remai
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
+
+// ─────────────────────────────────────────────────────────────────────────
+// The reference's block contract (solver2/blocks.py) as separate kernels on
+// stored per-face arrays: dt_pass (per-face wave-speed bounds), reduce_dt,
+// halo_bc (boundary ghost values), reconstruct (the reconstruction object),
+// flux_pass (F, Dp, Dm per face) and gather_update (per-cell gather over the
+// face-of-cell table, troubled flags).  The fused stage kernel computes the
+// same step without storing the face arrays; these serve callers that drive
+// the blocks themselves.  Faces are in the caller's reference-layout order
+// (zj_mesh_set_faces).
+// ─────────────────────────────────────────────────────────────────────────
+namespace {
+
+// Per-face bounds of the SWE eigenvalues {0, u.n - c, u.n, u.n + c} (the
+// bed row's 0 included) over both cell states, 1/h from the aux row when
+// given; a boundary face uses its owner on both sides (context.py:111-115).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_face_speeds(DevMesh m, Opts o, const T* __restrict__ Q,
+                                                       const T* __restrict__ hinv_aux,
+                                                       const int* __restrict__ fc,
+                                                       const int* __restrict__ fs, int F,
+                                                       const double2* __restrict__ sn,
+                                                       T* __restrict__ lo, T* __restrict__ hi) {
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= F) return;
+    const int N = m.N;
+    const int A = fc[f];
+    const int B = fc[F + f] >= 0 ? fc[F + f] : A;
+    const double2 n = sn[(size_t)fs[f] * N + A];
+    T l = T(0), h = T(0);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int c = side ? B : A;
+        T q[4];
+        load_q(Q, N, c, q);
+        const T hv = hinv_aux ? hinv_aux[c] : hinv_of(q[1], T(o.eps));
+        const T un = (q[2] * T(n.x) + q[3] * T(n.y)) * hv;
+        const T cs = sqrt(T(o.g) * tmax(q[1], T(0)));
+        l = tmin(l, un - cs);
+        h = tmax(h, un + cs);
+    }
+    lo[f] = l;
+    hi[f] = h;
+}
+
+// reduce_dt's per-face limit fac * 2 r_f / max(|lo|, |hi|) (fac = CFL /
+// (d (2k + 1))), block minima to part[]; a wave-free face gives +inf, a
+// non-finite speed 0 (so the dt guard raises).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_face_dt(const T* __restrict__ lo,
+                                                   const T* __restrict__ hi,
+                                                   const double* __restrict__ rf, int F,
+                                                   double fac, double* __restrict__ part) {
+    __shared__ double lds[BLOCK / 64];
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    double loc = INFINITY;
+    if (f < F) {
+        const double a = fmax(fabs((double)lo[f]), fabs((double)hi[f]));
+        loc = fac * 2.0 * rf[f] / a;
+        if (loc != loc || a != a) loc = 0.0;
+    }
+    double v = block_reduce(loc, [](double x, double y) { return fmin(x, y); }, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// halo_bc: the ghost of each boundary face's cell state (the periodic
+// partner's state at a seam, the prescribed rows of a Dirichlet face).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_bc_values(DevMesh m, const T* __restrict__ Q,
+                                                     const int* __restrict__ bcell,
+                                                     const int* __restrict__ bslot,
+                                                     const double2* __restrict__ sn,
+                                                     T* __restrict__ bfv) {
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= m.nbf) return;
+    const int c = bcell[b];
+    const double2 n = sn[(size_t)bslot[b] * m.N + c];
+    T q[4], g[4];
+    load_q(Q, m.N, c, q);
+    ghost_state(m, Q, b, q, T(n.x), T(n.y), g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bfv[(size_t)v * m.nbf + b] = g[v];
+}
+
+// flux_pass: one evaluation per face of the stored face states; a boundary
+// face's Q_R is recomputed as BC(Q_L) (the partner cell's state at a
+// periodic seam), blocks.py:207-229.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_flux_pass(DevMesh m, Opts o, const T* __restrict__ Q,
+                                                     const T* __restrict__ hinv_aux,
+                                                     const T* __restrict__ QL,
+                                                     const T* __restrict__ QR,
+                                                     const int* __restrict__ fc,
+                                                     const int* __restrict__ fs, int F,
+                                                     const double2* __restrict__ sn,
+                                                     T* __restrict__ Fo, T* __restrict__ Dpo,
+                                                     T* __restrict__ Dmo) {
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= F) return;
+    const int N = m.N;
+    const int A = fc[f], B = fc[F + f], sA = fs[f];
+    const double2 n = sn[(size_t)sA * N + A];
+    T qL[4], qR[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qL[v] = QL[(size_t)v * F + f];
+    if (B >= 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qR[v] = QR[(size_t)v * F + f];
+    } else {
+        ghost_state(m, Q, -1 - m.nbr[(size_t)sA * N + A], qL, T(n.x), T(n.y), qR);
+    }
+    const T eps = T(o.eps);
+    const T hA = hinv_aux ? hinv_aux[A] : hinv_of(Q[(size_t)N + A], eps);
+    const T hB = B < 0 ? hA : (hinv_aux ? hinv_aux[B] : hinv_of(Q[(size_t)N + B], eps));
+    T Fx[4], Dp[4], Dm[4];
+    face_split(o.rs, qL, qR, hA, hB, T(n.x), T(n.y), T(o.g), eps, Fx, Dp, Dm);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        Fo[(size_t)v * F + f] = Fx[v];
+        Dpo[(size_t)v * F + f] = Dp[v];
+        Dmo[(size_t)v * F + f] = Dm[v];
+    }
+}
+
+// gather_update: Q_cand = Q0 + a dt (-div(F + D) + cell_term + source) per
+// cell over its faces (blocks.py:241-287); the cell-interior term is the
+// given array or, from a limited gradient, -B(Q0) . grad with B[q_d, b] =
+// g h (march.py:144-149); the source is the explicit Manning friction of Q0
+// (1/h from the aux row when given).  troubled[c] = h < 0 or non-finite,
+// and their count is added to *count.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gather_update(
+    DevMesh m, Opts o, const T* __restrict__ Q0, const T* __restrict__ Fo,
+    const T* __restrict__ Dpo, const T* __restrict__ Dmo, const int* __restrict__ cface, int F,
+    const double* __restrict__ dtp, double a, const T* __restrict__ cell_term,
+    const T* __restrict__ lim_grad, int source, const T* __restrict__ hinv_aux,
+    T* __restrict__ Qc, uint8_t* __restrict__ troubled, int* __restrict__ count) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    int bad = 0;
+    if (c < m.N) {
+        const int N = m.N;
+        Geo G;
+        load_geo(m, c, G);
+        T q[4];
+        load_q(Q0, N, c, q);
+        T dQ[4] = {0, 0, 0, 0};
+        ZJ_FOR_FACES(k) {
+            const int f2 = cface[(size_t)k * N + c];
+            const int f = f2 >> 1;
+            const T r = T(G.len[k] / G.vol);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const T Fv = Fo[(size_t)v * F + f];
+                dQ[v] -= ((f2 & 1) ? (-Fv + Dpo[(size_t)v * F + f])
+                                   : (Fv + Dmo[(size_t)v * F + f])) * r;
+            }
+        }
+        if (cell_term) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dQ[v] += cell_term[(size_t)v * N + c];
+        } else if (lim_grad) {
+            const T gh = T(o.g) * q[1];
+            dQ[2] -= gh * lim_grad[(size_t)0 * N + c];
+            if (m.dim == 2) dQ[3] -= gh * lim_grad[(size_t)1 * N + c];
+        }
+        const T hv = hinv_aux ? hinv_aux[c] : hinv_of(q[1], T(o.eps));
+        if (source && o.manning > 0.0) {
+            const T u = q[2] * hv, w = q[3] * hv;
+            const T sp = sqrt(u * u + w * w);
+            const T kk = T(o.g * o.manning * o.manning) * cbrt(hv);
+            dQ[2] += -kk * u * sp;
+            dQ[3] += -kk * w * sp;
+        }
+        const T adt = T(a * *dtp);
+        T out[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            out[v] = q[v] + adt * dQ[v];
+            Qc[(size_t)v * N + c] = out[v];
+        }
+        bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
+              !isfinite(out[2]) || !isfinite(out[3]);
+        if (troubled) troubled[c] = (uint8_t)bad;
+    }
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
+    if (count && (threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
+static inline int fblocks(int n) { return (n + BLOCK - 1) / BLOCK; }
+
+}  // namespace
+
+extern "C" {
+
+int zj_mesh_set_faces(zj_mesh* m, int32_t n_faces, const int32_t* face_cells,
+                      const int32_t* face_slot) {
+    if (!m || n_faces <= 0 || !face_cells || !face_slot) return ZJ_EINVAL;
+    const int N = m->d.N, mf = m->d.mf, F = n_faces;
+    std::vector<int32_t> nbr((size_t)mf * N);
+    HIPCHK(hipMemcpy(nbr.data(), m->d.nbr, sizeof(int32_t) * nbr.size(), hipMemcpyDeviceToHost));
+    // every (cell, slot) must be named by exactly one face side, consistent
+    // with the mesh's neighbour table: the gathers then stay in bounds
+    std::vector<int32_t> cf((size_t)mf * N, -1);
+    for (int f = 0; f < F; ++f) {
+        const int A = face_cells[f], B = face_cells[F + f];
+        const int sA = face_slot[f], sB = face_slot[F + f];
+        if (A < 0 || A >= N || sA < 0 || sA >= mf || B >= N) return ZJ_EINVAL;
+        const int nA = nbr[(size_t)sA * N + A];
+        if (B >= 0) {
+            if (sB < 0 || sB >= mf || nA != B || nbr[(size_t)sB * N + B] != A) return ZJ_EINVAL;
+        } else if (nA >= 0) {
+            return ZJ_EINVAL;
+        }
+        int32_t& ca = cf[(size_t)sA * N + A];
+        if (ca >= 0) return ZJ_EINVAL;
+        ca = 2 * f;
+        if (B >= 0) {
+            int32_t& cb = cf[(size_t)sB * N + B];
+            if (cb >= 0) return ZJ_EINVAL;
+            cb = 2 * f + 1;
+        }
+    }
+    for (size_t i = 0; i < cf.size(); ++i)
+        if (cf[i] < 0) return ZJ_EINVAL;
+    int *fc = nullptr, *fs = nullptr, *cfd = nullptr;
+    if (hipMalloc((void**)&fc, sizeof(int) * 2 * F) != hipSuccess ||
+        hipMalloc((void**)&fs, sizeof(int) * 2 * F) != hipSuccess ||
+        hipMalloc((void**)&cfd, sizeof(int) * cf.size()) != hipSuccess) {
+        for (void* q : {(void*)fc, (void*)fs, (void*)cfd})
+            if (q) (void)hipFree(q);
+        return ZJ_ENOMEM;
+    }
+    if (hipMemcpy(fc, face_cells, sizeof(int) * 2 * F, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fs, face_slot, sizeof(int) * 2 * F, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(cfd, cf.data(), sizeof(int) * cf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        for (void* q : {(void*)fc, (void*)fs, (void*)cfd}) (void)hipFree(q);
+        return ZJ_EHIP;
+    }
+    for (void* q : {(void*)m->fc, (void*)m->fs, (void*)m->cell_face})
+        if (q) (void)hipFree(q);
+    m->fc = fc;
+    m->fs = fs;
+    m->cell_face = cfd;
+    m->nfaces = F;
+    return ZJ_OK;
+}
+
+int zj_face_speeds(zj_solver* s, const void* Q, const void* hinv_aux, void* lo, void* hi,
+                   void* stream) {
+    if (!s || !Q || !lo || !hi) return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (!M->fc) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_face_speeds<double>, dim3(fblocks(M->nfaces)), dim3(BLOCK), 0, st,
+                           M->d, s->o, (const double*)Q, (const double*)hinv_aux, M->fc, M->fs,
+                           M->nfaces, (const double2*)M->slot_n, (double*)lo, (double*)hi);
+    else
+        hipLaunchKernelGGL(k_face_speeds<float>, dim3(fblocks(M->nfaces)), dim3(BLOCK), 0, st,
+                           M->d, s->o, (const float*)Q, (const float*)hinv_aux, M->fc, M->fs,
+                           M->nfaces, (const double2*)M->slot_n, (float*)lo, (float*)hi);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_reduce_dt(zj_solver* s, const void* lo, const void* hi, const double* inradius_f,
+                 double cfl, int32_t dimension, int32_t degree, double dt_max, double clamp,
+                 double* dt_out, void* stream) {
+    if (!s || !lo || !hi || !inradius_f || !dt_out || dimension < 1 || degree < 0 ||
+        !(cfl > 0.0))
+        return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (!M->fc) return ZJ_EINVAL;
+    const int F = M->nfaces, nb = fblocks(F);
+    // dt_part holds npatch * 4 >= N / 64 slots; F <= 2 N + nbf faces need
+    // F / 256 of them
+    if (nb > M->pt.npatch * (SBLOCK / 64)) return ZJ_EUNSUP;
+    hipStream_t st = (hipStream_t)stream;
+    const double fac = cfl / ((double)dimension * (double)(2 * degree + 1));
+    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, (int*)nullptr);
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_face_dt<double>, dim3(nb), dim3(BLOCK), 0, st, (const double*)lo,
+                           (const double*)hi, inradius_f, F, fac, s->dt_part);
+    else
+        hipLaunchKernelGGL(k_face_dt<float>, dim3(nb), dim3(BLOCK), 0, st, (const float*)lo,
+                           (const float*)hi, inradius_f, F, fac, s->dt_part);
+    hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(nb)), dim3(1024), 0, st,
+                       (const double*)s->dt_part, nb, s->dt_red, (const int*)nullptr);
+    hipLaunchKernelGGL(k_dt_finalize_rem, dim3(1), dim3(1), 0, st, s->dt_red, dt_out, clamp,
+                       dt_max);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_bc_values(zj_solver* s, const void* Q, void* bf_values, void* stream) {
+    if (!s || !Q || !bf_values) return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (M->d.nbf == 0) return ZJ_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_bc_values<double>, dim3(fblocks(M->d.nbf)), dim3(BLOCK), 0, st, M->d,
+                           (const double*)Q, M->bf_cell, M->bf_slot, (const double2*)M->slot_n,
+                           (double*)bf_values);
+    else
+        hipLaunchKernelGGL(k_bc_values<float>, dim3(fblocks(M->d.nbf)), dim3(BLOCK), 0, st, M->d,
+                           (const float*)Q, M->bf_cell, M->bf_slot, (const double2*)M->slot_n,
+                           (float*)bf_values);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_reconstruct(zj_solver* s, const void* Q, const uint8_t* force_o1, int32_t order1,
+                   void* QL, void* QR, void* lim_grad, void* stream) {
+    if (!s || !Q || !QL || !QR) return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (!M->fc) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const int slopes = (s->o.order >= 2 && !order1) ? 1 : 0;
+    if (s->p.dtype == ZJ_F64) {
+        ReconOut<double> ro{(double*)QL, (double*)QR, slopes ? (double*)lim_grad : nullptr,
+                            M->cell_face, M->nfaces};
+        launch_stage<double>(s, (const double*)Q, (const double*)nullptr, (double*)nullptr,
+                             (const double*)nullptr, (double*)nullptr, 4, 0, 0, slopes,
+                             (int*)nullptr, (unsigned long long*)nullptr, st, force_o1,
+                             (uint8_t*)nullptr, (const int*)nullptr, ro);
+    } else {
+        ReconOut<float> ro{(float*)QL, (float*)QR, slopes ? (float*)lim_grad : nullptr,
+                           M->cell_face, M->nfaces};
+        launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)nullptr,
+                            (const double*)nullptr, (float*)nullptr, 4, 0, 0, slopes,
+                            (int*)nullptr, (unsigned long long*)nullptr, st, force_o1,
+                            (uint8_t*)nullptr, (const int*)nullptr, ro);
+    }
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_flux_pass(zj_solver* s, const void* Q, const void* hinv_aux, const void* QL,
+                 const void* QR, void* Fface, void* Dp, void* Dm, void* stream) {
+    if (!s || !Q || !QL || !QR || !Fface || !Dp || !Dm) return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (!M->fc) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_flux_pass<double>, dim3(fblocks(M->nfaces)), dim3(BLOCK), 0, st, M->d,
+                           s->o, (const double*)Q, (const double*)hinv_aux, (const double*)QL,
+                           (const double*)QR, M->fc, M->fs, M->nfaces, (const double2*)M->slot_n,
+                           (double*)Fface, (double*)Dp, (double*)Dm);
+    else
+        hipLaunchKernelGGL(k_flux_pass<float>, dim3(fblocks(M->nfaces)), dim3(BLOCK), 0, st, M->d,
+                           s->o, (const float*)Q, (const float*)hinv_aux, (const float*)QL,
+                           (const float*)QR, M->fc, M->fs, M->nfaces, (const double2*)M->slot_n,
+                           (float*)Fface, (float*)Dp, (float*)Dm);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_gather_update(zj_solver* s, const void* Q0, const void* Fface, const void* Dp,
+                     const void* Dm, const double* dt, double a_stage, const void* cell_term,
+                     const void* lim_grad, int32_t source, const void* hinv_aux, void* Qcand,
+                     uint8_t* troubled, int32_t* count, void* stream) {
+    if (!s || !Q0 || !Fface || !Dp || !Dm || !dt || !Qcand || Qcand == Q0) return ZJ_EINVAL;
+    const zj_mesh* M = s->m;
+    if (!M->fc) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_gather_update<double>, dim3(fblocks(M->d.N)), dim3(BLOCK), 0, st,
+                           M->d, s->o, (const double*)Q0, (const double*)Fface, (const double*)Dp,
+                           (const double*)Dm, M->cell_face, M->nfaces, dt, a_stage,
+                           (const double*)cell_term, (const double*)lim_grad, source,
+                           (const double*)hinv_aux, (double*)Qcand, troubled, count);
+    else
+        hipLaunchKernelGGL(k_gather_update<float>, dim3(fblocks(M->d.N)), dim3(BLOCK), 0, st,
+                           M->d, s->o, (const float*)Q0, (const float*)Fface, (const float*)Dp,
+                           (const float*)Dm, M->cell_face, M->nfaces, dt, a_stage,
+                           (const float*)cell_term, (const float*)lim_grad, source,
+                           (const float*)hinv_aux, (float*)Qcand, troubled, count);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_flux_operator_o1(zj_solver* s, const void* Q, const uint8_t* force_o1, void* dQ,
+                        void* stream) {
+    if (!s || !Q || !dQ || Q == dQ) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const int slopes = s->o.order >= 2 ? 1 : 0;
+    if (s->p.dtype == ZJ_F64)
+        launch_stage<double>(s, (const double*)Q, (const double*)nullptr, (double*)dQ,
+                             (const double*)nullptr, (double*)nullptr, 2, 0, 1, slopes,
+                             (int*)nullptr, (unsigned long long*)nullptr, st, force_o1);
+    else
+        launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)dQ,
+                            (const double*)nullptr, (float*)nullptr, 2, 0, 1, slopes,
+                            (int*)nullptr, (unsigned long long*)nullptr, st, force_o1);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+}  // extern "C"

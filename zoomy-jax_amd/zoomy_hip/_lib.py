@@ -92,6 +92,30 @@ _SIGS = {
                                              ctypes.c_double, ctypes.c_void_p,
                                              ctypes.c_void_p]),
     "zj_step": (ctypes.c_int, [ctypes.c_void_p] * 6),
+    "zj_mesh_set_faces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_face_speeds": (ctypes.c_int, [ctypes.c_void_p] * 6),
+    "zj_reduce_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_double, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_void_p,
+                                    ctypes.c_void_p]),
+    "zj_bc_values": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "zj_reconstruct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int32,
+                                      ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_flux_pass": (ctypes.c_int, [ctypes.c_void_p] * 9),
+    "zj_gather_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "zj_flux_operator_o1": (ctypes.c_int, [ctypes.c_void_p] * 5),
     "zj_update_aux": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "zj_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_void_p,

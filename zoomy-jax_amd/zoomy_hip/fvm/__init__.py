@@ -3,9 +3,11 @@ kernels)."""
 from .chorin import ChorinSplitVAMSolver, EllipticSolveError, elliptic_accept
 from .diffusion import DiffusionOperator
 from .solver import (DeviceMesh, HyperbolicSolver, IMEXSourceSolver,
-                     ShallowWater)
+                     Reconstruction, ShallowWater)
+from . import blocks
 
 __all__ = ["ChorinSplitVAMSolver", "DeviceMesh", "DiffusionOperator",
            "EllipticSolveError",
-           "HyperbolicSolver", "IMEXSourceSolver", "ShallowWater",
+           "HyperbolicSolver", "IMEXSourceSolver", "Reconstruction",
+           "ShallowWater", "blocks",
            "elliptic_accept"]

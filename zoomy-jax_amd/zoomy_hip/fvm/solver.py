@@ -303,22 +303,59 @@ class HyperbolicSolver:
         return self._dt
 
     # ── operators ─────────────────────────────────────────────────────
+    @property
+    def supports_force_o1(self) -> bool:
+        """Whether the reconstruction honours a per-cell first-order mask:
+        the eta reconstructions do, the others ignore it, as in the
+        reference (``solver_jax.py:580-582``, ``reconstruction_jax.py:
+        1058-1062``)."""
+        return self.reconstruction_variables in ("eta", "eta_hu")
+
     def get_flux_operator(self, mesh=None, runtime=None):
         lib = L.lib()
 
         def flux_operator(dt, time, Q, Qaux, parameters=None, dQ=None,
                           force_o1=None):
-            if force_o1 is not None:
-                raise NotImplementedError(
-                    "force_o1 (MOOD demotion) is not supported by this "
-                    "reconstruction")
             Q4 = self._full(Q)
             out = torch.empty_like(Q4)
-            L.check(lib.zj_flux_operator(self._rt.handle, _p(Q4), _p(out),
-                                         _stream()), "zj_flux_operator")
+            if force_o1 is not None and self.supports_force_o1:
+                m = torch.as_tensor(force_o1, device=self.device)
+                if m.numel() != Q4.shape[1]:
+                    raise ValueError("force_o1 must have one entry per cell")
+                m = m.reshape(-1).to(torch.uint8).contiguous()
+                L.check(lib.zj_flux_operator_o1(self._rt.handle, _p(Q4),
+                                                _p(m), _p(out), _stream()),
+                        "zj_flux_operator_o1")
+            else:
+                L.check(lib.zj_flux_operator(self._rt.handle, _p(Q4),
+                                             _p(out), _stream()),
+                        "zj_flux_operator")
             return self._view(out)
 
         return flux_operator
+
+    def _faces(self):
+        """The reference-layout face arrays on the device (uploaded once)."""
+        if getattr(self, "_meshrt", None) is None:
+            from .blocks import prepare_mesh
+            self._meshrt = prepare_mesh(self.mesh, self)
+        return self._meshrt
+
+    def get_compute_max_abs_eigenvalue(self, mesh=None, runtime=None):
+        """Per-face max |eigenvalue| of both cell states, ``(n_faces,)`` in
+        the order of ``Mesh.faces()`` (``solver_jax.py:434-457``)."""
+        from .blocks import dt_pass
+
+        def compute_max_abs_eigenvalue(Q, Qaux, parameters=None):
+            lo, hi = dt_pass(Q, Qaux, parameters, self._faces())
+            return torch.maximum(lo.abs(), hi.abs())
+
+        return compute_max_abs_eigenvalue
+
+    def get_reconstruction(self):
+        """The reconstruction object of this solver's numerics
+        (``reconstruction_jax.py:398-429``, ``:1307-1366``)."""
+        return Reconstruction(self)
 
     def compute_timestep(self, Q, Qaux=None, parameters=None,
                          t_remaining: Optional[float] = None):
@@ -449,6 +486,50 @@ class HyperbolicSolver:
     def solve(self, mesh: Mesh, model: ShallowWater, write_output=False):
         Q, Qaux = self.setup_simulation(mesh, model)
         return self.run_simulation(Q, Qaux, write_output=write_output)
+
+
+class Reconstruction:
+    """Reconstruction object over the HIP reconstruction kernel:
+    ``__call__(Q, bf_face_values, force_o1=None, Qaux=None, parameters=None,
+    bf_aux=None) -> (Q_L, Q_R)`` and ``reconstruct_with_grad(...) -> (Q_L,
+    Q_R, lim_grad)`` with face arrays ``(n_var, n_faces)`` in the order of
+    ``Mesh.faces()`` and ``lim_grad (n_var, dim, n_cells)`` in conservative
+    variables (``LSQMUSCLReconstructionJAX``, ``reconstruction_jax.py:
+    398-429``; the eta reconstruction ``EmittedWLSQMUSCLJAX``, :1307-1366).
+    The boundary values are computed on the device from Q (the BC ghosts of
+    the cell states, what every reference caller passes); Q_R of a boundary
+    face is that ghost."""
+
+    supports_grad_recon = True
+
+    def __init__(self, solver: HyperbolicSolver):
+        self.solver = solver
+        self.supports_force_o1 = solver.supports_force_o1
+
+    def _run(self, Q, force_o1, with_grad):
+        from .blocks import build_operators, reconstruct
+        s = self.solver
+        meshrt = s._faces()
+        ops = build_operators(s)
+        QL, QR, lg = reconstruct(
+            Q, None, meshrt, ops, None,
+            force_o1=force_o1 if self.supports_force_o1 else None)
+        n = s.n_state
+        QL, QR = QL[:n], QR[:n]
+        if not with_grad:
+            return QL, QR
+        if lg is None:
+            lg = torch.zeros((4, meshrt.dimension, meshrt.n_cells),
+                             dtype=s.dtype, device=s.device)
+        return QL, QR, lg[:n]
+
+    def __call__(self, Q, bf_face_values=None, force_o1=None, Qaux=None,
+                 parameters=None, bf_aux=None):
+        return self._run(Q, force_o1, False)
+
+    def reconstruct_with_grad(self, Q, bf_face_values=None, force_o1=None,
+                              Qaux=None, parameters=None, bf_aux=None):
+        return self._run(Q, force_o1, True)
 
 
 class IMEXSourceSolver(HyperbolicSolver):
