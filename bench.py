@@ -127,7 +127,7 @@ def cpu_leg(runs=5):
     cores = len(os.sched_getaffinity(0))
     out = {"cores": cores, "cpu_model": cpu_model(), "cells": m.n_cells}
     Qref = None
-    for threads, steps in ((cores, 40), (1, 8)):
+    for threads, steps in ((cores, 400), (1, 40)):
         OO.set_threads(threads)
         o = OO.Oracle(m, omp=True, order=2, limiter="minmod", riemann="hll",
                       cfl=0.9)

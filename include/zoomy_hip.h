@@ -239,6 +239,35 @@ int zj_gather_update(zj_solver *s, const void *Q0, const void *Fface,
 int zj_time_kernels(zj_solver *s, const void *Q, const double *dt,
                     int32_t reps, float *ms_out, void *stream);
 
+/* ───────────────────────── batched ensemble ─────────────────────────
+ * Independent instances stored as one mesh: the cells of instance i are
+ * cell_offsets[i] .. cell_offsets[i+1]-1 (each more than 64 cells) and no
+ * face joins two instances.  Every step advances every instance with its
+ * own dt (CFL law per instance) in the same two stage launches over the
+ * whole mesh; per-instance t and dt are device arrays (n_inst doubles).  The
+ * solver must be order 2 without MOOD or the frozen stage aux.  No
+ * counterpart in the reference (it marches one mesh at a time); the
+ * numerics per instance are zj_march's. */
+typedef struct zj_ensemble zj_ensemble;
+int zj_ensemble_create(zj_solver *s, int32_t n_inst, const int32_t *cell_offsets,
+                       zj_ensemble **out);
+int zj_ensemble_destroy(zj_ensemble *e);
+/* dt of every instance's state into dt_out (n_inst, device). */
+int zj_ensemble_dt(zj_ensemble *e, const void *Q, double *dt_out, void *stream);
+/* n_steps steps of every instance, Q in place, t_dev advanced; dt of each
+ * step capped by t_end - t (t_end < 0: no cap). */
+int zj_ensemble_march(zj_ensemble *e, void *Q, int32_t n_steps, double t_end,
+                      double *t_dev, double *dt_dev, void *stream);
+/* Captures n_steps steps on Q, t_dev, dt_dev into a HIP graph (kernel
+ * nodes only); zj_ensemble_replay launches it on `stream`, each replay
+ * continuing the march.  Capturing computes the dt of the current Q and
+ * synchronises the stream. */
+int zj_ensemble_capture(zj_ensemble *e, void *Q, int32_t n_steps, double t_end,
+                        double *t_dev, double *dt_dev, void *stream);
+int zj_ensemble_replay(zj_ensemble *e, void *stream);
+/* Compensated FP64 mass of every instance into mass_out (n_inst, device). */
+int zj_ensemble_mass(zj_ensemble *e, const void *Q, double *mass_out, void *stream);
+
 /* Compensated FP64 mass sum(h |cell|) into the device scalar *mass_out. */
 int zj_mass(zj_solver *s, const void *Q, double *mass_out, void *stream);
 

@@ -109,6 +109,7 @@ def _bind(L):
     L.zjo_step_mood.argtypes = [P(_Mesh), P(_Params), d, vp, vp,
                                 P(ctypes.c_int)]
     L.zjo_flux_operator.argtypes = [P(_Mesh), P(_Params), vp, vp]
+    L.zjo_flux_operator_o1.argtypes = [P(_Mesh), P(_Params), vp, vp, vp]
     L.zjo_reconstruct.argtypes = [P(_Mesh), P(_Params), vp, vp, vp, vp]
     L.zjo_update_aux.argtypes = [P(_Mesh), P(_Params), vp, vp]
     L.zjo_mass.restype = d
@@ -273,6 +274,17 @@ class Oracle:
                                    ctypes.byref(self.p), float(dt), _ptr(Q),
                                    _ptr(out), ctypes.byref(nd))
         return out, bad, nd.value
+
+    def flux_o1(self, Q, force_o1):
+        """Flux operator with a per-cell first-order mask."""
+        Q = self._q(Q)
+        mask = np.ascontiguousarray(force_o1, np.uint8)
+        assert mask.shape == (self.n,)
+        out = np.empty_like(Q)
+        self.L.zjo_flux_operator_o1(ctypes.byref(self.rm.c),
+                                    ctypes.byref(self.p), _ptr(Q),
+                                    _ptr(mask), _ptr(out))
+        return out
 
     def flux(self, Q):
         Q = self._q(Q)

@@ -678,6 +678,14 @@ void zjo_flux_operator(const zjo_mesh *m, const zjo_params *p, const double *Q,
     flux_operator_h(m, p, Q, Q, dQ);
 }
 
+void zjo_flux_operator_o1(const zjo_mesh *m, const zjo_params *p, const double *Q,
+                          const unsigned char *force_o1, double *dQ)
+{
+    g_force_o1 = force_o1;
+    flux_operator_h(m, p, Q, Q, dQ);
+    g_force_o1 = NULL;
+}
+
 static void flux_operator_h(const zjo_mesh *m, const zjo_params *p,
                             const double *Q, const double *H, double *dQ)
 {

@@ -99,6 +99,11 @@ void zjo_face_flux(const zjo_params *p, const double *qL, const double *qR,
 void zjo_flux_operator(const zjo_mesh *m, const zjo_params *p, const double *Q,
                        double *dQ);
 
+/* The same with a per-cell first-order mask (the flux operator's
+ * force_o1, solver_jax.py:724-785; honoured by the eta reconstructions). */
+void zjo_flux_operator_o1(const zjo_mesh *m, const zjo_params *p, const double *Q,
+                          const unsigned char *force_o1, double *dQ);
+
 /* dt law; t_remaining < 0 means "no cap". */
 double zjo_compute_dt(const zjo_mesh *m, const zjo_params *p, const double *Q,
                       double t_remaining);

@@ -312,3 +312,28 @@ def test_block_march_mood_redo_and_flat_bed_matches_fused_step():
     assert float(Qd2[1].min()) >= -1e-20
     m1 = float((Qd2[1].cpu().numpy() * mesh2.cell_volumes()).sum())
     assert abs(m1 - m0) <= 1e-12 * m0
+
+
+@pytest.mark.parametrize("recon", ["eta", "conservative"])
+def test_flux_operator_force_o1(recon):
+    """flux_operator(..., force_o1=mask): the eta reconstruction demotes the
+    masked cells (against the oracle with the same mask); the conservative
+    one ignores the mask, as the reference's flux operator does for a
+    reconstruction without supports_force_o1 (solver_jax.py:580-582)."""
+    from oracle.oracle import Oracle
+    mesh, Q = _case("wetdry")
+    kw = dict(order=2, limiter="vanleer", riemann="hll")
+    s = _solver(mesh, recon, **dict(kw))
+    o = Oracle(mesh, recon=recon, **kw)
+    mask = np.zeros(mesh.n_cells, np.uint8)
+    mask[(np.arange(mesh.n_cells) % 5) == 2] = 1
+    got = s.get_flux_operator()(0.0, 0.0, torch.as_tensor(Q, device="cuda"),
+                                None, force_o1=torch.as_tensor(mask))
+    got = got.cpu().numpy()
+    want = o.flux_o1(Q, mask) if recon == "eta" else o.flux(Q)
+    assert rel_l2(got[1], want[1]) < 1e-12
+    qs = max(np.linalg.norm(want[2]), np.linalg.norm(want[3]))
+    for r in (2, 3):
+        assert np.linalg.norm(got[r] - want[r]) <= 1e-12 * qs
+    if recon == "eta":
+        assert rel_l2(want[1], o.flux(Q)[1]) > 1e-6   # the mask acted

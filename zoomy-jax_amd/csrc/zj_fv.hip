@@ -599,19 +599,30 @@ struct ReconOut {
     int F;
 };
 
+// Ensemble of independent instances stored as one mesh (ENS = 1): the
+// instance of every cell, each instance's dt, and two dt partials per wave
+// (the minimum over the cells of the wave's first instance, and over the
+// cells of a second instance starting inside the wave; instances hold more
+// than 64 cells, so a wave meets at most two).
+struct EnsArgs {
+    const int* inst;
+    const double* dts;
+    double* part2;
+};
+
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
 // HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
 // the options.  Both instantiations evaluate the same formulas.  AUXONLY = 1
 // is the post_step aux refresh alone: staging and the own cells' gradients;
 // AUXONLY = 2 the reconstruction object alone (face states, no fluxes).
-template <typename T, int SPEC, int AUXONLY = 0>
+template <typename T, int SPEC, int AUXONLY = 0, int ENS = 0>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void k_stage(
     DevMesh m_in, DevPatch pt, Opts o_in, const T* __restrict__ Qin, const T* Q0, T* Qout,
     const double* __restrict__ dtp, T* __restrict__ aux_out, int mode, int final_stage,
     int write_b, int use_slopes, int* __restrict__ troubled, double* __restrict__ dt_part,
     const uint8_t* __restrict__ o1_mask, uint8_t* __restrict__ bad_out,
-    const int* __restrict__ gate, ReconOut<T> ro) {
+    const int* __restrict__ gate, ReconOut<T> ro, EnsArgs ea) {
     // gate: a MOOD redo launch runs only when the candidate had troubled cells
     if (gate && *gate == 0) return;
     DevMesh m = m_in;
@@ -983,7 +994,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         // the step state and dt of the RK average, fetched before the faces
         T q0[4] = {0, 0, 0, 0};
         if (mode == 1) load_q(Q0, N, gi, q0);
-        const T dt = mode == 2 ? T(0) : T(*dtp);
+        const T dt = mode == 2 ? T(0) : T(ENS ? ea.dts[ea.inst[gi]] : *dtp);
         uint16_t nls[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
@@ -1082,7 +1093,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             bad = (out[1] < T(0)) || !isfinite(out[0]) || !isfinite(out[1]) ||
                   !isfinite(out[2]) || !isfinite(out[3]);
             if (bad_out) bad_out[gi] = (uint8_t)bad;
-            if (dt_part) {
+            if (dt_part || (ENS && ea.part2)) {
                 // min over faces of r_f / a_f without a division per face
                 const T hn = hinv_of(out[1], eps);
                 const double c = sqrt(o.g * fmax((double)out[1], 0.0));
@@ -1114,6 +1125,26 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         if ((tid & 63) == 0) {
             if (troubled && bad) atomicAdd(troubled, bad);
             if (dt_part) dt_part[(size_t)p * (SBLOCK / 64) + (tid >> 6)] = loc_dt;
+        }
+    }
+    if (ENS && final_stage && mode != 2 && ea.part2) {
+        double loc_dt = best_r == INFINITY ? INFINITY
+                                           : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
+        if (loc_dt != loc_dt) loc_dt = 0.0;
+        // own cells are a prefix of the patch: lane 0 of a wave with cells
+        // has one, and names the wave's first instance
+        const int my = gi_keep >= 0 ? ea.inst[gi_keep] : -1;
+        const int i0 = __shfl(my, 0, 64);
+        double m0 = (my >= 0 && my == i0) ? loc_dt : INFINITY;
+        double m1 = (my >= 0 && my != i0) ? loc_dt : INFINITY;
+        for (int off = 32; off > 0; off >>= 1) {
+            m0 = fmin(m0, __shfl_xor(m0, off, 64));
+            m1 = fmin(m1, __shfl_xor(m1, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            const size_t w = (size_t)p * (SBLOCK / 64) + (tid >> 6);
+            ea.part2[2 * w] = m0;
+            ea.part2[2 * w + 1] = m1;
         }
     }
     if (gi_keep >= 0) {
@@ -1443,19 +1474,21 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   int mode, int final_stage, int write_b, int use_slopes, int* troubled,
                   unsigned long long* dt_red, hipStream_t st,
                   const uint8_t* o1_mask = nullptr, uint8_t* bad_out = nullptr,
-                  const int* gate = nullptr, ReconOut<T> ro = ReconOut<T>{}) {
+                  const int* gate = nullptr, ReconOut<T> ro = ReconOut<T>{},
+                  const EnsArgs* ens = nullptr) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                       s->o.recon == RECON_CONS && !s->o.stage_aux;
-    auto kern = mode == 4   ? k_stage<T, 0, 2>
+    auto kern = ens         ? k_stage<T, 0, 0, 1>
+                : mode == 4 ? k_stage<T, 0, 2>
                 : mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
                             : (spec ? k_stage<T, 1> : k_stage<T, 0>);
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
-                       o1_mask, bad_out, gate, ro);
+                       o1_mask, bad_out, gate, ro, ens ? *ens : EnsArgs{});
     if (dt_red)
         hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(M->pt.npatch * (SBLOCK / 64))),
                            dim3(1024), 0, st, (const double*)s->dt_part,
@@ -2276,6 +2309,339 @@ int zj_flux_operator_o1(zj_solver* s, const void* Q, const uint8_t* force_o1, vo
         launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)dQ,
                             (const double*)nullptr, (float*)nullptr, 2, 0, 1, slopes,
                             (int*)nullptr, (unsigned long long*)nullptr, st, force_o1);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+}  // extern "C"
+
+// ─────────────────────────────────────────────────────────────────────────
+// Batched ensemble: independent instances concatenated into one mesh (cells
+// of instance i are off[i] .. off[i+1]-1, no face joins two instances).  A
+// step is the same two stage launches over the whole union's patch list --
+// every patch is one work item, so uneven instances balance by themselves --
+// with each cell reading its instance's dt, and the per-wave dt partials
+// folded per instance.  The steps can be captured in a HIP graph.
+// ─────────────────────────────────────────────────────────────────────────
+namespace {
+
+// dt of each instance's state: per cell min over faces, two partials per
+// wave (waves of 64 consecutive cells: the block of 256 cells is one
+// "patch" of the fold below).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_dt_ens(DevMesh m, Opts o, const T* __restrict__ Q,
+                                                  const double2* __restrict__ sn,
+                                                  const double* __restrict__ sr,
+                                                  const int* __restrict__ inst,
+                                                  double* __restrict__ part2) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    double loc = INFINITY;
+    int my = -1;
+    if (c < m.N) {
+        const int N = m.N;
+        T q[4];
+        load_q(Q, N, c, q);
+        const T hn = hinv_of(q[1], T(o.eps));
+        ZJ_FOR_FACES(k) {
+            const double2 n = sn[(size_t)k * N + c];
+            const double a = (double)max_abs_eig(q, hn, T(n.x), T(n.y), T(o.g));
+            double l = o.cfl * 2.0 * sr[(size_t)k * N + c] / ((double)m.dim * a);
+            if (l != l) l = 0.0;
+            loc = fmin(loc, l);
+        }
+        my = inst[c];
+    }
+    const int i0 = __shfl(my, 0, 64);
+    double m0 = (my >= 0 && my == i0) ? loc : INFINITY;
+    double m1 = (my >= 0 && my != i0) ? loc : INFINITY;
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmin(m0, __shfl_xor(m0, off, 64));
+        m1 = fmin(m1, __shfl_xor(m1, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const size_t w = (size_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+        part2[2 * w] = m0;
+        part2[2 * w + 1] = m1;
+    }
+}
+
+// Per-instance fold of the wave partials (one block per instance): the waves
+// of patch p (P cells each) cover cells p P + 64 j .. + 63, j < 4; a wave's
+// first partial belongs to the instance of its first cell, its second to an
+// instance that starts inside it.  Deterministic (a minimum).
+__global__ __launch_bounds__(BLOCK) void k_reduce_inst(const int* __restrict__ off,
+                                                       const int* __restrict__ inst,
+                                                       const double* __restrict__ part2,
+                                                       int P, int N, double* __restrict__ red) {
+    __shared__ double lds[BLOCK / 64];
+    const int i = blockIdx.x;
+    const int c_lo = off[i], c_hi = off[i + 1];  // [c_lo, c_hi)
+    double v = INFINITY;
+    if (c_hi > c_lo) {
+        const int p0 = c_lo / P, p1 = (c_hi - 1) / P;
+        const int nw = (p1 - p0 + 1) * (SBLOCK / 64);
+        for (int t = threadIdx.x; t < nw; t += BLOCK) {
+            const int p = p0 + t / (SBLOCK / 64), j = t % (SBLOCK / 64);
+            const int fc = p * P + 64 * j;      // first cell of the wave
+            if (64 * j >= P || fc >= N) continue;
+            const int lc = min(fc + 63, min(p * P + P - 1, N - 1));  // its last cell
+            if (lc < c_lo || fc >= c_hi) continue;
+            const size_t w = (size_t)p * (SBLOCK / 64) + j;
+            v = fmin(v, part2[2 * w + (inst[fc] == i ? 0 : 1)]);
+        }
+    }
+    v = block_reduce(v, [](double a, double b) { return fmin(a, b); }, lds);
+    if (threadIdx.x == 0) red[i] = v;
+}
+
+// Per instance: t += the previous dt (advance), then dt = min(fold, dt_max,
+// t_end - t).
+__global__ void k_ens_head(int n, const double* __restrict__ red, double* __restrict__ t,
+                           double* __restrict__ dt, double t_end, double dt_max, int advance) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (advance) t[i] += dt[i];
+    double d = red[i];
+    if (dt_max > 0.0) d = fmin(d, dt_max);
+    if (t_end >= 0.0) d = fmin(d, t_end - t[i]);
+    dt[i] = d;
+}
+
+__global__ void k_ens_advance(int n, double* __restrict__ t, const double* __restrict__ dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] += dt[i];
+}
+
+// Compensated FP64 mass of each instance (one block per instance, fixed
+// fold order).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_mass_inst(DevMesh m, const T* __restrict__ Q,
+                                                     const int* __restrict__ off,
+                                                     double* __restrict__ out) {
+    __shared__ double2 lds[BLOCK];
+    const int i = blockIdx.x;
+    double s = 0.0, comp = 0.0;
+    for (int c = off[i] + threadIdx.x; c < off[i + 1]; c += BLOCK) {
+        double2 v[MAXF];
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k)
+            v[k] = k < m.mf ? m.verts[m.cv[k * m.N + c]] : make_double2(0.0, 0.0);
+        neumaier(s, comp, (double)Q[m.N + c] * cell_volume(m, v));
+    }
+    lds[threadIdx.x] = make_double2(s, comp);
+    fold_pairs<BLOCK>(lds);
+    if (threadIdx.x == 0) out[i] = lds[0].x + lds[0].y;
+}
+
+}  // namespace
+
+struct zj_ensemble {
+    zj_solver* s = nullptr;
+    int n = 0;
+    int* off = nullptr;        // (n + 1) cell offsets, device
+    int* inst = nullptr;       // (N) instance of each cell, device
+    double* part2 = nullptr;   // two dt partials per wave
+    double* red = nullptr;     // (n) per-instance dt fold
+    hipStream_t cap = nullptr; // stream the graph is captured on
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+void ens_dt(zj_ensemble* e, const T* Q, hipStream_t st) {
+    const zj_mesh* M = e->s->m;
+    const int nb = nblocks(M->d.N);
+    hipLaunchKernelGGL(k_dt_ens<T>, dim3(nb), dim3(BLOCK), 0, st, M->d, e->s->o, Q,
+                       (const double2*)M->slot_n, (const double*)M->slot_r,
+                       (const int*)e->inst, e->part2);
+    hipLaunchKernelGGL(k_reduce_inst, dim3(e->n), dim3(BLOCK), 0, st, (const int*)e->off,
+                       (const int*)e->inst, (const double*)e->part2, BLOCK, M->d.N, e->red);
+}
+
+// n_steps SSP-RK2 steps of every instance, Q in place; the first head uses
+// the dt fold of Q already in e->red.  Kernel launches only (capturable).
+template <typename T>
+void ens_steps(zj_ensemble* e, T* Q, int n_steps, double t_end, double* t, double* dt,
+               int advance_first, hipStream_t st) {
+    zj_solver* s = e->s;
+    const zj_mesh* M = s->m;
+    T* Q1 = (T*)s->Q1;
+    const EnsArgs stage1{e->inst, dt, nullptr};
+    const EnsArgs stage2{e->inst, dt, e->part2};
+    const int hb = (e->n + 63) / 64;
+    for (int k = 0; k < n_steps; ++k) {
+        hipLaunchKernelGGL(k_ens_head, dim3(hb), dim3(64), 0, st, e->n, (const double*)e->red, t,
+                           dt, t_end, s->p.dt_max, (k > 0 || advance_first) ? 1 : 0);
+        launch_stage<T>(s, (const T*)Q, (const T*)nullptr, Q1, (const double*)nullptr,
+                        (T*)nullptr, 0, 0, 0, 1, (int*)nullptr, (unsigned long long*)nullptr, st,
+                        (const uint8_t*)nullptr, (uint8_t*)nullptr, (const int*)nullptr,
+                        ReconOut<T>{}, &stage1);
+        launch_stage<T>(s, (const T*)Q1, (const T*)Q, Q, (const double*)nullptr, (T*)nullptr, 1,
+                        1, 0, 1, (int*)nullptr, (unsigned long long*)nullptr, st,
+                        (const uint8_t*)nullptr, (uint8_t*)nullptr, (const int*)nullptr,
+                        ReconOut<T>{}, &stage2);
+        hipLaunchKernelGGL(k_reduce_inst, dim3(e->n), dim3(BLOCK), 0, st, (const int*)e->off,
+                           (const int*)e->inst, (const double*)e->part2, M->pt.P, M->d.N, e->red);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int zj_ensemble_create(zj_solver* s, int32_t n_inst, const int32_t* cell_offsets,
+                       zj_ensemble** out) {
+    if (!s || n_inst <= 0 || !cell_offsets || !out) return ZJ_EINVAL;
+    const int N = s->m->d.N;
+    // the scheme the ensemble kernels run: order-2 conservative MUSCL, no
+    // MOOD, the step-start aux not frozen (in-place stage 2 reads only its
+    // own cells of Q)
+    if (s->o.order != 2 || s->mood || s->o.stage_aux || s->o.rs == RS_RUSANOV_NC)
+        return ZJ_EUNSUP;
+    if (cell_offsets[0] != 0 || cell_offsets[n_inst] != N) return ZJ_EINVAL;
+    std::vector<int32_t> inst(N);
+    for (int i = 0; i < n_inst; ++i) {
+        // every instance holds more than one wave of cells, so a wave of 64
+        // consecutive cells meets at most two instances
+        if (cell_offsets[i + 1] - cell_offsets[i] < 64) return ZJ_EINVAL;
+        for (int c = cell_offsets[i]; c < cell_offsets[i + 1]; ++c) inst[c] = i;
+    }
+    // no face may join two instances
+    {
+        const int mf = s->m->d.mf;
+        std::vector<int32_t> nbr((size_t)mf * N);
+        HIPCHK(hipMemcpy(nbr.data(), s->m->d.nbr, sizeof(int32_t) * nbr.size(),
+                         hipMemcpyDeviceToHost));
+        for (int k = 0; k < mf; ++k)
+            for (int c = 0; c < N; ++c) {
+                const int nb = nbr[(size_t)k * N + c];
+                if (nb >= 0 && inst[nb] != inst[c]) return ZJ_EINVAL;
+            }
+    }
+    zj_ensemble* e = new (std::nothrow) zj_ensemble();
+    if (!e) return ZJ_ENOMEM;
+    e->s = s;
+    e->n = n_inst;
+    const size_t nwaves = (size_t)std::max(s->m->pt.npatch, nblocks(N)) * (SBLOCK / 64);
+    if (hipMalloc((void**)&e->off, sizeof(int) * (n_inst + 1)) != hipSuccess ||
+        hipMalloc((void**)&e->inst, sizeof(int) * N) != hipSuccess ||
+        hipMalloc((void**)&e->part2, sizeof(double) * 2 * nwaves) != hipSuccess ||
+        hipMalloc((void**)&e->red, sizeof(double) * n_inst) != hipSuccess ||
+        hipStreamCreate(&e->cap) != hipSuccess) {
+        zj_ensemble_destroy(e);
+        return ZJ_ENOMEM;
+    }
+    if (hipMemcpy(e->off, cell_offsets, sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(e->inst, inst.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        zj_ensemble_destroy(e);
+        return ZJ_EHIP;
+    }
+    *out = e;
+    return ZJ_OK;
+}
+
+int zj_ensemble_destroy(zj_ensemble* e) {
+    if (!e) return ZJ_OK;
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->cap) (void)hipStreamDestroy(e->cap);
+    for (void* p : {(void*)e->off, (void*)e->inst, (void*)e->part2, (void*)e->red})
+        if (p) (void)hipFree(p);
+    delete e;
+    return ZJ_OK;
+}
+
+int zj_ensemble_dt(zj_ensemble* e, const void* Q, double* dt_out, void* stream) {
+    if (!e || !Q || !dt_out) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (e->s->p.dtype == ZJ_F64) ens_dt(e, (const double*)Q, st);
+    else ens_dt(e, (const float*)Q, st);
+    HIPCHK(hipMemcpyAsync(dt_out, e->red, sizeof(double) * e->n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_ensemble_march(zj_ensemble* e, void* Q, int32_t n_steps, double t_end, double* t_dev,
+                      double* dt_dev, void* stream) {
+    if (!e || !Q || !t_dev || !dt_dev || n_steps < 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    zj_solver* s = e->s;
+    const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+    // the stage buffer's bed row (stage 1 does not write it)
+    HIPCHK(hipMemcpyAsync(s->Q1, Q, esz * s->m->d.N, hipMemcpyDeviceToDevice, st));
+    if (s->p.dtype == ZJ_F64) {
+        ens_dt(e, (const double*)Q, st);
+        ens_steps(e, (double*)Q, n_steps, t_end, t_dev, dt_dev, 0, st);
+    } else {
+        ens_dt(e, (const float*)Q, st);
+        ens_steps(e, (float*)Q, n_steps, t_end, t_dev, dt_dev, 0, st);
+    }
+    if (n_steps > 0)
+        hipLaunchKernelGGL(k_ens_advance, dim3((e->n + 63) / 64), dim3(64), 0, st, e->n, t_dev,
+                           (const double*)dt_dev);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_ensemble_capture(zj_ensemble* e, void* Q, int32_t n_steps, double t_end, double* t_dev,
+                        double* dt_dev, void* stream) {
+    if (!e || !Q || !t_dev || !dt_dev || n_steps <= 0) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    zj_solver* s = e->s;
+    const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+    if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
+    if (e->graph) {
+        (void)hipGraphDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    // outside the graph: the stage buffer's bed row and the dt of Q; the
+    // graph holds kernel launches only (no memset or copy nodes)
+    HIPCHK(hipMemcpyAsync(s->Q1, Q, esz * s->m->d.N, hipMemcpyDeviceToDevice, st));
+    if (s->p.dtype == ZJ_F64) ens_dt(e, (const double*)Q, st);
+    else ens_dt(e, (const float*)Q, st);
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+    // every replay continues the march: its first head advances t by the
+    // last step's dt except on the first replay, which starts from the
+    // state's own dt in e->red -- the graph starts with an advance-free
+    // head and ends with the advance
+    if (s->p.dtype == ZJ_F64) ens_steps(e, (double*)Q, n_steps, t_end, t_dev, dt_dev, 0, e->cap);
+    else ens_steps(e, (float*)Q, n_steps, t_end, t_dev, dt_dev, 0, e->cap);
+    hipLaunchKernelGGL(k_ens_advance, dim3((e->n + 63) / 64), dim3(64), 0, e->cap, e->n, t_dev,
+                       (const double*)dt_dev);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->cap, &g);
+    if (ce != hipSuccess || !g) {
+        fprintf(stderr, "zoomy_hip: graph capture failed: %s\n", hipGetErrorString(ce));
+        return ZJ_EHIP;
+    }
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return ZJ_OK;
+}
+
+int zj_ensemble_replay(zj_ensemble* e, void* stream) {
+    if (!e || !e->exec) return ZJ_EINVAL;
+    HIPCHK(hipGraphLaunch(e->exec, (hipStream_t)stream));
+    return ZJ_OK;
+}
+
+int zj_ensemble_mass(zj_ensemble* e, const void* Q, double* mass_out, void* stream) {
+    if (!e || !Q || !mass_out) return ZJ_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const DevMesh& m = e->s->m->d;
+    if (e->s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_mass_inst<double>, dim3(e->n), dim3(BLOCK), 0, st, m,
+                           (const double*)Q, (const int*)e->off, mass_out);
+    else
+        hipLaunchKernelGGL(k_mass_inst<float>, dim3(e->n), dim3(BLOCK), 0, st, m, (const float*)Q,
+                           (const int*)e->off, mass_out);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }

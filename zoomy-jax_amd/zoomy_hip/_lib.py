@@ -116,6 +116,20 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "zj_flux_operator_o1": (ctypes.c_int, [ctypes.c_void_p] * 5),
+    "zj_ensemble_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zj_ensemble_dt": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "zj_ensemble_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int32, ctypes.c_double,
+                                         ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "zj_ensemble_capture": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int32, ctypes.c_double,
+                                           ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]),
+    "zj_ensemble_replay": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_ensemble_mass": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "zj_update_aux": (ctypes.c_int, [ctypes.c_void_p] * 4),
     "zj_march": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_void_p,

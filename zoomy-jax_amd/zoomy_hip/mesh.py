@@ -596,6 +596,50 @@ class Mesh:
             m._extra = e.astype(np.int32)
         return m
 
+    @staticmethod
+    def union(meshes) -> "Mesh":
+        """Disjoint union of 2-D meshes with the same tags and boundary
+        conditions: the cells of mesh k follow those of meshes 0..k-1, no
+        face joins two of them (the batched ensemble's layout)."""
+        if not meshes:
+            raise ValueError("union of no meshes")
+        m0 = meshes[0]
+        for m in meshes:
+            if m.dim != m0.dim or m.tags != m0.tags or m.bc != m0.bc or \
+                    m.periodic_partner is not None or m.bc_values:
+                raise ValueError("union needs meshes of one dimension, the "
+                                 "same tags and non-periodic, non-Dirichlet "
+                                 "boundary conditions")
+            if m.max_faces != m0.max_faces:
+                raise ValueError("union needs one number of faces per cell")
+        vo = np.cumsum([0] + [len(m.verts) for m in meshes])
+        co = np.cumsum([0] + [m.n_cells for m in meshes])
+        bo = np.cumsum([0] + [m.n_bfaces for m in meshes])
+        nbr = []
+        for k, m in enumerate(meshes):
+            nb = m.cell_nbr.astype(np.int64)
+            nbr.append(np.where(nb >= 0, nb + co[k], nb - bo[k]))
+        ex = [m.extra_stencil().astype(np.int64) for m in meshes]
+        w = max(e.shape[1] for e in ex)
+        exu = np.full((co[-1], w), -1, np.int64)
+        for k, e in enumerate(ex):
+            exu[co[k]:co[k + 1], :e.shape[1]] = np.where(e >= 0, e + co[k],
+                                                         -1)
+        u = Mesh(dim=m0.dim,
+                 verts=np.concatenate([m.verts for m in meshes]),
+                 cell_vertices=np.concatenate(
+                     [m.cell_vertices.astype(np.int64) + vo[k]
+                      for k, m in enumerate(meshes)]).astype(np.int32),
+                 cell_nbr=np.concatenate(nbr).astype(np.int32),
+                 bface_cell=np.concatenate(
+                     [m.bface_cell.astype(np.int64) + co[k]
+                      for k, m in enumerate(meshes)]).astype(np.int32),
+                 bface_slot=np.concatenate([m.bface_slot for m in meshes]),
+                 bface_tag=np.concatenate([m.bface_tag for m in meshes]),
+                 tags=list(m0.tags), bc=dict(m0.bc))
+        u._extra = np.ascontiguousarray(exu, np.int32)
+        return u
+
     def adjacency_csr(self):
         """Face-neighbour graph as CSR (indptr, indices), int32."""
         N, MF = self.cell_nbr.shape
@@ -626,6 +670,23 @@ def jittered_points(nx: int, ny: int, box, jitter: float, seed: int):
     X = X + np.where(interior_x, jx, 0.0)
     Y = Y + np.where(interior_y, jy, 0.0)
     return np.stack([X.ravel(), Y.ravel()], 1)
+
+
+def lattice_triangles(nx: int, ny: int, box, jitter: float, seed: int) -> Mesh:
+    """Triangle mesh of an ``nx x ny`` jittered lattice of quads over
+    ``box`` (the points of :func:`jittered_points`), each quad cut into two
+    triangles along its (0,0)-(1,1) diagonal: ``2 nx ny`` triangles without a
+    Delaunay pass (a jitter below half the spacing keeps every quad convex,
+    so both triangles are positively oriented)."""
+    pts = jittered_points(nx + 1, ny + 1, box, jitter, seed)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    a = i * (ny + 1) + j
+    b = a + (ny + 1)
+    c = b + 1
+    d = a + 1
+    tri = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)],
+                   -2).reshape(-1, 3)
+    return Mesh.triangles(pts, tri, box=box)
 
 
 def delaunay_mesh(points: np.ndarray, box) -> Mesh:
