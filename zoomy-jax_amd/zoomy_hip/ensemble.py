@@ -58,8 +58,9 @@ def instance_mesh(inst: Instance) -> Mesh:
 
 def instance_state(mesh: Mesh, inst: Instance) -> np.ndarray:
     """Dam break over a seeded bed: eight Gaussian bumps (amplitude U[0,
-    0.3], width U[0.03, 0.1] of the side), free surface ``h0`` for x < 0.4
-    side and 0.5 beyond (wet everywhere), at rest."""
+    0.3], width U[0.03, 0.1] of the side; where they overlap the bed is
+    scaled to stay at or below 0.35), free surface ``h0`` for x < 0.4 side
+    and 0.5 beyond -- wet everywhere -- at rest."""
     rng = np.random.default_rng(inst.seed + 1)
     xc = mesh.cell_centers()
     L_ = inst.side
@@ -69,11 +70,33 @@ def instance_state(mesh: Mesh, inst: Instance) -> np.ndarray:
         a = rng.uniform(0.0, 0.3)
         w = rng.uniform(0.03, 0.1) * L_
         b += a * np.exp(-((xc - c) ** 2).sum(1) / (2.0 * w * w))
+    if b.max() > 0.35:
+        b *= 0.35 / b.max()
     eta = np.where(xc[:, 0] < 0.4 * L_, inst.h0, 0.5)
     Q = np.zeros((4, mesh.n_cells))
     Q[0] = b
     Q[1] = np.maximum(eta - b, 0.0)
     return Q
+
+
+def _build_instance(inst: Instance):
+    m = instance_mesh(inst)
+    m.extra_stencil()          # computed here, carried to the union
+    return m, instance_state(m, inst)
+
+
+def build_instances(spec: List[Instance], workers: int = 1):
+    """Meshes and initial states of the instances, built by ``workers``
+    processes (host-side set-up only)."""
+    if workers <= 1:
+        out = [_build_instance(i) for i in spec]
+    else:
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        with cf.ProcessPoolExecutor(max_workers=workers,
+                                    mp_context=mp.get_context("fork")) as ex:
+            out = list(ex.map(_build_instance, spec, chunksize=4))
+    return [m for m, _ in out], [q for _, q in out]
 
 
 class EnsembleSolver:

@@ -523,6 +523,9 @@ class Mesh:
         ex = getattr(self, "_extra", None)
         if ex is not None:
             return ex
+        if self.dim == 2:
+            self._extra = self._extra_stencil_2d()
+            return self._extra
         st = self.vertex_stencil()
         nf = np.count_nonzero(self.cell_nbr >= 0, axis=1)
         N, MS = st.shape
@@ -536,6 +539,42 @@ class Mesh:
         width = int((out >= 0).sum(1).max()) if out.size else 0
         self._extra = np.ascontiguousarray(out[:, :max(width, 1)])
         return self._extra
+
+    def _extra_stencil_2d(self) -> np.ndarray:
+        """The vertex neighbours of each cell that are not its face
+        neighbours, ascending, -1 padded -- the same table as the generic
+        path through :meth:`vertex_stencil`, built with fixed-width row
+        sorts (the cells of each vertex padded to the largest vertex
+        degree)."""
+        N, MF = self.cell_vertices.shape
+        nv = len(self.verts)
+        cv = self.cell_vertices.astype(np.int64)
+        flat = cv.ravel()
+        cells = np.repeat(np.arange(N, dtype=np.int64), MF)
+        o = np.argsort(flat, kind="stable")
+        deg = np.bincount(flat, minlength=nv)
+        W = int(deg.max())
+        start = np.zeros(nv + 1, np.int64)
+        start[1:] = np.cumsum(deg)
+        rank = np.arange(len(flat)) - start[flat[o]]
+        vc = np.full((nv, W), -1, np.int64)
+        vc[flat[o], rank] = cells[o]
+        big = np.iinfo(np.int64).max
+        cand = vc[cv].reshape(N, MF * W)
+        me = np.arange(N, dtype=np.int64)[:, None]
+        drop = (cand < 0) | (cand == me)
+        fn = self.cell_nbr.astype(np.int64)
+        for k in range(MF):
+            drop |= cand == np.where(fn[:, k:k + 1] >= 0, fn[:, k:k + 1], -2)
+        cand = np.where(drop, big, cand)
+        cand.sort(axis=1)
+        dup = np.zeros_like(cand, bool)
+        dup[:, 1:] = cand[:, 1:] == cand[:, :-1]
+        cand[dup] = big
+        cand.sort(axis=1)
+        width = max(int((cand != big).sum(1).max()), 1)
+        out = cand[:, :width]
+        return np.ascontiguousarray(np.where(out == big, -1, out), np.int32)
 
     # ── reordering ────────────────────────────────────────────────────
     def morton_permutation(self, bits: int = 16) -> np.ndarray:
