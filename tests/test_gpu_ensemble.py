@@ -78,7 +78,9 @@ def test_graph_replays_equal_the_eager_march_and_conserve_mass():
     for k, q in enumerate(es.split()):
         h = q[1].double().cpu().numpy()
         exact = math.fsum((h * vols[k]).tolist())
-        assert abs(m0[k] - exact) <= 1e-15 * exact
+        # the device recomputes the cell areas from the vertices: last-bit
+        # differences of the areas, not of the sum
+        assert abs(m0[k] - exact) <= 1e-14 * exact
     Q_init = eager.Q.clone()
     es.capture(5)
     for r in range(3):

@@ -111,7 +111,7 @@ def test_fast_extra_stencil_equals_the_vertex_stencil_path():
     for m in (Mesh.quad(13, 9),
               delaunay_mesh(jittered_points(20, 15, (0, 1, 0, 1), 0.3, 1),
                             (0, 1, 0, 1)),
-              lattice_triangles(30, 20, (0, 1, 0, 1), 0.3, 2)):
+              lattice_triangles(30, 20, (0, 1, 0, 1), 0.15, 2)):
         st = m.vertex_stencil()
         nf = np.count_nonzero(m.cell_nbr >= 0, axis=1)
         rest = np.full((m.n_cells, st.shape[1]), -1, np.int64)
@@ -127,10 +127,16 @@ def test_lattice_triangles_and_union():
     areas summing to the box, every interior slot paired), and the union of
     meshes keeps each member's cells, faces and stencils, shifted."""
     import numpy as np
+    import pytest
     from zoomy_hip.mesh import Mesh, lattice_triangles
-    m = lattice_triangles(12, 7, (0, 2, 0, 1), 0.3, 5).set_bc("wall")
+    m = lattice_triangles(12, 7, (0, 2, 0, 1), 0.15, 5).set_bc("wall")
     assert m.n_cells == 2 * 12 * 7
     assert np.all(m.cell_volumes() > 0)
+    # the jitter bound keeps the triangles fat: inradius >= 0.08 spacing
+    big = lattice_triangles(200, 200, (0, 1, 0, 1), 0.15, 9)
+    assert big.cell_inradius().min() > 0.08 / 200
+    with pytest.raises(ValueError):
+        lattice_triangles(4, 4, (0, 1, 0, 1), 0.3, 1)
     assert abs(m.cell_volumes().sum() - 2.0) < 1e-12
     nb = m.cell_nbr
     for c in range(m.n_cells):
@@ -138,7 +144,7 @@ def test_lattice_triangles_and_union():
             if nb[c, k] >= 0:
                 assert c in nb[nb[c, k]]
     m2 = Mesh.quad(5, 4).set_bc("wall")
-    m3 = lattice_triangles(4, 4, (0, 1, 0, 1), 0.3, 6).set_bc("wall")
+    m3 = lattice_triangles(4, 4, (0, 1, 0, 1), 0.15, 6).set_bc("wall")
     u = Mesh.union([m, m3])
     assert u.n_cells == m.n_cells + m3.n_cells
     assert u.n_bfaces == m.n_bfaces + m3.n_bfaces
@@ -147,6 +153,5 @@ def test_lattice_triangles_and_union():
     e3 = m3.extra_stencil()
     np.testing.assert_array_equal(ex[:, :e3.shape[1]],
                                   np.where(e3 >= 0, e3 + m.n_cells, -1))
-    import pytest
     with pytest.raises(ValueError):
         Mesh.union([m, m2])        # quads and triangles do not mix

@@ -46,13 +46,13 @@ def ensemble_spec(n: int = 256, n_min: int = 16384, n_max: int = 1 << 20,
 
 def instance_mesh(inst: Instance) -> Mesh:
     """A seeded triangle mesh of about ``inst.n_cells`` cells on the square
-    of side ``inst.side`` (a jittered lattice cut into triangles), cells in
-    Morton order, walls."""
+    of side ``inst.side`` (a lattice jittered by 0.15 of its spacing, cut
+    into triangles), cells in Morton order, walls."""
     nq = max(inst.n_cells // 2, 4)
     nx = max(int(round(np.sqrt(nq))), 2)
     ny = max(nq // nx, 2)
     L_ = inst.side
-    m = lattice_triangles(nx, ny, (0.0, L_, 0.0, L_), 0.3, inst.seed)
+    m = lattice_triangles(nx, ny, (0.0, L_, 0.0, L_), 0.15, inst.seed)
     return m.reorder(m.morton_permutation()).set_bc("wall")
 
 

@@ -715,8 +715,12 @@ def lattice_triangles(nx: int, ny: int, box, jitter: float, seed: int) -> Mesh:
     """Triangle mesh of an ``nx x ny`` jittered lattice of quads over
     ``box`` (the points of :func:`jittered_points`), each quad cut into two
     triangles along its (0,0)-(1,1) diagonal: ``2 nx ny`` triangles without a
-    Delaunay pass (a jitter below half the spacing keeps every quad convex,
-    so both triangles are positively oriented)."""
+    Delaunay pass.  A point can move toward the diagonal opposite it by
+    sqrt(2) jitter and so can that diagonal, against a nominal distance of
+    spacing / sqrt(2): a jitter of at most 0.15 of the spacing keeps every
+    triangle's height above 0.28 of the spacing."""
+    if not 0.0 <= jitter <= 0.15:
+        raise ValueError("lattice_triangles: jitter must be in [0, 0.15]")
     pts = jittered_points(nx + 1, ny + 1, box, jitter, seed)
     i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
     a = i * (ny + 1) + j
