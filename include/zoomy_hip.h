@@ -276,6 +276,28 @@ int zj_mass(zj_solver *s, const void *Q, double *mass_out, void *stream);
 int zj_imex_local(zj_solver *s, const double *dt, int32_t maxiter,
                   const void *Qexp, void *Qout, void *stream);
 
+/* Global implicit source stage (_build_implicit_global,
+ * solver_imex_jax.py:344-396): Newton on R(Q) = Q - Qexp - dt S(Q, aux(Q))
+ * for a source that reads the derivative aux rows -- rows q_d: the model's
+ * Manning friction and -alpha g h dh/dx_d, dh/dx_d the LSQ aux row of the
+ * post-step refresh -- so the stage couples every cell to its stencil.
+ * Each Newton step solves J delta = -R by GMRES(restart) with the exact
+ * Jacobian-vector product (the linearisation, through aux(v)), relative
+ * tolerance gmres_tol, at most gmres_maxiter restart cycles; a GMRES that
+ * misses its tolerance takes delta = -R (the reference's fallback).  The
+ * Newton loop stops at maxiter steps or ||R||_2 <= tol.  FP64 solvers only;
+ * host-synchronous (the Krylov scalars are read by the host). */
+typedef struct zj_imex_global zj_imex_global;
+int zj_imex_global_create(zj_solver *s, double alpha, int32_t restart,
+                          zj_imex_global **out);
+int zj_imex_global_destroy(zj_imex_global *g);
+int zj_imex_global_solve(zj_imex_global *g, double dt, const double *Qexp,
+                         double *Qout, int32_t maxiter, double tol,
+                         double gmres_tol, int32_t gmres_maxiter, void *stream);
+/* {Newton steps, GMRES iterations, GMRES fallbacks} and the final ||R||_2
+ * of the last solve. */
+int zj_imex_global_stats(zj_imex_global *g, int32_t *out3, double *rnorm);
+
 /* ─────────────────── implicit diffusion (Crank-Nicolson) ───────────────────
  * The scalar nu-Laplacian of DiffusionOperatorJAX (zoomy_jax/fvm/
  * reconstruction_jax.py:1423-1520), which the IMEX solver applies per state

@@ -610,3 +610,27 @@ def test_openmp_build_is_bit_identical(recon, kw):
         assert tp == ts
         np.testing.assert_array_equal(Qp, Qs)
         np.testing.assert_array_equal(op.aux(Qp), o1.aux(Qs))
+
+
+def test_global_imex_oracle_newton_root():
+    """The oracle's global implicit source stage: Newton with the
+    central-difference Jacobian drives R(Q) = Q - Qexp - dt S(Q, aux(Q)) to
+    round-off, converging superlinearly, and its root differs from the
+    local (aux-free) stage exactly by the depth-gradient term's coupling:
+    with alpha = 0 it is the closed-form implicit Manning update of
+    zjo_imex_local."""
+    from oracle import imex_global as IG
+    mesh = Mesh.quad(10, 8).set_bc("wall")
+    Q = circular_dam_state(mesh, seed=2)
+    rng = np.random.default_rng(3)
+    Q[2:] = rng.normal(0.0, 0.4, (2, mesh.n_cells))
+    o = Oracle(mesh, order=2, riemann="hll", manning_n=0.05)
+    Qs, norms = IG.newton(o, Q, 0.05, 0.5, n=0.05)
+    assert norms[-1] < 1e-11 and len(norms) <= 8
+    assert norms[2] < 1e-2 * norms[1]
+    R = IG.residual(o, Qs, Q, 0.05, 0.5, n=0.05)
+    assert np.abs(R).max() < 1e-12
+    # alpha = 0: the cell-local Newton of the IMEX local stage
+    Q0, _ = IG.newton(o, Q, 0.05, 0.0, n=0.05)
+    Ql = o.imex_local(0.05, Q, maxiter=30)
+    assert np.abs(Q0 - Ql).max() < 1e-12

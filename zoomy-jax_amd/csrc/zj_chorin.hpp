@@ -33,9 +33,12 @@ struct Hier {
     void release();
 };
 
-// precision (ZJ_F64 / ZJ_F32) and cell count of an FV solver context
+// precision (ZJ_F64 / ZJ_F32), cell count, dimension and numerics of an
+// FV solver context
 int solver_dtype(const zj_solver* s);
 int solver_cells(const zj_solver* s);
+int solver_dim(const zj_solver* s);
+zj_params solver_params(const zj_solver* s);
 
 // Network of the learned preconditioner on an nx x ny grid with fine-level
 // FP32 stencil (diag, cE, cN): z (+)= N(r).  slot 0, 1 keep the activations

@@ -1209,6 +1209,8 @@ struct zj_solver {
 namespace zj {
 int solver_dtype(const zj_solver* s) { return s ? s->p.dtype : -1; }
 int solver_cells(const zj_solver* s) { return s ? s->m->d.N : -1; }
+int solver_dim(const zj_solver* s) { return s ? s->m->d.dim : -1; }
+zj_params solver_params(const zj_solver* s) { return s->p; }
 }  // namespace zj
 
 #define HIPCHK(x)                                                     \

@@ -116,6 +116,15 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p]),
     "zj_flux_operator_o1": (ctypes.c_int, [ctypes.c_void_p] * 5),
+    "zj_imex_global_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                             ctypes.c_int32, ctypes.c_void_p]),
+    "zj_imex_global_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "zj_imex_global_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double,
+                                            ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int32, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int32,
+                                            ctypes.c_void_p]),
+    "zj_imex_global_stats": (ctypes.c_int, [ctypes.c_void_p] * 3),
     "zj_ensemble_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "zj_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -57,6 +57,11 @@ class ShallowWater:
     # diffusion of those rows in IMEXSourceSolver (the model's scalar
     # diffusion operators, solver_imex_jax.py:396-460)
     diffusion: Optional[dict] = None
+    # alpha of the momentum source -alpha g h dh/dx_d, which reads the
+    # derivative aux rows: a source with derivative specs, which the IMEX
+    # solver's "auto" source mode integrates with the global Newton-GMRES
+    # stage (solver_imex_jax.py:182-188, :344-396); IMEX only
+    depth_gradient_source: float = 0.0
 
     @property
     def n_state(self):
@@ -230,6 +235,11 @@ class HyperbolicSolver:
         return Q
 
     def setup_simulation(self, mesh: Mesh, model: ShallowWater):
+        if model.depth_gradient_source and not isinstance(
+                self, IMEXSourceSolver):
+            raise NotImplementedError(
+                "the depth-gradient source is integrated implicitly by "
+                "IMEXSourceSolver only")
         if model.dimension != mesh.dim:
             raise ValueError(f"model dimension {model.dimension} does not "
                              f"match mesh dimension {mesh.dim}")
@@ -539,7 +549,20 @@ class IMEXSourceSolver(HyperbolicSolver):
     ``implicit_maxiter = 6``, :142)."""
 
     implicit_maxiter = 6
+    # the global stage (solver_imex_jax.py:140-146)
+    source_mode = "auto"          # "auto" | "local" | "global"
+    implicit_tol = 1e-8
+    gmres_tol = 1e-7
+    gmres_maxiter = 30
+    gmres_restart = 20
     _ROWS = {"b": 0, "h": 1, "hu": 2, "q_0": 2, "hv": 3, "q_1": 3}
+
+    def resolved_source_mode(self) -> str:
+        """"global" for a source that reads derivative aux rows, else
+        "local" (``_resolve_source_mode``, solver_imex_jax.py:182-188)."""
+        if self.source_mode in ("local", "global"):
+            return self.source_mode
+        return "global" if self.model.depth_gradient_source else "local"
 
     def step(self, dt, time, Q, Qaux=None, parameters=None):
         Q4 = self._full(Q)
@@ -547,10 +570,35 @@ class IMEXSourceSolver(HyperbolicSolver):
         for row, op in self._diffusion:
             Qe[row] = op.implicit_solve(Qe[row], float(dt))
         out = torch.empty_like(Q4)
+        if self._global is not None:
+            L.check(L.lib().zj_imex_global_solve(
+                self._global, float(dt), _p(Qe), _p(out),
+                int(self.implicit_maxiter), float(self.implicit_tol),
+                float(self.gmres_tol), int(self.gmres_maxiter), _stream()),
+                "zj_imex_global_solve")
+            return self._view(out)
         L.check(L.lib().zj_imex_local(self._rt.handle, _p(self._dt_dev(dt)),
                                       int(self.implicit_maxiter), _p(Qe),
                                       _p(out), _stream()), "zj_imex_local")
         return self._view(out)
+
+    def implicit_stats(self):
+        """{newton, gmres, fallbacks, rnorm} of the last global stage."""
+        if self._global is None:
+            return None
+        st = np.zeros(3, np.int32)
+        rn = ctypes.c_double()
+        L.check(L.lib().zj_imex_global_stats(self._global, st.ctypes.data,
+                                             ctypes.byref(rn)),
+                "zj_imex_global_stats")
+        return {"newton": int(st[0]), "gmres": int(st[1]),
+                "fallbacks": int(st[2]), "rnorm": float(rn.value)}
+
+    def __del__(self):
+        g = getattr(self, "_global", None)
+        if g is not None and L is not None and L._lib is not None:
+            L._lib.zj_imex_global_destroy(g)
+            self._global = None
 
     def setup_simulation(self, mesh, model):
         out = super().setup_simulation(mesh, model)
@@ -563,6 +611,18 @@ class IMEXSourceSolver(HyperbolicSolver):
         # aux, as the reference's IMEX explicit stage does
         # (solver_imex_jax.py:604-611)
         self._rt_h = self._make_runtime(m2)
+        self._global = None
+        if self.source_mode not in ("auto", "local", "global"):
+            raise ValueError("source_mode must be auto, local or global")
+        if self.resolved_source_mode() == "global":
+            if self.dtype != torch.float64:
+                raise ValueError("the global implicit source stage is FP64")
+            g = ctypes.c_void_p()
+            L.check(L.lib().zj_imex_global_create(
+                self._rt.handle, float(model.depth_gradient_source),
+                int(self.gmres_restart), ctypes.byref(g)),
+                "zj_imex_global_create")
+            self._global = g
         self._diffusion = []
         if model.diffusion:
             if self.dtype != torch.float64:
