@@ -610,6 +610,7 @@ struct EnsArgs {
     double* part2;
 };
 
+// SPEC = 2 fixes only the cell shape of 2-D triangle meshes (3 faces).
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
 // HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
@@ -627,12 +628,16 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     if (gate && *gate == 0) return;
     DevMesh m = m_in;
     Opts o = o_in;
-    if (SPEC) {
+    if (SPEC == 1) {
         m.dim = 2;
         m.mf = 4;
         o.lim = LIM_MINMOD;
         o.rs = RS_HLL;
         o.recon = RECON_CONS;
+    } else if (SPEC == 2) {
+        // triangle meshes: the face count fixed, the numerics at run time
+        m.dim = 2;
+        m.mf = 3;
     }
     using T2 = typename Vec2<T>::type;
     // dynamic LDS, 16-byte aligned (pairs of doubles)
@@ -1003,7 +1008,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         const T hic = Hs[ls];
         // the fluxes' 1/h: this stage's state, or in a frozen second stage
         // the step state's (the face reconstruction keeps this stage's)
-        const bool frz = !SPEC && o.stage_aux && mode == 1;
+        const bool frz = SPEC != 1 && o.stage_aux && mode == 1;
         const T hif = frz ? hinv_of(q0[1], eps) : hic;
         T sc[8];
         if (use_slopes) {
@@ -1483,10 +1488,13 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
                       (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                       s->o.recon == RECON_CONS && !s->o.stage_aux;
-    auto kern = ens         ? k_stage<T, 0, 0, 1>
+    const bool tri = M->d.dim == 2 && M->d.mf == 3;
+    auto kern = ens         ? (tri ? k_stage<T, 2, 0, 1> : k_stage<T, 0, 0, 1>)
                 : mode == 4 ? k_stage<T, 0, 2>
                 : mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
-                            : (spec ? k_stage<T, 1> : k_stage<T, 0>);
+                : spec      ? k_stage<T, 1>
+                : tri       ? k_stage<T, 2>
+                            : k_stage<T, 0>;
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
