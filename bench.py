@@ -384,13 +384,34 @@ def wetdry_tri_leg(steps, warmup):
     _lib.check(_lib.lib().zj_rcm(base.n_cells, ip.ctypes.data,
                                  ix.ctypes.data, rcm.ctypes.data), "zj_rcm")
     delaunay_s = time.perf_counter() - t_setup
+    # the RCM of this mesh against scipy's (a check, not a step of the leg)
+    try:
+        import numpy as np_
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        A = csr_matrix((np_.ones(len(ix)), ix, ip),
+                       shape=(base.n_cells, base.n_cells))
+        A.sort_indices()
+        sp = reverse_cuthill_mckee(A, symmetric_mode=True).astype(np_.int32)
+        if sp[-1] == rcm[-1]:
+            rcm_vs_scipy = bool(np_.array_equal(sp, rcm))
+        else:
+            # another start among equal degrees: compare from scipy's start
+            alt = np_.zeros(base.n_cells, np_.int32)
+            _lib.check(_lib.lib().zj_rcm_from(base.n_cells, ip.ctypes.data,
+                                              ix.ctypes.data, int(sp[-1]),
+                                              alt.ctypes.data), "zj_rcm_from")
+            rcm_vs_scipy = "same start: " + str(bool(np_.array_equal(sp, alt)))
+    except ImportError:
+        rcm_vs_scipy = None
     out = {"workload": "Delaunay of 2000x1000 jittered points on [0,10]x"
                        "[0,5], fractal bed + valley slope, reservoir h=3 in "
                        "x<2, dry elsewhere, walls; van Leer on the eta "
                        "reconstruction, HLL, front pre-detector 0.5 + MOOD, "
                        "FP32 state, FP64 mass",
            "cells": base.n_cells, "steps": steps,
-           "delaunay_and_rcm_s": delaunay_s}
+           "delaunay_and_rcm_s": delaunay_s,
+           "rcm_equals_scipy": rcm_vs_scipy}
     log(f"triangulated {base.n_cells} cells in {delaunay_s:.1f} s")
     for name, perm in (("rcm", rcm), ("morton", None)):
         log(f"wet/dry march in {name} order")

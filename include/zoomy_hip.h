@@ -87,6 +87,11 @@ int zj_mesh_destroy(zj_mesh *m);
  * scipy.sparse.csgraph.reverse_cuthill_mckee with a stable degree sort. */
 int zj_rcm(int32_t n, const int32_t *indptr, const int32_t *indices,
            int32_t *perm_out);
+/* The same ordering with the search of start's component begun at start
+ * (a caller's pseudo-peripheral node, or the start node another RCM chose
+ * among equal-degree candidates); the other components as in zj_rcm. */
+int zj_rcm_from(int32_t n, const int32_t *indptr, const int32_t *indices,
+                int32_t start, int32_t *perm_out);
 
 /* Host-only check of the LDS patch decomposition zj_mesh_create builds:
  * patches of P consecutive cells with their face ring and stencil ring,

@@ -62,7 +62,31 @@ def rcm_fixtures():
                             points=pts)
 
 
+def rcm_tied_fixture():
+    """scipy's RCM of the adjacency of a reduced wet/dry triangle mesh
+    (Delaunay of 100 x 50 jittered points on [0, 10] x [0, 5], seed 3), ties
+    of the degree left in place: scipy starts from np.argsort(degree)[0],
+    whose choice among equal degrees is numpy's unstable sort; the fixture
+    records the result and its start node perm[-1]."""
+    sys.path.insert(0, str(HERE.parents[1] / "zoomy-jax_amd"))
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    from zoomy_hip.mesh import delaunay_mesh, jittered_points
+    pts = jittered_points(100, 50, (0.0, 10.0, 0.0, 5.0), 0.3, 3)
+    m = delaunay_mesh(pts, (0.0, 10.0, 0.0, 5.0))
+    indptr, indices = m.adjacency_csr()
+    N = m.n_cells
+    A = csr_matrix((np.ones(len(indices)), indices, indptr), shape=(N, N))
+    A.sort_indices()
+    perm = reverse_cuthill_mckee(A, symmetric_mode=True)
+    np.savez_compressed(HERE / "rcm_tied_wetdry.npz",
+                        indptr=A.indptr.astype(np.int32),
+                        indices=A.indices.astype(np.int32),
+                        perm=perm.astype(np.int32))
+
+
 if __name__ == "__main__":
     copy_refs()
     rcm_fixtures()
+    rcm_tied_fixture()
     print("fixtures written to", HERE)

@@ -84,6 +84,35 @@ def test_rcm_matches_scipy(f):
     assert np.array_equal(out, d["perm"])
 
 
+def test_rcm_matches_scipy_with_tied_degrees():
+    """scipy's RCM of the adjacency of a reduced wet/dry triangle mesh
+    (9,702 Delaunay cells), the BFS levels full of equal degrees and no
+    de-tying edges: bit-exact, from the lowest-degree start and, through
+    zj_rcm_from, from the start node scipy chose (perm[-1])."""
+    d = np.load(GOLDEN / "rcm_tied_wetdry.npz")
+    n = len(d["indptr"]) - 1
+    ip = np.ascontiguousarray(d["indptr"], np.int32)
+    ix = np.ascontiguousarray(d["indices"], np.int32)
+    deg = np.diff(ip)
+    # many equal degrees inside the levels
+    assert np.bincount(deg).max() > n // 2
+    out = np.zeros(n, np.int32)
+    assert _lib.lib().zj_rcm(n, ip.ctypes.data, ix.ctypes.data,
+                             out.ctypes.data) == 0
+    assert np.array_equal(out, d["perm"])
+    out2 = np.zeros(n, np.int32)
+    assert _lib.lib().zj_rcm_from(n, ip.ctypes.data, ix.ctypes.data,
+                                  int(d["perm"][-1]), out2.ctypes.data) == 0
+    assert np.array_equal(out2, d["perm"])
+    # another start gives another valid permutation ending at that start
+    other = int(np.argmax(deg))
+    assert _lib.lib().zj_rcm_from(n, ip.ctypes.data, ix.ctypes.data, other,
+                                  out2.ctypes.data) == 0
+    assert out2[-1] == other and sorted(out2) == list(range(n))
+    assert _lib.lib().zj_rcm_from(n, ip.ctypes.data, ix.ctypes.data, n,
+                                  out2.ctypes.data) == _lib.ZJ_EINVAL
+
+
 def test_rcm_reduces_bandwidth():
     t = delaunay_mesh(jittered_points(40, 30, (0.0, 10.0, 0.0, 5.0), 0.3, 1),
                       (0.0, 10.0, 0.0, 5.0))

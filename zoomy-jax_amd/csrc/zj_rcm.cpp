@@ -16,9 +16,13 @@
 
 #include "../../include/zoomy_hip.h"
 
-extern "C" int zj_rcm(int32_t n, const int32_t* indptr, const int32_t* indices,
-                      int32_t* perm_out) {
-    if (n < 0 || (n > 0 && (!indptr || !perm_out))) return ZJ_EINVAL;
+namespace {
+
+// RCM with the component of `start` (>= 0) visited first from `start`; the
+// other components from their lowest-degree node.
+int rcm_impl(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t start,
+             int32_t* perm_out) {
+    if (n < 0 || (n > 0 && (!indptr || !perm_out)) || start >= n) return ZJ_EINVAL;
     if (n == 0) return ZJ_OK;
     std::vector<int32_t> degree(n);
     for (int32_t i = 0; i < n; ++i) {
@@ -41,11 +45,10 @@ extern "C" int zj_rcm(int32_t n, const int32_t* indptr, const int32_t* indices,
     std::vector<int32_t> rev(n);
     for (int32_t i = 0; i < n; ++i) rev[inds[i]] = i;
     std::vector<int32_t> order(n);
-    std::vector<int32_t> tmp;
     int32_t N = 0;
-    for (int32_t zz = 0; zz < n; ++zz) {
-        if (inds[zz] != -1) {
-            int32_t seed = inds[zz];
+    for (int32_t zz = start >= 0 ? -1 : 0; zz < n; ++zz) {
+        if (zz < 0 || inds[zz] != -1) {
+            int32_t seed = zz < 0 ? start : inds[zz];
             order[N++] = seed;
             inds[rev[seed]] = -1;
             int32_t level_start = N - 1, level_end = N;
@@ -80,4 +83,17 @@ extern "C" int zj_rcm(int32_t n, const int32_t* indptr, const int32_t* indices,
     }
     for (int32_t i = 0; i < n; ++i) perm_out[i] = order[n - 1 - i];
     return ZJ_OK;
+}
+
+}  // namespace
+
+extern "C" int zj_rcm(int32_t n, const int32_t* indptr, const int32_t* indices,
+                      int32_t* perm_out) {
+    return rcm_impl(n, indptr, indices, -1, perm_out);
+}
+
+extern "C" int zj_rcm_from(int32_t n, const int32_t* indptr, const int32_t* indices,
+                           int32_t start, int32_t* perm_out) {
+    if (start < 0) return ZJ_EINVAL;
+    return rcm_impl(n, indptr, indices, start, perm_out);
 }

@@ -78,6 +78,9 @@ _SIGS = {
     "zj_mesh_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "zj_rcm": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p,
                               ctypes.c_void_p, ctypes.c_void_p]),
+    "zj_rcm_from": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int32,
+                                   ctypes.c_void_p]),
     "zj_patch_check": (ctypes.c_int, [ctypes.POINTER(MeshDesc),
                                       ctypes.c_int32, ctypes.c_void_p]),
     "zj_solver_create": (ctypes.c_int, [ctypes.c_void_p,
