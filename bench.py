@@ -391,7 +391,6 @@ def wetdry_tri_leg(steps, warmup):
         from scipy.sparse.csgraph import reverse_cuthill_mckee
         A = csr_matrix((np_.ones(len(ix)), ix, ip),
                        shape=(base.n_cells, base.n_cells))
-        A.sort_indices()
         sp = reverse_cuthill_mckee(A, symmetric_mode=True).astype(np_.int32)
         if sp[-1] == rcm[-1]:
             rcm_vs_scipy = bool(np_.array_equal(sp, rcm))

@@ -680,14 +680,17 @@ class Mesh:
         return u
 
     def adjacency_csr(self):
-        """Face-neighbour graph as CSR (indptr, indices), int32."""
+        """Face-neighbour graph as CSR (indptr, indices), int32, the column
+        indices of each row ascending (the canonical CSR, as scipy's
+        sort_indices leaves it)."""
         N, MF = self.cell_nbr.shape
-        nb = self.cell_nbr
+        nb = self.cell_nbr.astype(np.int64)
         m = nb >= 0
         counts = m.sum(1)
         indptr = np.zeros(N + 1, np.int64)
         indptr[1:] = np.cumsum(counts)
-        indices = nb[m]
+        srt = np.sort(np.where(m, nb, np.iinfo(np.int64).max), axis=1)
+        indices = srt[srt != np.iinfo(np.int64).max]
         return indptr.astype(np.int32), indices.astype(np.int32)
 
 
