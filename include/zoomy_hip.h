@@ -66,6 +66,13 @@ typedef struct {
      * bface_value[v * n_bf + b]; the other rows extrapolate */
     const int32_t *bface_mask;
     const double *bface_value;
+    /* patch decomposition of the stage kernel (0, 0: automatic):
+     * patch_cells = cells per patch (a power of two, 32..256; halved while
+     * the FP64 working set exceeds the LDS budget); patch_order = 1:
+     * consecutive ids, 2: clusters of the Morton order of the cell centres
+     * (2-D), 0: consecutive ids unless their ring cells exceed 3x the patch
+     * and clusters do better */
+    int32_t patch_cells, patch_order;
 } zj_mesh_desc;
 
 typedef struct zj_mesh zj_mesh;       /* device-resident mesh   */
@@ -99,6 +106,10 @@ int zj_rcm_from(int32_t n, const int32_t *indptr, const int32_t *indices,
  * = {P, npatch, ntemplates, max_load, max_slope, max_vert}.  No counterpart
  * in the reference (it keeps no device mesh layout). */
 int zj_patch_check(const zj_mesh_desc *desc, int32_t P, int32_t *stats_out);
+/* Host-only: the decomposition zj_mesh_create picks for desc.  stats_out[7]
+ * = {P, npatch, ntemplates, max_load, max_slope, max_vert, clustered}.  No
+ * counterpart in the reference. */
+int zj_patch_plan(const zj_mesh_desc *desc, int32_t *stats_out);
 
 /* Solver context: owns the per-stage scratch (limited slopes, stage state,
  * reduction scalars).  Replaces HyperbolicSolver.setup_simulation

@@ -1,0 +1,75 @@
+"""The stage kernel's patch decomposition is a device layout only: patches of
+consecutive ids, patches cut from the Morton order of the cell centres (own
+cells listed explicitly) and smaller patches give bit-identical FP64
+marches, and the clustered layout matches the FP64 oracle like the
+consecutive one.  FP32 stages the centres as offsets from each patch's first
+own cell, so its marches under different layouts agree to FP32 round-off."""
+import numpy as np
+import pytest
+
+from helpers import circular_dam_state, rcm_wetdry_triangles
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _march(mesh, Q0, dtype, n, layout, solver_kw, model_kw):
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.fvm.solver import patch_plan
+    cells, order = layout
+    s = HyperbolicSolver(time_end=np.inf, cfl=0.9, dtype=dtype,
+                         patch_cells=cells, patch_order=order, **solver_kw)
+    s.setup_simulation(mesh, ShallowWater(dimension=2, **model_kw))
+    rc, st = patch_plan(mesh, cells, order)
+    assert rc == 0
+    print(f"layout {layout}: {st}", flush=True)
+    Q = torch.as_tensor(Q0, dtype=dtype, device="cuda").contiguous()
+    s.march_device(Q, n)
+    torch.cuda.synchronize()
+    return Q.cpu().numpy(), st
+
+
+LAYOUTS = [(0, "auto"), (0, "consecutive"), (0, "clusters"), (64, "clusters"),
+           (128, "consecutive")]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_wetdry_triangles_layouts_agree(dtype):
+    """The wet/dry triangle recipe in RCM order (eta, van Leer, HLL, front
+    detector, MOOD): 25 device steps under five patch layouts (FP32: the
+    same to 2e-5 of the depth and momentum scales)."""
+    mesh, Q0 = rcm_wetdry_triangles(100, 50)
+    kw = dict(reconstruction_variables="eta", mood=True, front_theta_tol=0.5)
+    mk = dict(order=2, limiter="vanleer", riemann="hll")
+    ref, st = _march(mesh, Q0, dtype, 25, LAYOUTS[0], kw, mk)
+    assert st["clustered"] == 1 and st["P"] == 256
+    assert np.isfinite(ref).all() and ref[1].min() >= 0.0
+    for lay in LAYOUTS[1:]:
+        Q, _ = _march(mesh, Q0, dtype, 25, lay, kw, mk)
+        if dtype == torch.float64:
+            assert np.array_equal(Q, ref), lay
+        else:
+            assert np.abs(Q[1] - ref[1]).max() <= 2e-5 * ref[1].max(), lay
+            qs = np.abs(ref[2:]).max()
+            assert np.abs(Q[2:] - ref[2:]).max() <= 2e-5 * qs, lay
+
+
+def test_strip_ordered_quads_clustered_layout_matches_oracle():
+    """Quads numbered along lines (a patch of consecutive ids is one line of
+    cells): the automatic layout clusters them; 20 FP64 device steps of the
+    circular dam break (minmod + HLL) against the oracle's march."""
+    from oracle.oracle import Oracle
+    from zoomy_hip.mesh import Mesh
+    mesh = Mesh.quad(48, 256).set_bc("wall")
+    Q0 = circular_dam_state(mesh, seed=2)
+    mk = dict(order=2, limiter="minmod", riemann="hll")
+    Qc, st = _march(mesh, Q0, torch.float64, 20, (0, "auto"), {}, mk)
+    assert st["clustered"] == 1
+    Qs, _ = _march(mesh, Q0, torch.float64, 20, (0, "consecutive"), {}, mk)
+    assert np.array_equal(Qc, Qs)
+    o = Oracle(mesh, cfl=0.9, **mk)
+    Qo, _, _ = o.march(Q0, n_steps=20)
+    for r in (1, 2, 3):
+        scale = max(np.abs(Qo[r]).max(), 1.0)
+        assert np.abs(Qc[r] - Qo[r]).max() <= 1e-12 * scale, r

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from zoomy_hip import _lib
-from zoomy_hip.fvm.solver import patch_stats
+from zoomy_hip.fvm.solver import patch_plan, patch_stats
 from zoomy_hip.mesh import Mesh, delaunay_mesh, jittered_points
 
 GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
@@ -70,6 +70,51 @@ def test_patch_templates_shared_on_regular_mesh():
     # interior 16x16 tiles share one template; only boundary tiles differ
     assert st["ntemplates"] <= 16
     assert st["max_load"] <= 448 and st["max_slope"] <= 320
+
+
+def _rcm_delaunay(nx, ny):
+    t = delaunay_mesh(jittered_points(nx, ny, (0.0, 10.0, 0.0, 5.0), 0.3, 3),
+                      (0.0, 10.0, 0.0, 5.0))
+    ip, ix = t.adjacency_csr()
+    perm = np.zeros(t.n_cells, np.int32)
+    assert _lib.lib().zj_rcm(t.n_cells, ip.ctypes.data, ix.ctypes.data,
+                             perm.ctypes.data) == 0
+    return t.reorder(perm)
+
+
+def test_patch_plan_clusters_strip_orders():
+    """The plan zj_mesh_create follows: a cell order whose id runs are thin
+    strips (RCM on a Delaunay mesh, row-major quads) is cut into clusters of
+    the Morton order of the centres, at the full patch size, with the ring
+    of a Morton-ordered mesh; Morton-ordered meshes keep consecutive ids.
+    Every plan validates (each cell owned by exactly one patch)."""
+    t = _rcm_delaunay(200, 100)
+    rc, cons = patch_plan(t, patch_order="consecutive")
+    assert rc == 0 and cons["clustered"] == 0 and cons["P"] < 256
+    rc, st = patch_plan(t)
+    assert rc == 0 and st["clustered"] == 1 and st["P"] == 256
+    assert st["max_load"] <= int(2.6 * 256)
+    tm = t.reorder(t.morton_permutation())
+    rc, sm = patch_plan(tm)
+    assert rc == 0 and sm["clustered"] == 0 and sm["P"] == 256
+    q = Mesh.quad(64, 256)        # a patch of consecutive ids is one line
+    rc, sq = patch_plan(q)
+    assert rc == 0 and sq["clustered"] == 1 and sq["max_load"] <= 448
+    qm = q.reorder(q.morton_permutation())
+    rc, sqm = patch_plan(qm)
+    assert rc == 0 and sqm["clustered"] == 0
+    rc, forced = patch_plan(qm, 64, "clusters")
+    assert rc == 0 and forced["clustered"] == 1 and forced["P"] == 64
+    assert forced["npatch"] == -(-qm.n_cells // 64)
+
+
+def test_patch_plan_arguments():
+    m = Mesh.quad(20, 20)
+    assert patch_plan(m, 100)[0] == _lib.ZJ_EINVAL      # not a power of two
+    assert patch_plan(m, 512)[0] == _lib.ZJ_EINVAL      # above the block
+    assert patch_plan(m, 16)[0] == _lib.ZJ_EINVAL
+    assert patch_plan(Mesh.line(100), 0,
+                      "clusters")[0] == _lib.ZJ_EUNSUP  # 2-D only
 
 
 @pytest.mark.parametrize("f", sorted(glob.glob(str(GOLDEN / "rcm_*.npz"))))

@@ -484,14 +484,17 @@ __global__ __launch_bounds__(BLOCK) void k_imex(DevMesh m, Opts o, const double*
 // ─────────────────────────────────────────────────────────────────────────
 // LDS-staged stage kernel: one block per patch of up to BLOCK own cells.
 // Phase 1 stages the state and centre of every cell the patch touches and
-// the vertices of its own and face-ring cells in LDS (coalesced: own cells
-// are contiguous ids); phase 2 computes the limited slopes of own and
+// the vertices of its own and face-ring cells in LDS (coalesced where the
+// own cells are contiguous ids; a mesh whose cell order cuts into thin
+// strips gets patches of spatial clusters, with their own cells listed
+// explicitly: expl); phase 2 computes the limited slopes of own and
 // face-ring cells from LDS; phase 3 computes the fluxes of the own cells.
 // No slope array round-trips through HBM and no gather chains global ->
 // global remain (boundary faces read their BC kind from global memory).
 // ─────────────────────────────────────────────────────────────────────────
 struct DevPatch {
     int P, npatch, ML, MS, MV;
+    int expl;          // own cells listed first in load_ids (else ids p P + i)
     const int4* meta;  // per patch: template, loaded cells, vertices, slope cells | own << 16
     const int* load_ids;
     const int* vert_ids;
@@ -504,23 +507,29 @@ struct DevPatch {
 constexpr uint16_t LNONE = 0xFFFF;
 
 // Face centres of a cell's slots from the staged vertices (cheap: no sqrt).
+template <typename V>
+__device__ __forceinline__ double2 c2d(V a) {
+    return make_double2((double)a.x, (double)a.y);
+}
+
+// Face centres of cell ls relative to org (the frame of the staged centres).
 __device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs,
-                                             const uint16_t* tv, int MS, int ls,
+                                             const uint16_t* tv, int MS, int ls, double2 org,
                                              double2* v, double* fx, double* fy) {
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? Vs[tv[k * MS + ls]] : make_double2(0.0, 0.0);
     if (m.dim == 1) {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            fx[k] = v[k].x;
+            fx[k] = v[k].x - org.x;
             fy[k] = 0.0;
         }
     } else {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             const double2 b = (k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0];
-            fx[k] = 0.5 * (v[k].x + b.x);
-            fy[k] = 0.5 * (v[k].y + b.y);
+            fx[k] = 0.5 * (v[k].x + b.x) - org.x;
+            fy[k] = 0.5 * (v[k].y + b.y) - org.y;
         }
     }
 }
@@ -650,23 +659,31 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const int4 md = pt.meta[p];
     const int tpl = md.x, nload = md.y, nvert = md.z;
     const int nslope = md.w & 0xFFFF, nown = md.w >> 16;
-    const int c0 = p * pt.P;  // own cells are the ids c0 .. c0 + nown - 1
-    double2* Cs = (double2*)smem_d;         // ML centres
-    double2* Vs = Cs + ML;                  // MV vertices
-    double* Rs = (double*)(Vs + MV);        // MS inradius (final stage)
+    const int c0 = p * pt.P;  // own cells: the ids c0 .. c0 + nown - 1, or
+                              // with expl the first nown entries of lids
+    const bool ex = pt.expl != 0;
+    T2* Cs = (T2*)smem_d;                   // ML centres (relative to org)
+    double2* Vs = (double2*)(Cs + ML);      // MV vertices
+    T* Rs = (T*)(Vs + MV);                  // MS inradius (final stage)
     T2* Qs = (T2*)(Rs + MS);                // 2 x ML state pairs
     T* Hs = (T*)(Qs + 2 * ML);              // ML hinv
     T2* Ss = (T2*)(Hs + ML);                // 4 x MS limited slope pairs (x, y)
     const T eps = T(o.eps), g = T(o.g);
     const int* lids = pt.load_ids + (size_t)p * ML;
+    // FP32 stages the centres as offsets from the patch's first own cell:
+    // their differences (the LSQ geometry and the face offsets) keep what
+    // FP32 slopes can use of the FP64 centres, in half the LDS; FP64 stages
+    // them as they are (org = 0).  Vertices stay FP64 in the global frame,
+    // so both cells of a face compute the same normal and length.
+    const double2 org = sizeof(T) == 4 ? pt.cen2[ex ? lids[0] : c0] : make_double2(0.0, 0.0);
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
     const uint16_t* te = pt.tpl_extra + (size_t)tpl * m.mx * MS;
     const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
 
     // ── phase 1: stage cells and vertices ────────────────────────────
     // Two slots per thread and pass, every global load of a pass issued
-    // before the first LDS store; a slot past the end reads cell c0 /
-    // vertex 0 and stores nothing.
+    // before the first LDS store; a slot past the end reads the first own
+    // cell / vertex 0 and stores nothing.
     {
         const int* vids = pt.vert_ids + (size_t)p * MV;
         const int nit = nload > nvert ? nload : nvert;
@@ -675,7 +692,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int i = i0 + u * SBLOCK;
-                gc[u] = i < nown ? c0 + i : (i < nload ? lids[i] : c0);
+                gc[u] = (i < nown && !ex) ? c0 + i : (i < nload ? lids[i] : ex ? lids[0] : c0);
                 gv[u] = i < nvert ? vids[i] : 0;
             }
             T2 a[2], b[2];
@@ -699,8 +716,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     Qs[i] = a[u];
                     Qs[ML + i] = b[u];
                     Hs[i] = hinv_of(a[u].y, eps);
-                    Cs[i] = cc[u];
-                    if (final_stage && i < nslope) Rs[i] = rr[u];
+                    T2 cl;
+                    cl.x = T(cc[u].x - org.x);
+                    cl.y = T(cc[u].y - org.y);
+                    Cs[i] = cl;
+                    if (final_stage && i < nslope) Rs[i] = T(rr[u]);
                 }
                 if (i < nvert) Vs[i] = vx[u];
             }
@@ -714,7 +734,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         // cells' gradients, no limiter, no slopes, no face pass
         const int nsl = AUXONLY == 1 ? nown : nslope;
         for (int ls = tid; ls < nsl; ls += SBLOCK) {
-            const int gi = ls < nown ? c0 + ls : lids[ls];
+            const int gi = (ls < nown && !ex) ? c0 + ls : lids[ls];
             uint16_t nls[MAXF];
 #pragma unroll
             for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
@@ -725,8 +745,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             to_w(o.recon, q, hic, w);
             double2 v[MAXF];
             double fx[MAXF], fy[MAXF];
-            face_centres(m, Vs, tv, MS, ls, v, fx, fy);
-            const double2 cc = Cs[ls];
+            face_centres(m, Vs, tv, MS, ls, org, v, fx, fy);
+            const double2 cc = c2d(Cs[ls]);
             double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
             double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
             T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
@@ -741,7 +761,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     T qn[4], wn[4];
                     lds_q4(Qs, ML, nl, qn);
                     to_w(o.recon, qn, Hs[nl], wn);
-                    const double2 cn = Cs[nl];
+                    const double2 cn = c2d(Cs[nl]);
                     const double rx = cn.x - cc.x, ry = cn.y - cc.y;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
@@ -788,7 +808,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 T qn[4], wn[4];
                 lds_q4(Qs, ML, el, qn);
                 to_w(o.recon, qn, Hs[el], wn);
-                const double2 cn = Cs[el];
+                const double2 cn = c2d(Cs[el]);
                 const double rx = cn.x - cc.x, ry = cn.y - cc.y;
                 Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
@@ -948,7 +968,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         // face arrays; a boundary face's Q_R is the ghost of the cell state
         // (the bf values of halo_bc, blocks.py:129-157)
         if (tid < nown) {
-            const int gi = c0 + tid;
+            const int gi = ex ? lids[tid] : c0 + tid;
             T q[4];
             lds_q4(Qs, ML, tid, q);
             const T hic = Hs[tid];
@@ -963,8 +983,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
             double2 v[MAXF];
             double fx[MAXF], fy[MAXF];
-            face_centres(m, Vs, tv, MS, tid, v, fx, fy);
-            const double2 cc = Cs[tid];
+            face_centres(m, Vs, tv, MS, tid, org, v, fx, fy);
+            const double2 cc = c2d(Cs[tid]);
             ZJ_FOR_FACES(k) {
                 T qf[4] = {q[0], q[1], q[2], q[3]};
                 if (use_slopes)
@@ -995,7 +1015,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     int gi_keep = -1;
     const int ls = tid;
     if (ls < nown) {
-        const int gi = c0 + ls;
+        const int gi = ex ? lids[ls] : c0 + ls;
         // the step state and dt of the RK average, fetched before the faces
         T q0[4] = {0, 0, 0, 0};
         if (mode == 1) load_q(Q0, N, gi, q0);
@@ -1021,8 +1041,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
         double2 v[MAXF];
         double fx[MAXF], fy[MAXF];
-        face_centres(m, Vs, tv, MS, ls, v, fx, fy);
-        const double2 cc = Cs[ls];
+        face_centres(m, Vs, tv, MS, ls, org, v, fx, fy);
+        const double2 cc = c2d(Cs[ls]);
         double nxs[MAXF], nys[MAXF];
         T acc[4] = {0, 0, 0, 0};
         ZJ_FOR_FACES(k) {
@@ -1041,7 +1061,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 lds_q4(Qs, ML, nl, qn);
                 hiR = Hs[nl];
                 hfR = hiR;
-                if (frz) hfR = hinv_of(Q0[(size_t)N + (nl < nown ? c0 + nl : lids[nl])], eps);
+                if (frz) hfR = hinv_of(Q0[(size_t)N + ((nl < nown && !ex) ? c0 + nl : lids[nl])], eps);
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
@@ -1050,7 +1070,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         sn[2 * vv] = sv.x;
                         sn[2 * vv + 1] = sv.y;
                     }
-                    const double2 cn = Cs[nl];
+                    const double2 cn = c2d(Cs[nl]);
                     recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
                 } else {
                     qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
@@ -1182,11 +1202,61 @@ struct zj_mesh {
     int* cell_face = nullptr;   // (mf, N) 2 f + side of each cell slot
 };
 
+// centres (2 words) + state and hinv (5 words) per loaded cell, inradius (1
+// word) + limited slopes (8 words) per slope cell, FP64 vertices
 static size_t stage_lds_bytes(const DevPatch& pt, size_t esz) {
-    return (size_t)pt.ML * 16 + (size_t)pt.MV * 16 + (size_t)pt.MS * 8 +
-           (size_t)pt.ML * 5 * esz + (size_t)pt.MS * 8 * esz;
+    return (size_t)pt.ML * 7 * esz + (size_t)pt.MV * 16 + (size_t)pt.MS * 9 * esz;
 }
-static constexpr size_t LDS_LIMIT = 64 * 1024;
+// FP64 working set of one patch: two blocks per CU (160 KB of LDS)
+static constexpr size_t LDS_LIMIT = 80 * 1024;
+
+// The patch decomposition zj_mesh_create builds: patches of the largest
+// size (BLOCK, or the caller's patch_cells) whose FP64 working set fits the
+// LDS budget, cut from consecutive ids or -- when that order drags in more
+// than 3x a patch's size of ring cells (orders whose id runs are thin
+// strips, such as RCM) and clusters do better, or when asked -- from
+// clusters of the Morton order of the cell centres.
+static int plan_patches(const zj_mesh_desc* d, zj::PatchHost& ph) {
+    const int N = d->n_cells, mf = d->max_faces, mx = d->max_extra;
+    std::vector<int32_t> ex_dummy(N, -1);
+    const int32_t* exh = mx ? d->cell_extra : ex_dummy.data();
+    int P0 = d->patch_cells > 0 ? d->patch_cells : BLOCK;
+    if (P0 < 32 || P0 > BLOCK || (P0 & (P0 - 1))) return ZJ_EINVAL;
+    if (d->patch_order < 0 || d->patch_order > 2) return ZJ_EINVAL;
+    if (d->patch_order == 2 && d->dim != 2) return ZJ_EUNSUP;
+    auto fit = [&](const int32_t* order, zj::PatchHost& out) -> int {
+        for (int P = P0;; P /= 2) {
+            if (zj::build_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_vertices, d->cell_nbr,
+                                  exh, P, order, out))
+                return ZJ_EINVAL;
+            DevPatch probe{};
+            probe.ML = out.max_load; probe.MS = out.max_slope; probe.MV = out.max_vert;
+            if (stage_lds_bytes(probe, 8) <= LDS_LIMIT) return ZJ_OK;
+            if (P <= 32) return ZJ_EUNSUP;
+        }
+    };
+    std::vector<int32_t> order;
+    int rc = ZJ_OK;
+    if (d->patch_order != 2) rc = fit(nullptr, ph);
+    const bool strips = rc != ZJ_OK || ph.max_load > 3 * ph.P;
+    if (d->patch_order == 2 || (d->patch_order == 0 && d->dim == 2 && strips)) {
+        order = zj::morton_cluster_order(N, d->cell_center);
+        zj::PatchHost pc;
+        const int rcc = fit(order.data(), pc);
+        if (rcc == ZJ_OK && (d->patch_order == 2 || rc != ZJ_OK || pc.P > ph.P ||
+                             (pc.P == ph.P && 5 * pc.max_load < 4 * ph.max_load))) {
+            ph = std::move(pc);
+            rc = ZJ_OK;
+        } else if (d->patch_order == 2) {
+            rc = rcc;
+        }
+    }
+    if (rc != ZJ_OK) return rc;
+    if (zj::validate_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_nbr, exh, d->cell_vertices,
+                             ph))
+        return ZJ_EINVAL;
+    return ZJ_OK;
+}
 
 struct zj_solver {
     zj_mesh* m;
@@ -1241,9 +1311,42 @@ static int dev_upload(zj_mesh* m, const void* src, size_t bytes, void** dst) {
 
 static inline int nblocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
+// Dynamic LDS above 64 KB is granted per kernel: every stage-kernel
+// instantiation of element type T may stage the mesh's patch working set.
+template <typename T>
+static int grant_stage_lds(size_t lds) {
+    if (lds <= 64 * 1024) return ZJ_OK;
+    const void* ks[] = {(const void*)k_stage<T, 0>,       (const void*)k_stage<T, 1>,
+                        (const void*)k_stage<T, 2>,       (const void*)k_stage<T, 0, 1>,
+                        (const void*)k_stage<T, 1, 1>,    (const void*)k_stage<T, 0, 2>,
+                        (const void*)k_stage<T, 0, 0, 1>, (const void*)k_stage<T, 2, 0, 1>};
+    for (const void* k : ks)
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+            return ZJ_EHIP;
+    return ZJ_OK;
+}
+
 extern "C" {
 
 const char* zj_version(void) { return "zoomy_hip 0.1 gfx950"; }
+
+int zj_patch_plan(const zj_mesh_desc* d, int32_t* stats_out) {
+    if (!d || !stats_out || d->n_cells <= 0 || d->dim < 1 || d->dim > 2 || d->max_faces < 2 ||
+        d->max_faces > MAXF || d->max_extra < 0)
+        return ZJ_EINVAL;
+    zj::PatchHost ph;
+    const int rc = plan_patches(d, ph);
+    if (rc != ZJ_OK) return rc;
+    stats_out[0] = ph.P;
+    stats_out[1] = ph.npatch;
+    stats_out[2] = ph.ntpl;
+    stats_out[3] = ph.max_load;
+    stats_out[4] = ph.max_slope;
+    stats_out[5] = ph.max_vert;
+    stats_out[6] = ph.expl;
+    return ZJ_OK;
+}
 
 int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     if (!d || !out) return ZJ_EINVAL;
@@ -1316,36 +1419,18 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         zj_mesh_destroy(m);
         return ZJ_ENOMEM;
     }
-    // patch decomposition for the LDS-staged stage kernel: the largest patch
-    // whose FP64 working set fits the LDS budget
+    // patch decomposition for the LDS-staged stage kernel
     {
         std::vector<int32_t> ex_dummy(N, -1);
-        const int32_t* exh = mx ? d->cell_extra : ex_dummy.data();
         zj::PatchHost ph;
-        int P = BLOCK;
-        for (;;) {
-            if (zj::build_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_vertices, d->cell_nbr,
-                                  exh, P, ph)) {
-                zj_mesh_destroy(m);
-                return ZJ_EINVAL;
-            }
-            DevPatch probe{};
-            probe.ML = ph.max_load; probe.MS = ph.max_slope; probe.MV = ph.max_vert;
-            if (stage_lds_bytes(probe, 8) <= LDS_LIMIT || P <= 32) break;
-            P /= 2;
-        }
-        if (zj::validate_patches(N, mf, mx ? mx : 1, d->n_verts, d->cell_nbr, exh,
-                                 d->cell_vertices, ph)) {
+        const int prc = plan_patches(d, ph);
+        if (prc != ZJ_OK) {
             zj_mesh_destroy(m);
-            return ZJ_EINVAL;
+            return prc;
         }
         DevPatch& pt = m->pt;
         pt.P = ph.P; pt.npatch = ph.npatch; pt.ML = ph.max_load; pt.MS = ph.max_slope;
-        pt.MV = ph.max_vert;
-        if (stage_lds_bytes(pt, 8) > LDS_LIMIT) {
-            zj_mesh_destroy(m);
-            return ZJ_EUNSUP;
-        }
+        pt.MV = ph.max_vert; pt.expl = ph.expl;
         std::vector<double> cen2((size_t)2 * N);
         for (int c = 0; c < N; ++c) {
             cen2[2 * (size_t)c] = d->cell_center[c];
@@ -1433,6 +1518,12 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
         hipMalloc((void**)&s->mass_part, sizeof(double2) * s->mass_blocks) != hipSuccess) {
         zj_solver_destroy(s);
         return ZJ_ENOMEM;
+    }
+    const int grc = esz == 8 ? grant_stage_lds<double>(stage_lds_bytes(m->pt, 8))
+                             : grant_stage_lds<float>(stage_lds_bytes(m->pt, 4));
+    if (grc != ZJ_OK) {
+        zj_solver_destroy(s);
+        return grc;
     }
     *out = s;
     return ZJ_OK;
@@ -2510,6 +2601,8 @@ int zj_ensemble_create(zj_solver* s, int32_t n_inst, const int32_t* cell_offsets
     // own cells of Q)
     if (s->o.order != 2 || s->mood || s->o.stage_aux || s->o.rs == RS_RUSANOV_NC)
         return ZJ_EUNSUP;
+    // the per-wave dt partials need patches of consecutive cells
+    if (s->m->pt.expl) return ZJ_EUNSUP;
     if (cell_offsets[0] != 0 || cell_offsets[n_inst] != N) return ZJ_EINVAL;
     std::vector<int32_t> inst(N);
     for (int i = 0; i < n_inst; ++i) {

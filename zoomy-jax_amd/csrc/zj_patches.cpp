@@ -1,8 +1,10 @@
 // Patch decomposition of the mesh for the LDS-staged stage kernel (host).
 //
-// Cells are cut into patches of P consecutive ids.  For patch p the kernel
-// needs, in LDS:
-//   own cells        [p P, p P + n_own)                       local 0..n_own-1
+// Cells are cut into patches of P consecutive ids, or of P consecutive
+// entries of a given cell order (spatial clusters, for cell orders whose id
+// runs are thin strips); the own cells are then listed explicitly.  For
+// patch p the kernel needs, in LDS:
+//   own cells        [p P, p P + n_own) (or their listed ids) local 0..n_own-1
 //   face ring        face neighbours of own cells             slopes computed
 //   outer ring       face + vertex neighbours of the above    state only
 //   vertices         of every own / face-ring cell            geometry
@@ -15,14 +17,16 @@
 
 #include "../../include/zoomy_hip.h"
 
+#include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <unordered_map>
 
 namespace zj {
 
 int build_patches(int N, int mf, int mx, int nverts, const int32_t* cv, const int32_t* nbr,
-                  const int32_t* extra, int P, PatchHost& out) {
+                  const int32_t* extra, int P, const int32_t* order, PatchHost& out) {
     if (P <= 0 || P > 1024) return 1;
     const int npatch = (N + P - 1) / P;
     std::vector<int32_t> gen(N, -1), loc(N, 0), vgen(nverts, -1), vloc(nverts, 0);
@@ -40,6 +44,7 @@ int build_patches(int N, int mf, int mx, int nverts, const int32_t* cv, const in
     };
     std::vector<Local> tpls;
     int max_load = 0, max_slope = 0, max_vert = 0;
+    std::vector<int32_t> own;
     for (int p = 0; p < npatch; ++p) {
         const int c0 = p * P, n_own = std::min(P, N - c0);
         std::vector<int32_t>& L = loads[p];
@@ -51,10 +56,15 @@ int build_patches(int N, int mf, int mx, int nverts, const int32_t* cv, const in
                 L.push_back(c);
             }
         };
-        for (int i = 0; i < n_own; ++i) add(c0 + i);
+        // own cells in ascending id order: runs of consecutive ids fall on
+        // consecutive threads (coalesced where the cell order allows)
+        own.resize(n_own);
+        for (int i = 0; i < n_own; ++i) own[i] = order ? order[c0 + i] : c0 + i;
+        if (order) std::sort(own.begin(), own.end());
+        for (int i = 0; i < n_own; ++i) add(own[i]);
         for (int i = 0; i < n_own; ++i)
             for (int k = 0; k < mf; ++k) {
-                int32_t nb = nbr[(size_t)k * N + c0 + i];
+                int32_t nb = nbr[(size_t)k * N + own[i]];
                 if (nb >= 0) add(nb);
             }
         const int n_slope = (int)L.size();
@@ -121,6 +131,7 @@ int build_patches(int N, int mf, int mx, int nverts, const int32_t* cv, const in
     // pad to multiples of 64 so LDS arrays of the kernel stay aligned
     auto up = [](int x) { return (x + 63) / 64 * 64; };
     out.P = P;
+    out.expl = order ? 1 : 0;
     out.npatch = npatch;
     out.mf = mf;
     out.mx = mx > 0 ? mx : 1;
@@ -167,6 +178,7 @@ int build_patches(int N, int mf, int mx, int nverts, const int32_t* cv, const in
 int validate_patches(int N, int mf, int mx, int nverts, const int32_t* nbr, const int32_t* extra,
                      const int32_t* cv, const PatchHost& ph) {
     const int S = ph.max_slope;
+    std::vector<uint8_t> owned(N, 0);
     for (int p = 0; p < ph.npatch; ++p) {
         const int t = ph.tpl_id[p];
         if (t < 0 || t >= ph.ntpl) return 10;
@@ -179,8 +191,11 @@ int validate_patches(int N, int mf, int mx, int nverts, const int32_t* nbr, cons
             if (L[i] < 0 || L[i] >= N) return 12;
         for (int i = 0; i < nv; ++i)
             if (V[i] < 0 || V[i] >= nverts) return 13;
-        for (int i = 0; i < no; ++i)
-            if (L[i] != p * ph.P + i) return 14;
+        if (!ph.expl && no != std::min(ph.P, N - p * ph.P)) return 11;
+        for (int i = 0; i < no; ++i) {
+            if (!ph.expl && L[i] != p * ph.P + i) return 14;
+            if (owned[L[i]]++) return 14;  // owned twice
+        }
         for (int ls = 0; ls < ns; ++ls) {
             const int32_t c = L[ls];
             for (int k = 0; k < mf; ++k) {
@@ -206,7 +221,36 @@ int validate_patches(int N, int mf, int mx, int nverts, const int32_t* nbr, cons
             }
         }
     }
+    for (int c = 0; c < N; ++c)
+        if (!owned[c]) return 21;  // owned by no patch
     return 0;
+}
+
+std::vector<int32_t> morton_cluster_order(int N, const double* cen) {
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < N; ++c) {
+            lo[a] = std::min(lo[a], cen[(size_t)a * N + c]);
+            hi[a] = std::max(hi[a], cen[(size_t)a * N + c]);
+        }
+    // one scale for both axes, so that clusters are square in space
+    const double span = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+    std::vector<uint64_t> key(N);
+    for (int c = 0; c < N; ++c) {
+        uint32_t q[2];
+        for (int a = 0; a < 2; ++a) {
+            const double u = (cen[(size_t)a * N + c] - lo[a]) / span;
+            q[a] = (uint32_t)std::min(65535.0, std::max(0.0, u * 65536.0));
+        }
+        uint64_t code = 0;
+        for (int b = 15; b >= 0; --b)
+            code = (code << 2) | (((q[1] >> b) & 1u) << 1) | ((q[0] >> b) & 1u);
+        key[c] = (code << 32) | (uint32_t)c;
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> order(N);
+    for (int c = 0; c < N; ++c) order[c] = (int32_t)(key[c] & 0xFFFFFFFFu);
+    return order;
 }
 
 }  // namespace zj
@@ -222,7 +266,7 @@ extern "C" int zj_patch_check(const zj_mesh_desc* d, int32_t P, int32_t* stats_o
     const int32_t* exh = d->max_extra > 0 ? d->cell_extra : ex_dummy.data();
     zj::PatchHost ph;
     if (zj::build_patches(N, d->max_faces, mx, d->n_verts, d->cell_vertices, d->cell_nbr, exh, P,
-                          ph))
+                          nullptr, ph))
         return ZJ_EINVAL;
     int rc = zj::validate_patches(N, d->max_faces, mx, d->n_verts, d->cell_nbr, exh,
                                   d->cell_vertices, ph);

@@ -20,6 +20,7 @@ RECON = {"conservative": 0, "eta": 1, "xz": 2, "eta_hu": 3}
 POSITIVITY = {"": 0, "zhang_shu": 1}
 STAGE_AUX = {"refresh": 0, "frozen": 1}
 DTYPES = {"float64": 0, "float32": 1}
+PATCH_ORDER = {"auto": 0, "consecutive": 1, "clusters": 2}
 
 
 class ZoomyHipError(RuntimeError):
@@ -41,7 +42,9 @@ class MeshDesc(ctypes.Structure):
                 ("bface_kind", ctypes.c_void_p),
                 ("bface_partner", ctypes.c_void_p),
                 ("bface_mask", ctypes.c_void_p),
-                ("bface_value", ctypes.c_void_p)]
+                ("bface_value", ctypes.c_void_p),
+                ("patch_cells", ctypes.c_int32),
+                ("patch_order", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -81,6 +84,8 @@ _SIGS = {
     "zj_rcm_from": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_int32,
                                    ctypes.c_void_p]),
+    "zj_patch_plan": (ctypes.c_int, [ctypes.POINTER(MeshDesc),
+                                     ctypes.c_void_p]),
     "zj_patch_check": (ctypes.c_int, [ctypes.POINTER(MeshDesc),
                                       ctypes.c_int32, ctypes.c_void_p]),
     "zj_solver_create": (ctypes.c_int, [ctypes.c_void_p,

@@ -125,7 +125,8 @@ class EnsembleSolver:
             np.int32)
         union = Mesh.union(meshes)
         self.mesh = union
-        self._dmesh = DeviceMesh(union)
+        # the per-wave dt partials need patches of consecutive cells
+        self._dmesh = DeviceMesh(union, patch_order="consecutive")
         self._rt = _Runtime(self._dmesh, self.model, self.cfl, self.dtype,
                             "conservative")
         h = ctypes.c_void_p()

@@ -82,8 +82,11 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def mesh_desc(mesh: Mesh):
-    """``zj_mesh_desc`` of a host mesh; returns (desc, arrays kept alive)."""
+def mesh_desc(mesh: Mesh, patch_cells: int = 0, patch_order: str = "auto"):
+    """``zj_mesh_desc`` of a host mesh; returns (desc, arrays kept alive).
+    ``patch_cells`` / ``patch_order`` ("auto", "consecutive", "clusters")
+    steer the stage kernel's patch decomposition (0 / "auto": chosen by the
+    library)."""
     N = mesh.n_cells
     ex = mesh.extra_stencil()
     dmask, dval = mesh.bface_dirichlet()
@@ -107,8 +110,19 @@ def mesh_desc(mesh: Mesh):
                    keep["ex"].shape[0], mesh.n_bfaces,
                    *[keep[n].ctypes.data for n in (
                        "cv", "nbr", "ex", "verts", "cen", "rin", "bcell",
-                       "bslot", "bkind", "bpart", "dmask", "dval")])
+                       "bslot", "bkind", "bpart", "dmask", "dval")],
+                   int(patch_cells), L.PATCH_ORDER[patch_order])
     return d, keep
+
+
+def patch_plan(mesh: Mesh, patch_cells: int = 0, patch_order: str = "auto"):
+    """Host-only: the patch decomposition ``zj_mesh_create`` picks."""
+    d, keep = mesh_desc(mesh, patch_cells, patch_order)
+    st = (ctypes.c_int32 * 7)()
+    rc = L.lib().zj_patch_plan(ctypes.byref(d),
+                               ctypes.cast(st, ctypes.c_void_p))
+    return rc, dict(zip(("P", "npatch", "ntemplates", "max_load",
+                         "max_slope", "max_vert", "clustered"), list(st)))
 
 
 def patch_stats(mesh: Mesh, P: int = 256):
@@ -124,10 +138,11 @@ def patch_stats(mesh: Mesh, P: int = 256):
 class DeviceMesh:
     """Mesh uploaded to the device (``zj_mesh_create``)."""
 
-    def __init__(self, mesh: Mesh):
+    def __init__(self, mesh: Mesh, patch_cells: int = 0,
+                 patch_order: str = "auto"):
         self.mesh = mesh
         lib = L.lib()
-        d, self._keep = mesh_desc(mesh)
+        d, self._keep = mesh_desc(mesh, patch_cells, patch_order)
         h = ctypes.c_void_p()
         L.check(lib.zj_mesh_create(ctypes.byref(d), ctypes.byref(h)),
                 "zj_mesh_create")
@@ -181,7 +196,8 @@ class HyperbolicSolver:
                  compute_dt: Optional[Callable] = None, mood: bool = False,
                  positivity_method: str = "",
                  front_theta_tol: Optional[float] = None,
-                 stage_aux: str = "refresh"):
+                 stage_aux: str = "refresh", patch_cells: int = 0,
+                 patch_order: str = "auto"):
         """``reconstruction_variables``: ``"conservative"``, ``"eta"``
         (well-balanced wet/dry recipe on (b, eta, u)), ``"eta_hu"`` (the
         same on (b, eta, hu) with the 1e-3 depth threshold of the class it
@@ -192,11 +208,18 @@ class HyperbolicSolver:
         (``solver_jax.py:196-226``).  ``stage_aux``: ``"refresh"`` (the
         second SSP-RK2 stage's fluxes read that stage's 1/h,
         ``solver_jax.py:1026-1041``) or ``"frozen"`` (the step-start aux,
-        with which the reference's order-2 goldens were recorded)."""
+        with which the reference's order-2 goldens were recorded).
+        ``patch_cells`` / ``patch_order`` steer the device layout only (the
+        stage kernel's patches; results do not depend on them)."""
         if stage_aux not in L.STAGE_AUX:
             raise ValueError(
                 f"stage_aux must be one of {sorted(L.STAGE_AUX)}")
         self.stage_aux = stage_aux
+        if patch_order not in L.PATCH_ORDER:
+            raise ValueError(
+                f"patch_order must be one of {sorted(L.PATCH_ORDER)}")
+        self.patch_cells = int(patch_cells)
+        self.patch_order = patch_order
         if reconstruction_variables not in L.RECON:
             raise ValueError(
                 f"reconstruction_variables must be one of {sorted(L.RECON)}")
@@ -248,7 +271,7 @@ class HyperbolicSolver:
         self.mesh = mesh
         self.model = model
         self.n_state = model.n_state
-        self._dmesh = DeviceMesh(mesh)
+        self._dmesh = DeviceMesh(mesh, self.patch_cells, self.patch_order)
         self._rt = self._make_runtime(model)
         dev = _device()
         self.device = dev

@@ -583,10 +583,11 @@ class Mesh:
         ``gnn_blueprint/mesh_spectral_geometry.py:34-46``).  On a 2^k x 2^k
         grid every run of 4^m consecutive cells is a 2^m x 2^m tile, so a
         block of 256 cells gathers its neighbours from one 16 x 16 tile and
-        its one-cell ring."""
+        its one-cell ring.  Both axes share one scale, so that runs of the
+        curve are square in space on any domain."""
         xc = self.cell_centers()
         lo = self.verts.min(0)
-        span = np.maximum(self.verts.max(0) - lo, 1e-300)
+        span = max(float((self.verts.max(0) - lo).max()), 1e-300)
         q = np.floor((xc - lo) / span * (1 << bits)).astype(np.int64)
         q = np.clip(q, 0, (1 << bits) - 1)
         key = np.zeros(len(xc), np.int64)
