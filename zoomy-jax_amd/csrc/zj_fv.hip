@@ -622,8 +622,9 @@ struct EnsArgs {
 // SPEC = 2 fixes only the cell shape of 2-D triangle meshes (3 faces).
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
-// HLL flux, conservative variables); SPEC = 0 reads them from the mesh and
-// the options.  Both instantiations evaluate the same formulas.  AUXONLY = 1
+// HLL flux, conservative variables), SPEC = 3 the same numerics on triangle
+// meshes; SPEC = 0 reads them from the mesh and the options.  Every
+// instantiation evaluates the same formulas.  AUXONLY = 1
 // is the post_step aux refresh alone: staging and the own cells' gradients;
 // AUXONLY = 2 the reconstruction object alone (face states, no fluxes).
 template <typename T, int SPEC, int AUXONLY = 0, int ENS = 0>
@@ -637,9 +638,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     if (gate && *gate == 0) return;
     DevMesh m = m_in;
     Opts o = o_in;
-    if (SPEC == 1) {
+    if (SPEC == 1 || SPEC == 3) {
         m.dim = 2;
-        m.mf = 4;
+        m.mf = SPEC == 1 ? 4 : 3;
         o.lim = LIM_MINMOD;
         o.rs = RS_HLL;
         o.recon = RECON_CONS;
@@ -1028,7 +1029,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         const T hic = Hs[ls];
         // the fluxes' 1/h: this stage's state, or in a frozen second stage
         // the step state's (the face reconstruction keeps this stage's)
-        const bool frz = SPEC != 1 && o.stage_aux && mode == 1;
+        const bool frz = SPEC != 1 && SPEC != 3 && o.stage_aux && mode == 1;
         const T hif = frz ? hinv_of(q0[1], eps) : hic;
         T sc[8];
         if (use_slopes) {
@@ -1317,9 +1318,11 @@ template <typename T>
 static int grant_stage_lds(size_t lds) {
     if (lds <= 64 * 1024) return ZJ_OK;
     const void* ks[] = {(const void*)k_stage<T, 0>,       (const void*)k_stage<T, 1>,
-                        (const void*)k_stage<T, 2>,       (const void*)k_stage<T, 0, 1>,
-                        (const void*)k_stage<T, 1, 1>,    (const void*)k_stage<T, 0, 2>,
-                        (const void*)k_stage<T, 0, 0, 1>, (const void*)k_stage<T, 2, 0, 1>};
+                        (const void*)k_stage<T, 2>,       (const void*)k_stage<T, 3>,
+                        (const void*)k_stage<T, 0, 1>,    (const void*)k_stage<T, 1, 1>,
+                        (const void*)k_stage<T, 3, 1>,    (const void*)k_stage<T, 0, 2>,
+                        (const void*)k_stage<T, 0, 0, 1>, (const void*)k_stage<T, 2, 0, 1>,
+                        (const void*)k_stage<T, 3, 0, 1>};
     for (const void* k : ks)
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
             hipSuccess)
@@ -1576,14 +1579,16 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   const EnsArgs* ens = nullptr) {
     const zj_mesh* M = s->m;
     const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
-    const bool spec = M->d.dim == 2 && M->d.mf == 4 &&
-                      (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
-                      s->o.recon == RECON_CONS && !s->o.stage_aux;
+    const bool fixed = (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
+                       s->o.recon == RECON_CONS && !s->o.stage_aux;
     const bool tri = M->d.dim == 2 && M->d.mf == 3;
-    auto kern = ens         ? (tri ? k_stage<T, 2, 0, 1> : k_stage<T, 0, 0, 1>)
+    const bool spec = M->d.dim == 2 && M->d.mf == 4 && fixed;
+    const bool spec_tri = tri && fixed;
+    auto kern = ens ? (spec_tri ? k_stage<T, 3, 0, 1> : tri ? k_stage<T, 2, 0, 1> : k_stage<T, 0, 0, 1>)
                 : mode == 4 ? k_stage<T, 0, 2>
-                : mode == 3 ? (spec ? k_stage<T, 1, 1> : k_stage<T, 0, 1>)
+                : mode == 3 ? (spec ? k_stage<T, 1, 1> : spec_tri ? k_stage<T, 3, 1> : k_stage<T, 0, 1>)
                 : spec      ? k_stage<T, 1>
+                : spec_tri  ? k_stage<T, 3>
                 : tri       ? k_stage<T, 2>
                             : k_stage<T, 0>;
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
