@@ -499,12 +499,13 @@ struct DevPatch {
     const int* load_ids;
     const int* vert_ids;
     const uint16_t* tpl_nbr;
-    const uint16_t* tpl_extra;
+    const uint32_t* tpl_extra2;  // extra-stencil local ids, packed in pairs
     const uint16_t* tpl_vert;
     const double2* cen2;
 };
 
 constexpr uint16_t LNONE = 0xFFFF;
+
 
 // Face centres of a cell's slots from the staged vertices (cheap: no sqrt).
 template <typename V>
@@ -650,6 +651,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         m.mf = 3;
     }
     using T2 = typename Vec2<T>::type;
+    // extra-stencil ids fetched per batch on triangle meshes (a vertex
+    // stencil of up to 12 cells); the batch of the run-time numerics is
+    // smaller, for its register budget
+    constexpr int MXB = SPEC == 3 ? 12 : SPEC == 2 ? 6 : 0;
     // dynamic LDS, 16-byte aligned (pairs of doubles)
     extern __shared__ double2 smem_v2[];
     double* smem_d = reinterpret_cast<double*>(smem_v2);
@@ -678,7 +683,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // so both cells of a face compute the same normal and length.
     const double2 org = sizeof(T) == 4 ? pt.cen2[ex ? lids[0] : c0] : make_double2(0.0, 0.0);
     const uint16_t* tn = pt.tpl_nbr + (size_t)tpl * m.mf * MS;
-    const uint16_t* te = pt.tpl_extra + (size_t)tpl * m.mx * MS;
+    // word j2 of slope cell ls: extra-stencil entries 2 j2 (low half) and
+    // 2 j2 + 1 (high half)
+    const int mxh = (m.mx + 1) >> 1;
+    const uint32_t* te = pt.tpl_extra2 + (size_t)tpl * mxh * MS;
     const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
 
     // ── phase 1: stage cells and vertices ────────────────────────────
@@ -803,9 +811,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     }
                 }
             }
-            for (int j = 0; j < m.mx; ++j) {
-                const uint16_t el = te[j * MS + ls];
-                if (el == LNONE) break;
+            auto extra_row = [&](uint16_t el) {
                 T qn[4], wn[4];
                 lds_q4(Qs, ML, el, qn);
                 to_w(o.recon, qn, Hs[el], wn);
@@ -817,6 +823,30 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     T dv = wn[vv] - w[vv];
                     bx[vv] += T(rx) * dv;
                     by[vv] += T(ry) * dv;
+                }
+            };
+            if (MXB > 0) {
+                // the ids of a batch of MXB entries are fetched before the
+                // first is used: one round trip per batch, not per entry
+                for (int b2 = 0; b2 < mxh; b2 += MXB / 2) {
+                    uint32_t els[MXB > 0 ? MXB / 2 : 1];
+#pragma unroll
+                    for (int u = 0; u < MXB / 2; ++u)
+                        els[u] = b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
+#pragma unroll
+                    for (int j = 0; j < MXB; ++j) {
+                        const uint16_t el = (uint16_t)(els[j >> 1] >> (16 * (j & 1)));
+                        if (el != LNONE) extra_row(el);
+                    }
+                }
+            } else {
+                for (int j2 = 0; j2 < mxh; ++j2) {
+                    const uint32_t pair = te[j2 * MS + ls];
+                    const uint16_t a = (uint16_t)(pair & 0xFFFFu), b = (uint16_t)(pair >> 16);
+                    if (a == LNONE) break;
+                    extra_row(a);
+                    if (b == LNONE) break;
+                    extra_row(b);
                 }
             }
             T gx[4], gy[4], ax[4], ay[4];
@@ -1452,7 +1482,22 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         rc |= dev_upload(m, ph.load_ids.data(), sizeof(int) * ph.load_ids.size(), &p); pt.load_ids = (const int*)p;
         rc |= dev_upload(m, ph.vert_ids.data(), sizeof(int) * ph.vert_ids.size(), &p); pt.vert_ids = (const int*)p;
         rc |= dev_upload(m, ph.tpl_nbr.data(), 2 * ph.tpl_nbr.size(), &p); pt.tpl_nbr = (const uint16_t*)p;
-        rc |= dev_upload(m, ph.tpl_extra.data(), 2 * ph.tpl_extra.size(), &p); pt.tpl_extra = (const uint16_t*)p;
+        {
+            // extra-stencil ids packed in pairs (one 32-bit load per two)
+            const int S = ph.max_slope, mxh = (ph.mx + 1) / 2;
+            std::vector<uint32_t> te2((size_t)ph.ntpl * mxh * S);
+            for (int t = 0; t < ph.ntpl; ++t)
+                for (int j2 = 0; j2 < mxh; ++j2)
+                    for (int ls = 0; ls < S; ++ls) {
+                        const uint32_t lo = ph.tpl_extra[((size_t)t * ph.mx + 2 * j2) * S + ls];
+                        const uint32_t hi = 2 * j2 + 1 < ph.mx
+                                                ? ph.tpl_extra[((size_t)t * ph.mx + 2 * j2 + 1) * S + ls]
+                                                : 0xFFFFu;
+                        te2[((size_t)t * mxh + j2) * S + ls] = lo | hi << 16;
+                    }
+            rc |= dev_upload(m, te2.data(), 4 * te2.size(), &p);
+            pt.tpl_extra2 = (const uint32_t*)p;
+        }
         rc |= dev_upload(m, ph.tpl_vert.data(), 2 * ph.tpl_vert.size(), &p); pt.tpl_vert = (const uint16_t*)p;
         rc |= dev_upload(m, cen2.data(), sizeof(double) * cen2.size(), &p); pt.cen2 = (const double2*)p;
         // the kernels index m.mx stencil rows: keep at least one (all -1)
@@ -2518,16 +2563,22 @@ __global__ void k_ens_advance(int n, double* __restrict__ t, const double* __res
     if (i < n) t[i] += dt[i];
 }
 
-// Compensated FP64 mass of each instance (one block per instance, fixed
-// fold order).
+// Compensated FP64 mass of each instance: every instance is cut into chunks
+// of MCHUNK cells (chunk b holds cells [lo[b], hi[b]) of one instance), one
+// block per chunk writes its compensated partial, and k_mass_inst_fold adds
+// an instance's chunk partials in order -- a fixed decomposition and fold
+// order, so the result is deterministic.
+constexpr int MCHUNK = 8192;
+
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_mass_inst(DevMesh m, const T* __restrict__ Q,
-                                                     const int* __restrict__ off,
-                                                     double* __restrict__ out) {
+__global__ __launch_bounds__(BLOCK) void k_mass_chunk(DevMesh m, const T* __restrict__ Q,
+                                                      const int* __restrict__ lo,
+                                                      const int* __restrict__ hi,
+                                                      double2* __restrict__ part) {
     __shared__ double2 lds[BLOCK];
-    const int i = blockIdx.x;
+    const int b = blockIdx.x;
     double s = 0.0, comp = 0.0;
-    for (int c = off[i] + threadIdx.x; c < off[i + 1]; c += BLOCK) {
+    for (int c = lo[b] + threadIdx.x; c < hi[b]; c += BLOCK) {
         double2 v[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; ++k)
@@ -2536,7 +2587,21 @@ __global__ __launch_bounds__(BLOCK) void k_mass_inst(DevMesh m, const T* __restr
     }
     lds[threadIdx.x] = make_double2(s, comp);
     fold_pairs<BLOCK>(lds);
-    if (threadIdx.x == 0) out[i] = lds[0].x + lds[0].y;
+    if (threadIdx.x == 0) part[b] = lds[0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_mass_inst_fold(int n, const int* __restrict__ chunk0,
+                                                          const double2* __restrict__ part,
+                                                          double* __restrict__ out) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0, comp = 0.0;
+    for (int b = chunk0[i]; b < chunk0[i + 1]; ++b) {
+        const double2 p = part[b];
+        neumaier(s, comp, p.x);
+        comp += p.y;
+    }
+    out[i] = s + comp;
 }
 
 }  // namespace
@@ -2546,6 +2611,11 @@ struct zj_ensemble {
     int n = 0;
     int* off = nullptr;        // (n + 1) cell offsets, device
     int* inst = nullptr;       // (N) instance of each cell, device
+    int nchunk = 0;            // mass chunks (MCHUNK cells of one instance)
+    int* chunk_lo = nullptr;   // (nchunk) first cell, device
+    int* chunk_hi = nullptr;   // (nchunk) one past the last cell, device
+    int* chunk0 = nullptr;     // (n + 1) first chunk of each instance, device
+    double2* mpart = nullptr;  // (nchunk) compensated partials
     double* part2 = nullptr;   // two dt partials per wave
     double* red = nullptr;     // (n) per-instance dt fold
     hipStream_t cap = nullptr; // stream the graph is captured on
@@ -2633,17 +2703,37 @@ int zj_ensemble_create(zj_solver* s, int32_t n_inst, const int32_t* cell_offsets
     e->s = s;
     e->n = n_inst;
     const size_t nwaves = (size_t)std::max(s->m->pt.npatch, nblocks(N)) * (SBLOCK / 64);
+    std::vector<int32_t> clo, chi, c0(n_inst + 1);
+    for (int i = 0; i < n_inst; ++i) {
+        c0[i] = (int32_t)clo.size();
+        for (int c = cell_offsets[i]; c < cell_offsets[i + 1]; c += MCHUNK) {
+            clo.push_back(c);
+            chi.push_back(std::min(c + MCHUNK, cell_offsets[i + 1]));
+        }
+    }
+    c0[n_inst] = (int32_t)clo.size();
+    e->nchunk = (int)clo.size();
     if (hipMalloc((void**)&e->off, sizeof(int) * (n_inst + 1)) != hipSuccess ||
         hipMalloc((void**)&e->inst, sizeof(int) * N) != hipSuccess ||
         hipMalloc((void**)&e->part2, sizeof(double) * 2 * nwaves) != hipSuccess ||
         hipMalloc((void**)&e->red, sizeof(double) * n_inst) != hipSuccess ||
+        hipMalloc((void**)&e->chunk_lo, sizeof(int) * e->nchunk) != hipSuccess ||
+        hipMalloc((void**)&e->chunk_hi, sizeof(int) * e->nchunk) != hipSuccess ||
+        hipMalloc((void**)&e->chunk0, sizeof(int) * (n_inst + 1)) != hipSuccess ||
+        hipMalloc((void**)&e->mpart, sizeof(double2) * e->nchunk) != hipSuccess ||
         hipStreamCreate(&e->cap) != hipSuccess) {
         zj_ensemble_destroy(e);
         return ZJ_ENOMEM;
     }
     if (hipMemcpy(e->off, cell_offsets, sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice) !=
             hipSuccess ||
-        hipMemcpy(e->inst, inst.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(e->inst, inst.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->chunk_lo, clo.data(), sizeof(int) * e->nchunk, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(e->chunk_hi, chi.data(), sizeof(int) * e->nchunk, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(e->chunk0, c0.data(), sizeof(int) * (n_inst + 1), hipMemcpyHostToDevice) !=
+            hipSuccess) {
         zj_ensemble_destroy(e);
         return ZJ_EHIP;
     }
@@ -2656,7 +2746,8 @@ int zj_ensemble_destroy(zj_ensemble* e) {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     if (e->cap) (void)hipStreamDestroy(e->cap);
-    for (void* p : {(void*)e->off, (void*)e->inst, (void*)e->part2, (void*)e->red})
+    for (void* p : {(void*)e->off, (void*)e->inst, (void*)e->part2, (void*)e->red,
+                    (void*)e->chunk_lo, (void*)e->chunk_hi, (void*)e->chunk0, (void*)e->mpart})
         if (p) (void)hipFree(p);
     delete e;
     return ZJ_OK;
@@ -2745,11 +2836,15 @@ int zj_ensemble_mass(zj_ensemble* e, const void* Q, double* mass_out, void* stre
     hipStream_t st = (hipStream_t)stream;
     const DevMesh& m = e->s->m->d;
     if (e->s->p.dtype == ZJ_F64)
-        hipLaunchKernelGGL(k_mass_inst<double>, dim3(e->n), dim3(BLOCK), 0, st, m,
-                           (const double*)Q, (const int*)e->off, mass_out);
+        hipLaunchKernelGGL(k_mass_chunk<double>, dim3(e->nchunk), dim3(BLOCK), 0, st, m,
+                           (const double*)Q, (const int*)e->chunk_lo, (const int*)e->chunk_hi,
+                           e->mpart);
     else
-        hipLaunchKernelGGL(k_mass_inst<float>, dim3(e->n), dim3(BLOCK), 0, st, m, (const float*)Q,
-                           (const int*)e->off, mass_out);
+        hipLaunchKernelGGL(k_mass_chunk<float>, dim3(e->nchunk), dim3(BLOCK), 0, st, m,
+                           (const float*)Q, (const int*)e->chunk_lo, (const int*)e->chunk_hi,
+                           e->mpart);
+    hipLaunchKernelGGL(k_mass_inst_fold, dim3((e->n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st,
+                       e->n, (const int*)e->chunk0, (const double2*)e->mpart, mass_out);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
