@@ -506,6 +506,26 @@ struct DevPatch {
 
 constexpr uint16_t LNONE = 0xFFFF;
 
+// The template rows of local cell ls that a pass reads from global memory:
+// face neighbours and vertices (MAXF each) and the first batch of NB packed
+// extra-stencil pairs.  Loaded ahead of the barrier before the pass, so
+// that their latency overlaps the wait.
+template <int NB>
+struct TplRow {
+    uint16_t nbr[MAXF], vert[MAXF];
+    uint32_t ex[NB > 0 ? NB : 1];
+    __device__ __forceinline__ void load(const uint16_t* tn, const uint16_t* tv,
+                                         const uint32_t* te, int mf, int mxh, int MS, int ls) {
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            nbr[k] = k < mf ? tn[k * MS + ls] : LNONE;
+            vert[k] = k < mf ? tv[k * MS + ls] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) ex[u] = u < mxh ? te[u * MS + ls] : ~0u;
+    }
+};
+
 
 // Face centres of a cell's slots from the staged vertices (cheap: no sqrt).
 template <typename V>
@@ -513,12 +533,13 @@ __device__ __forceinline__ double2 c2d(V a) {
     return make_double2((double)a.x, (double)a.y);
 }
 
-// Face centres of cell ls relative to org (the frame of the staged centres).
+// Face centres of a cell with local vertex ids vl, relative to org (the
+// frame of the staged centres).
 __device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs,
-                                             const uint16_t* tv, int MS, int ls, double2 org,
-                                             double2* v, double* fx, double* fy) {
+                                             const uint16_t* vl, double2 org, double2* v,
+                                             double* fx, double* fy) {
 #pragma unroll
-    for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? Vs[tv[k * MS + ls]] : make_double2(0.0, 0.0);
+    for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? Vs[vl[k]] : make_double2(0.0, 0.0);
     if (m.dim == 1) {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
@@ -735,18 +756,25 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
         }
     }
+    // the first slope pass's template rows, in flight during the barrier
+    // (the quad instantiation's templates are shared by most patches and
+    // cache-resident: in FP64 it loads them in the pass, for its register
+    // budget)
+    constexpr bool PRE_TPL = !(SPEC == 1 && sizeof(T) == 8);
+    const int nsl = AUXONLY == 1 ? nown : nslope;
+    TplRow<MXB / 2> row;
+    if (PRE_TPL && use_slopes && tid < nsl) row.load(tn, tv, te, m.mf, mxh, MS, tid);
     __syncthreads();
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
     if (use_slopes) {
         // AUXONLY (post_step aux rows only, launched as mode 3): the own
         // cells' gradients, no limiter, no slopes, no face pass
-        const int nsl = AUXONLY == 1 ? nown : nslope;
         for (int ls = tid; ls < nsl; ls += SBLOCK) {
+            if (!PRE_TPL || ls != tid) row.load(tn, tv, te, m.mf, mxh, MS, ls);
             const int gi = (ls < nown && !ex) ? c0 + ls : lids[ls];
-            uint16_t nls[MAXF];
-#pragma unroll
-            for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
+            const uint16_t* nls = row.nbr;
+            const uint32_t* pre = row.ex;
             T q[4];
             lds_q4(Qs, ML, ls, q);
             const T hic = Hs[ls];
@@ -754,7 +782,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             to_w(o.recon, q, hic, w);
             double2 v[MAXF];
             double fx[MAXF], fy[MAXF];
-            face_centres(m, Vs, tv, MS, ls, org, v, fx, fy);
+            face_centres(m, Vs, row.vert, org, v, fx, fy);
             const double2 cc = c2d(Cs[ls]);
             double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
             double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
@@ -832,7 +860,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     uint32_t els[MXB > 0 ? MXB / 2 : 1];
 #pragma unroll
                     for (int u = 0; u < MXB / 2; ++u)
-                        els[u] = b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
+                        els[u] = b2 == 0 ? pre[u] : b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
 #pragma unroll
                     for (int j = 0; j < MXB; ++j) {
                         const uint16_t el = (uint16_t)(els[j >> 1] >> (16 * (j & 1)));
@@ -991,15 +1019,31 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
             }
         }
-        __syncthreads();
     }
+    // the face pass's global inputs (own cell id, step state and dt of the
+    // RK average, template rows), in flight during the barrier; in-place
+    // stage 2 reads only the own cell of Q0, which this thread writes last
+    int gi3 = 0;
+    T q0[4] = {0, 0, 0, 0};
+    T dt3 = T(0);
+    TplRow<0> row3;
+    if (AUXONLY != 1 && tid < nown) {
+        gi3 = ex ? lids[tid] : c0 + tid;
+        if (PRE_TPL) row3.load(tn, tv, te, m.mf, mxh, MS, tid);
+        if (AUXONLY == 0) {
+            if (mode == 1) load_q(Q0, N, gi3, q0);
+            dt3 = mode == 2 ? T(0) : T(ENS ? ea.dts[ea.inst[gi3]] : *dtp);
+        }
+    }
+    if (use_slopes) __syncthreads();
     if (AUXONLY == 1) return;
+    if (!PRE_TPL && tid < nown) row3.load(tn, tv, te, m.mf, mxh, MS, tid);
     if (AUXONLY == 2) {
         // the reconstruction object: each own cell's face values into the
         // face arrays; a boundary face's Q_R is the ghost of the cell state
         // (the bf values of halo_bc, blocks.py:129-157)
         if (tid < nown) {
-            const int gi = ex ? lids[tid] : c0 + tid;
+            const int gi = gi3;
             T q[4];
             lds_q4(Qs, ML, tid, q);
             const T hic = Hs[tid];
@@ -1014,7 +1058,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
             double2 v[MAXF];
             double fx[MAXF], fy[MAXF];
-            face_centres(m, Vs, tv, MS, tid, org, v, fx, fy);
+            face_centres(m, Vs, row3.vert, org, v, fx, fy);
             const double2 cc = c2d(Cs[tid]);
             ZJ_FOR_FACES(k) {
                 T qf[4] = {q[0], q[1], q[2], q[3]};
@@ -1046,14 +1090,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     int gi_keep = -1;
     const int ls = tid;
     if (ls < nown) {
-        const int gi = ex ? lids[ls] : c0 + ls;
-        // the step state and dt of the RK average, fetched before the faces
-        T q0[4] = {0, 0, 0, 0};
-        if (mode == 1) load_q(Q0, N, gi, q0);
-        const T dt = mode == 2 ? T(0) : T(ENS ? ea.dts[ea.inst[gi]] : *dtp);
-        uint16_t nls[MAXF];
-#pragma unroll
-        for (int k = 0; k < MAXF; ++k) nls[k] = k < m.mf ? tn[k * MS + ls] : LNONE;
+        const int gi = gi3;
+        const T dt = dt3;
+        const uint16_t* nls = row3.nbr;
         T q[4];
         lds_q4(Qs, ML, ls, q);
         const T hic = Hs[ls];
@@ -1072,7 +1111,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
         double2 v[MAXF];
         double fx[MAXF], fy[MAXF];
-        face_centres(m, Vs, tv, MS, ls, org, v, fx, fy);
+        face_centres(m, Vs, row3.vert, org, v, fx, fy);
         const double2 cc = c2d(Cs[ls]);
         double nxs[MAXF], nys[MAXF];
         T acc[4] = {0, 0, 0, 0};
