@@ -711,25 +711,27 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const uint16_t* tv = pt.tpl_vert + (size_t)tpl * m.mf * MS;
 
     // ── phase 1: stage cells and vertices ────────────────────────────
-    // Two slots per thread and pass, every global load of a pass issued
+    // SL slots per thread and pass (triangle patches: one pass for up to
+    // 768 loaded cells or vertices), every global load of a pass issued
     // before the first LDS store; a slot past the end reads the first own
     // cell / vertex 0 and stores nothing.
     {
+        constexpr int SL = SPEC == 1 ? 2 : 3;
         const int* vids = pt.vert_ids + (size_t)p * MV;
         const int nit = nload > nvert ? nload : nvert;
-        for (int i0 = tid; i0 < nit; i0 += 2 * SBLOCK) {
-            int gc[2], gv[2];
+        for (int i0 = tid; i0 < nit; i0 += SL * SBLOCK) {
+            int gc[SL], gv[SL];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < SL; ++u) {
                 const int i = i0 + u * SBLOCK;
                 gc[u] = (i < nown && !ex) ? c0 + i : (i < nload ? lids[i] : ex ? lids[0] : c0);
                 gv[u] = i < nvert ? vids[i] : 0;
             }
-            T2 a[2], b[2];
-            double2 cc[2], vx[2];
-            double rr[2];
+            T2 a[SL], b[SL];
+            double2 cc[SL], vx[SL];
+            double rr[SL];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < SL; ++u) {
                 const int gi = gc[u];
                 a[u].x = Qin[gi];
                 a[u].y = Qin[N + gi];
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 vx[u] = m.verts[gv[u]];
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < SL; ++u) {
                 const int i = i0 + u * SBLOCK;
                 if (i < nload) {
                     Qs[i] = a[u];
