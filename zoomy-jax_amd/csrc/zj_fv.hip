@@ -1664,7 +1664,10 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                   const int* gate = nullptr, ReconOut<T> ro = ReconOut<T>{},
                   const EnsArgs* ens = nullptr) {
     const zj_mesh* M = s->m;
-    const size_t lds = stage_lds_bytes(M->pt, sizeof(T));
+    // the aux-only pass stores no limited slopes: its LDS ends before them
+    const size_t lds = mode == 3 ? stage_lds_bytes(M->pt, sizeof(T)) -
+                                       (size_t)M->pt.MS * 8 * sizeof(T)
+                                 : stage_lds_bytes(M->pt, sizeof(T));
     const bool fixed = (s->o.lim == LIM_BJ || s->o.lim == LIM_MINMOD) && s->o.rs == RS_HLL &&
                        s->o.recon == RECON_CONS && !s->o.stage_aux;
     const bool tri = M->d.dim == 2 && M->d.mf == 3;
