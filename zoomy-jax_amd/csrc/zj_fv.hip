@@ -645,8 +645,10 @@ struct EnsArgs {
 // SPEC = 1 compiles the quad-mesh path of the 2-D workload with every
 // run-time switch fixed (dim 2, 4 faces, Barth-Jespersen / minmod limiter,
 // HLL flux, conservative variables), SPEC = 3 the same numerics on triangle
-// meshes; SPEC = 0 reads them from the mesh and the options.  Every
-// instantiation evaluates the same formulas.  AUXONLY = 1
+// meshes, SPEC = 4 the wet/dry numerics on triangle meshes (van Leer on the
+// eta reconstruction, HLL; the positivity guards stay run-time switches);
+// SPEC = 0 reads them from the mesh and the options.  Every instantiation
+// evaluates the same formulas.  AUXONLY = 1
 // is the post_step aux refresh alone: staging and the own cells' gradients;
 // AUXONLY = 2 the reconstruction object alone (face states, no fluxes).
 template <typename T, int SPEC, int AUXONLY = 0, int ENS = 0>
@@ -670,12 +672,18 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         // triangle meshes: the face count fixed, the numerics at run time
         m.dim = 2;
         m.mf = 3;
+    } else if (SPEC == 4) {
+        m.dim = 2;
+        m.mf = 3;
+        o.lim = LIM_VANLEER;
+        o.rs = RS_HLL;
+        o.recon = RECON_ETA;
     }
     using T2 = typename Vec2<T>::type;
     // extra-stencil ids fetched per batch on triangle meshes (a vertex
     // stencil of up to 12 cells); the batch of the run-time numerics is
     // smaller, for its register budget
-    constexpr int MXB = SPEC == 3 ? 12 : SPEC == 2 ? 6 : SPEC == 1 ? 4 : 0;
+    constexpr int MXB = SPEC == 3 ? 12 : (SPEC == 2 || SPEC == 4) ? 6 : SPEC == 1 ? 4 : 0;
     // dynamic LDS, 16-byte aligned (pairs of doubles)
     extern __shared__ double2 smem_v2[];
     double* smem_d = reinterpret_cast<double*>(smem_v2);
@@ -1390,6 +1398,7 @@ static int grant_stage_lds(size_t lds) {
     if (lds <= 64 * 1024) return ZJ_OK;
     const void* ks[] = {(const void*)k_stage<T, 0>,       (const void*)k_stage<T, 1>,
                         (const void*)k_stage<T, 2>,       (const void*)k_stage<T, 3>,
+                        (const void*)k_stage<T, 4>,
                         (const void*)k_stage<T, 0, 1>,    (const void*)k_stage<T, 1, 1>,
                         (const void*)k_stage<T, 3, 1>,    (const void*)k_stage<T, 0, 2>,
                         (const void*)k_stage<T, 0, 0, 1>, (const void*)k_stage<T, 2, 0, 1>,
@@ -1673,13 +1682,16 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
     const bool tri = M->d.dim == 2 && M->d.mf == 3;
     const bool spec = M->d.dim == 2 && M->d.mf == 4 && fixed;
     const bool spec_tri = tri && fixed;
+    const bool wetdry_tri = tri && s->o.lim == LIM_VANLEER && s->o.rs == RS_HLL &&
+                            s->o.recon == RECON_ETA;
     auto kern = ens ? (spec_tri ? k_stage<T, 3, 0, 1> : tri ? k_stage<T, 2, 0, 1> : k_stage<T, 0, 0, 1>)
-                : mode == 4 ? k_stage<T, 0, 2>
-                : mode == 3 ? (spec ? k_stage<T, 1, 1> : spec_tri ? k_stage<T, 3, 1> : k_stage<T, 0, 1>)
-                : spec      ? k_stage<T, 1>
-                : spec_tri  ? k_stage<T, 3>
-                : tri       ? k_stage<T, 2>
-                            : k_stage<T, 0>;
+                : mode == 4  ? k_stage<T, 0, 2>
+                : mode == 3  ? (spec ? k_stage<T, 1, 1> : spec_tri ? k_stage<T, 3, 1> : k_stage<T, 0, 1>)
+                : spec       ? k_stage<T, 1>
+                : spec_tri   ? k_stage<T, 3>
+                : wetdry_tri ? k_stage<T, 4>
+                : tri        ? k_stage<T, 2>
+                             : k_stage<T, 0>;
     hipLaunchKernelGGL(kern, dim3(M->pt.npatch), dim3(SBLOCK),
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
