@@ -1705,15 +1705,16 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
 template <typename T>
 int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled,
               unsigned long long* dt_red, hipStream_t st, T* aux_of_Q = nullptr,
-              bool copy_b = true) {
+              bool copy_b = true, bool copy_b_out = true) {
     const DevMesh& m = s->m->d;
     T* Q1 = (T*)s->Q1;
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
     // solvers that never change b leave the stage buffer's b row alone: it
-    // is copied once per call (once per march: copy_b on the first step)
+    // is copied once per call (once per march: copy_b on the first step),
+    // and into an output buffer unless it already holds it (copy_b_out)
     if (!write_b && copy_b)
         HIPCHK(hipMemcpyAsync(Q1, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
-    if (!write_b && Qout != Q)
+    if (!write_b && Qout != Q && copy_b_out)
         HIPCHK(hipMemcpyAsync(Qout, Q, sizeof(T) * m.N, hipMemcpyDeviceToDevice, st));
     if (s->o.order < 2) {
         launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, (T*)nullptr, 0, 1, write_b, 0, troubled,
@@ -1742,11 +1743,13 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
         HIPCHK(hipGetLastError());
         return ZJ_OK;
     }
-    // A-posteriori MOOD (solver_jax.py:1055-1071): the candidate goes to Qc
-    // with its troubled mask; a redo with the flagged cells at first order
-    // runs on the device only when the candidate had troubled cells (the
-    // redo kernels read the count and return at once when it is 0).
-    T* Qc = (T*)s->Qc;
+    // A-posteriori MOOD (solver_jax.py:1055-1071): the candidate goes to the
+    // output (to Qc when the step runs in place: the redo needs Q) with its
+    // troubled mask; a redo with the flagged cells at first order runs on
+    // the device only when the candidate had troubled cells (the redo
+    // kernels read the count and return at once when it is 0).
+    const bool in_place = (const void*)Qout == (const void*)Q;
+    T* Qc = in_place ? (T*)s->Qc : Qout;
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, (unsigned long long*)nullptr,
                        s->mood_cnt);
     launch_stage<T>(s, (const T*)Q1, Q, Qc, dt, (T*)nullptr, 1, 1, write_b, 1, s->mood_cnt, dt_red,
@@ -1760,8 +1763,9 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
                     s->mood_mask, (uint8_t*)nullptr, s->mood_cnt);
     hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt,
                        s->mood_stats);
-    HIPCHK(hipMemcpyAsync(Qout + (write_b ? 0 : m.N), Qc + (write_b ? 0 : m.N),
-                          sizeof(T) * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice, st));
+    if (in_place)
+        HIPCHK(hipMemcpyAsync(Qout + (write_b ? 0 : m.N), Qc + (write_b ? 0 : m.N),
+                              sizeof(T) * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1921,20 +1925,37 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     // step is order 2; the state after the last step gets its own pass.
     const bool fuse = Qaux && s->o.order >= 2 &&
                       (s->o.recon == RECON_CONS || s->o.recon == RECON_XZ);
+    // with MOOD the steps alternate between Q and the solver's Qc (the
+    // candidate and its redo are written out of place), so no step copies
+    // its candidate back; an odd step count ends with one copy into Q
+    const bool ping = s->mood && s->o.order >= 2;
+    const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
+    const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
+    if (ping && !write_b && n_steps > 0)
+        HIPCHK(hipMemcpyAsync(s->Qc, Q, esz * m.N, hipMemcpyDeviceToDevice, st));
+    void* buf[2] = {Q, ping ? s->Qc : Q};
     for (int k = 0; k < n_steps; ++k) {
+        void* in = buf[k & 1];
+        void* out = buf[(k + 1) & 1];
         hipLaunchKernelGGL(k_dt_next, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev, t_dev, t_end,
                            s->p.dt_max, k > 0 ? 1 : 0);
         int rc = s->p.dtype == ZJ_F64
-                     ? step_impl(s, dt_dev, (const double*)Q, (double*)Q, troubled_out, s->dt_red, st,
-                                 fuse && k > 0 ? (double*)Qaux : (double*)nullptr, k == 0)
-                     : step_impl(s, dt_dev, (const float*)Q, (float*)Q, troubled_out, s->dt_red, st,
-                                 fuse && k > 0 ? (float*)Qaux : (float*)nullptr, k == 0);
+                     ? step_impl(s, dt_dev, (const double*)in, (double*)out, troubled_out,
+                                 s->dt_red, st, fuse && k > 0 ? (double*)Qaux : (double*)nullptr,
+                                 k == 0, !ping)
+                     : step_impl(s, dt_dev, (const float*)in, (float*)out, troubled_out,
+                                 s->dt_red, st, fuse && k > 0 ? (float*)Qaux : (float*)nullptr,
+                                 k == 0, !ping);
         if (rc) return rc;
         if (Qaux && !fuse) {
-            rc = zj_update_aux(s, Q, Qaux, stream);
+            rc = zj_update_aux(s, out, Qaux, stream);
             if (rc) return rc;
         }
     }
+    if (ping && (n_steps & 1))
+        HIPCHK(hipMemcpyAsync((char*)Q + (write_b ? 0 : esz * m.N),
+                              (const char*)s->Qc + (write_b ? 0 : esz * m.N),
+                              esz * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice, st));
     if (n_steps > 0)
         hipLaunchKernelGGL(k_t_advance, dim3(1), dim3(1), 0, st, t_dev, (const double*)dt_dev);
     if (Qaux && fuse && n_steps > 0) {
