@@ -522,6 +522,33 @@ def test_mood_step_parity():
     assert max_rel_rows(Qg, Qo) < 1e-12
 
 
+@pytest.mark.parametrize("n", [7, 8])
+def test_mood_march_matches_oracle_mood_steps(n):
+    """The device MOOD march (the state and the candidate buffer alternate
+    between steps; the head kernel picks the redo's dt when a redo ran and
+    keeps the statistics) against the oracle's MOOD steps: the same state
+    to round-off after an odd and an even number of steps, and the same
+    count of redone steps and demoted cells."""
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    mesh, Q, o = _mood_case()
+    model = ShallowWater(dimension=2, order=2, limiter="vanleer", riemann="hll")
+    s = HyperbolicSolver(time_end=np.inf, cfl=0.9,
+                         reconstruction_variables="eta", mood=True)
+    s.setup_simulation(mesh, model)
+    Qd = _to_dev(Q)
+    s.march_device(Qd, n)
+    Qo = Q.copy()
+    redone = demoted = 0
+    for _ in range(n):
+        Qo, bad, nd = o.step_mood(o.dt(Qo), Qo)
+        if nd:
+            redone += 1
+            demoted += nd
+    assert redone >= 2
+    assert s.mood_stats() == (redone, demoted)
+    assert max_rel_rows(Qd.cpu().numpy(), Qo) < 1e-11
+
+
 def test_mood_march_keeps_depth_nonnegative_and_mass():
     from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
     mesh, Q, o = _mood_case()
