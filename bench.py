@@ -334,6 +334,73 @@ def diffusion_leg(mesh, Q0, reps=5):
             "finite": bool(torch.isfinite(out).all())}
 
 
+def imex_global_leg(mesh, Q0, reps=3):
+    """One IMEX step with the global implicit source stage on the 2048 x 2048
+    dam-break mesh (the reference's _build_implicit_global,
+    solver_imex_jax.py:344-396): explicit MUSCL-HLL stage, then Newton-GMRES
+    on R(Q) = Q - Qexp - dt S(Q, aux(Q)) with the Manning friction (n =
+    0.03) and the depth-gradient source -alpha g h dh/dx_d (alpha = 0.5),
+    which reads the LSQ derivative aux rows; seeded velocities of 0.5 so
+    that the friction acts; Newton to 1e-8, GMRES(20) to 1e-7, FP64.  A
+    GMRES iteration is one exact Jacobian-vector product (a patch-kernel
+    gradient pass of v and the source linearisation) plus the classical
+    Gram-Schmidt work."""
+    import numpy as np
+    import torch
+    from zoomy_hip.fvm import IMEXSourceSolver, ShallowWater
+    model = ShallowWater(dimension=2, order=2, limiter="minmod",
+                         riemann="hll", manning_n=0.03,
+                         depth_gradient_source=0.5,
+                         boundary_conditions="wall")
+    s = IMEXSourceSolver(time_end=float("inf"), cfl=0.9)
+    s.setup_simulation(mesh, model)
+    rng = np.random.default_rng(11)
+    Q = Q0.copy()
+    Q[2] = Q[1] * rng.uniform(-0.5, 0.5, mesh.n_cells)
+    Q[3] = Q[1] * rng.uniform(-0.5, 0.5, mesh.n_cells)
+    Qd = torch.as_tensor(Q, dtype=torch.float64, device="cuda").contiguous()
+    dt = float(s.compute_timestep(Qd, None))
+    out = s.step(dt, 0.0, Qd)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        out = s.step(dt, 0.0, Qd)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_step = e0.elapsed_time(e1) / reps
+    # the global stage alone, from the explicit stage's output
+    Qe = s._hyperbolic_only(dt, s._full(Qd))
+    buf = torch.empty_like(Qe)
+    import ctypes
+    from zoomy_hip import _lib as L
+    args = (s._global, float(dt), ctypes.c_void_p(Qe.data_ptr()),
+            ctypes.c_void_p(buf.data_ptr()), int(s.implicit_maxiter),
+            float(s.implicit_tol), float(s.gmres_tol),
+            int(s.gmres_maxiter), ctypes.c_void_p(st.cuda_stream))
+    L.check(L.lib().zj_imex_global_solve(*args), "zj_imex_global_solve")
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        L.check(L.lib().zj_imex_global_solve(*args), "zj_imex_global_solve")
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_stage = e0.elapsed_time(e1) / reps
+    stats = s.implicit_stats()
+    return {"workload": "one IMEX step on the 2048x2048 dam-break mesh: "
+                        "explicit MUSCL-HLL stage + global Newton-GMRES "
+                        "source stage (Manning 0.03, depth-gradient "
+                        "source 0.5, seeded velocities), FP64",
+            "dt": dt, "ms_per_step": ms_step,
+            "ms_per_global_stage": ms_stage,
+            "newton_steps": stats["newton"], "gmres_iterations": stats["gmres"],
+            "us_per_gmres_iteration": 1e3 * ms_stage / max(stats["gmres"], 1),
+            "rnorm": stats["rnorm"], "fallbacks": stats["fallbacks"],
+            "finite": bool(torch.isfinite(out).all())}
+
+
 def wetdry_tri_bed(xc, seed=3):
     """Seeded fractal bed on [0, 10] x [0, 5]: four octaves of random-phase
     plane waves (amplitude halving, frequency doubling), scaled to a
@@ -545,6 +612,10 @@ def main():
     ap.add_argument("--no-chorin", action="store_true")
     ap.add_argument("--chorin-only", action="store_true")
     ap.add_argument("--no-diffusion", action="store_true")
+    ap.add_argument("--no-imex", action="store_true",
+                    help="skip the global IMEX source-stage leg")
+    ap.add_argument("--imex-only", action="store_true",
+                    help="only the global IMEX source-stage leg")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched ensemble leg (256 instances, "
                          "~62M triangles; a minute or two of set-up)")
@@ -585,6 +656,8 @@ def main():
         print(json.dumps({"wetdry_tri": wetdry_tri_leg(args.steps,
                                                        args.warmup)}))
         return
+    if args.imex_only:
+        args.no_chorin = True
     chorin = None
     if not args.no_chorin:
         log("chorin legs (plain, mixed, learned, learned_guess)")
@@ -634,6 +707,9 @@ def main():
     mesh = mesh.reorder(mesh.morton_permutation()).set_bc("wall")
     Q0 = build_state(mesh)
     N = mesh.n_cells
+    if args.imex_only:
+        print(json.dumps({"imex_global": imex_global_leg(mesh, Q0)}))
+        return
     log("FP64 and FP32 marches on the 2048x2048 quad mesh")
     r64 = None if args.no_fp64 else gpu_leg(mesh, Q0, torch.float64,
                                              args.steps, args.warmup)
@@ -682,6 +758,9 @@ def main():
     if not args.no_diffusion:
         log("diffusion leg")
         out["diffusion"] = diffusion_leg(mesh, Q0)
+    if not args.no_imex:
+        log("global IMEX source-stage leg")
+        out["imex_global"] = imex_global_leg(mesh, Q0)
     if not args.no_wetdry:
         log("wet/dry triangle leg (Delaunay of 2M points, RCM and Morton)")
         out["wetdry_tri"] = wetdry_tri_leg(args.steps, args.warmup)

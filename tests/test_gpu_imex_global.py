@@ -68,7 +68,8 @@ def test_global_stage_matches_oracle_newton(kind):
     Qg = out.cpu().numpy()
     rows = (1, 2, 3) if mesh.dim == 2 else (1, 2)
     assert max_rel_rows(Qg, Qo, rows) < 1e-10
-    # the coupling matters: the local stage gives another state
+    # the depth-gradient term matters: the local stage (Manning only) gives
+    # another state
     Ql = o.imex_local(dt, Q, maxiter=30)
     assert max_rel_rows(Ql, Qo, rows) > 1e-4
 

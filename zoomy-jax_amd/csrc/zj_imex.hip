@@ -12,10 +12,14 @@
 // reference's "ad" backend differentiates): J v = v - dt (dS/dQ v +
 // dS/daux . aux(v)), where aux(v) -- the LSQ gradients of v -- is linear in
 // v, and the 1/h of the Manning term is differentiated through its
-// desingularised formula.  GMRES is restarted GMRES(m) with classical
-// Gram-Schmidt and one reorthogonalisation, Givens rotations on the host.
-// Every reduction stores one partial per block and a fold kernel sums them
-// in a fixed order, so the stage is deterministic.
+// desingularised formula.  The stage changes the momentum rows only (S has
+// no b or h rows, and Q starts at Qexp), so the Newton unknowns and every
+// Krylov vector live in the dim momentum rows: v_h = 0, hence the aux(v)
+// rows the source reads (dh/dx_d) vanish and aux(Q) is the same for every
+// Newton step -- it is refreshed once per solve.  GMRES is restarted
+// GMRES(m) with classical Gram-Schmidt and one reorthogonalisation, Givens
+// rotations on the host.  Every reduction stores one partial per block and
+// a fold kernel sums them in a fixed order, so the stage is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,27 +57,24 @@ __device__ __forceinline__ double hinv_d(double h, double eps) {
     return 1.4142135623730951 * h / sqrt(h * h * h * h + m * m * m * m);
 }
 
-// d hinv / d h of the desingularised 1/h
-__device__ __forceinline__ double dhinv_d(double h, double eps) {
-    if (h > eps) {
-        const double hi = hinv_d(h, eps);
-        return -hi * hi;
-    }
-    const double h4 = h * h * h * h, e4 = eps * eps * eps * eps;
-    return 1.4142135623730951 * (e4 - h4) / ((h4 + e4) * sqrt(h4 + e4));
-}
-
 // aux row of dh/dx_d: rows per direction d are [dq_d/dx_d, dh/dx_d]
 __device__ __forceinline__ int row_dh(int d) { return 2 * d + 1; }
 
-// S(q, aux) rows q_d at cell c
-__device__ __forceinline__ void source_at(const SrcPrm& p, const double* q, const double* hx,
-                                          double* S) {
-    const double hi = hinv_d(q[1], p.eps);
+// S(q, aux) rows q_d at cell c; kf = g n^2 hinv(h)^(7/3), the Manning
+// coefficient of the cell (h does not change during the stage)
+__device__ __forceinline__ void source_at(const SrcPrm& p, const double* q, double kf,
+                                          const double* hx, double* S) {
     const double qq = sqrt(q[2] * q[2] + q[3] * q[3]);
-    const double m = p.g * p.n2 * pow(hi, 7.0 / 3.0) * qq;
+    const double m = kf * qq;
     S[0] = -p.alpha * p.g * q[1] * hx[0] - m * q[2];
     S[1] = p.dim == 2 ? -p.alpha * p.g * q[1] * hx[1] - m * q[3] : -m * q[3];
+}
+
+// kf[c] = g n^2 hinv(h_c)^(7/3), once per stage
+__global__ __launch_bounds__(IB) void k_friction(SrcPrm p, const double* __restrict__ Q,
+                                                 double* __restrict__ kf) {
+    for (int c = blockIdx.x * IB + threadIdx.x; c < p.N; c += gridDim.x * IB)
+        kf[c] = p.g * p.n2 * pow(hinv_d(Q[(size_t)p.N + c], p.eps), 7.0 / 3.0);
 }
 
 __device__ __forceinline__ void block_sum(double v, double* lds, double* out) {
@@ -88,66 +89,55 @@ __device__ __forceinline__ void block_sum(double v, double* lds, double* out) {
     __syncthreads();
 }
 
-// R = Q - Qexp - dt S(Q, aux); part[block] = sum of R^2 over the block
+// R = Q - Qexp - dt S(Q, aux) in the momentum rows (R[d N + c], d < dim;
+// the b and h rows of Q equal Qexp's); part[block] = sum of R^2 over the
+// block
 __global__ __launch_bounds__(IB) void k_src_residual(SrcPrm p, const double* __restrict__ Q,
                                                      const double* __restrict__ Qe,
                                                      const double* __restrict__ aux,
+                                                     const double* __restrict__ kf,
                                                      double* __restrict__ R,
                                                      double* __restrict__ part) {
     __shared__ double lds[IB / 64];
     double acc = 0.0;
     const int N = p.N;
     for (int c = blockIdx.x * IB + threadIdx.x; c < N; c += gridDim.x * IB) {
-        double q[4], hx[2] = {0.0, 0.0};
-        for (int v = 0; v < 4; ++v) q[v] = Q[(size_t)v * N + c];
+        double q[4] = {0.0, 0.0, 0.0, 0.0}, hx[2] = {0.0, 0.0};
+        for (int v = 1; v < 4; ++v) q[v] = Q[(size_t)v * N + c];
         for (int d = 0; d < p.dim; ++d) hx[d] = aux[(size_t)row_dh(d) * N + c];
         double S[2];
-        source_at(p, q, hx, S);
-        double r[4] = {q[0] - Qe[c], q[1] - Qe[(size_t)N + c],
-                       q[2] - Qe[(size_t)2 * N + c] - p.dt * S[0],
-                       q[3] - Qe[(size_t)3 * N + c] - p.dt * S[1]};
-        for (int v = 0; v < 4; ++v) {
-            R[(size_t)v * N + c] = r[v];
-            acc += r[v] * r[v];
+        source_at(p, q, kf[c], hx, S);
+        for (int d = 0; d < p.dim; ++d) {
+            const double r = q[2 + d] - Qe[(size_t)(2 + d) * N + c] - p.dt * S[d];
+            R[(size_t)d * N + c] = r;
+            acc += r * r;
         }
     }
     block_sum(acc, lds, part + blockIdx.x);
 }
 
-// out = J v = v - dt (dS/dQ v + dS/daux aux(v)); auxv: the LSQ gradients of v
+// out = J v = v - dt dS/dq v for momentum vectors v (v[d N + c], d < dim):
+// with v_h = 0 the terms of dh and of aux(v) (whose dh/dx_d rows are the
+// gradients of v_h) vanish
 __global__ __launch_bounds__(IB) void k_src_jvp(SrcPrm p, const double* __restrict__ Q,
-                                                const double* __restrict__ aux,
+                                                const double* __restrict__ kf,
                                                 const double* __restrict__ V,
-                                                const double* __restrict__ auxv,
                                                 double* __restrict__ out) {
     const int N = p.N;
     for (int c = blockIdx.x * IB + threadIdx.x; c < N; c += gridDim.x * IB) {
-        double q[4], v[4], hx[2] = {0.0, 0.0}, vx[2] = {0.0, 0.0};
-        for (int k = 0; k < 4; ++k) {
-            q[k] = Q[(size_t)k * N + c];
-            v[k] = V[(size_t)k * N + c];
-        }
-        for (int d = 0; d < p.dim; ++d) {
-            hx[d] = aux[(size_t)row_dh(d) * N + c];
-            vx[d] = auxv[(size_t)row_dh(d) * N + c];
-        }
-        const double hi = hinv_d(q[1], p.eps);
-        const double dhi = dhinv_d(q[1], p.eps);
+        double q[4] = {0.0, 0.0, 0.0, 0.0}, v[2] = {0.0, 0.0};
+        for (int k = 2; k < 4; ++k) q[k] = Q[(size_t)k * N + c];
+        for (int d = 0; d < p.dim; ++d) v[d] = V[(size_t)d * N + c];
         const double qq = sqrt(q[2] * q[2] + q[3] * q[3]);
-        const double h73 = pow(hi, 7.0 / 3.0);
-        const double k = p.g * p.n2;
-        // d(|q| hinv^(7/3)) = (q.dq / |q|) hinv^(7/3) + |q| (7/3) hinv^(4/3) dhinv dh
-        const double dqq = qq > 0.0 ? (q[2] * v[2] + q[3] * v[3]) / qq : 0.0;
-        const double dm = k * (dqq * h73 + qq * (7.0 / 3.0) * pow(hi, 4.0 / 3.0) * dhi * v[1]);
-        const double m = k * h73 * qq;
-        double dS[2];
-        dS[0] = -p.alpha * p.g * (v[1] * hx[0] + q[1] * vx[0]) - (dm * q[2] + m * v[2]);
-        dS[1] = (p.dim == 2 ? -p.alpha * p.g * (v[1] * hx[1] + q[1] * vx[1]) : 0.0) -
-                (dm * q[3] + m * v[3]);
-        out[c] = v[0];
-        out[(size_t)N + c] = v[1];
-        out[(size_t)2 * N + c] = v[2] - p.dt * dS[0];
-        out[(size_t)3 * N + c] = v[3] - p.dt * dS[1];
+        const double k = kf[c];
+        // d(|q| k) = k (q.dq / |q|) at fixed h
+        const double dqq = qq > 0.0 ? (q[2] * v[0] + (p.dim == 2 ? q[3] * v[1] : 0.0)) / qq : 0.0;
+        const double dm = k * dqq;
+        const double m = k * qq;
+        for (int d = 0; d < p.dim; ++d) {
+            const double dS = -(dm * q[2 + d] + m * v[d]);
+            out[(size_t)d * N + c] = v[d] - p.dt * dS;
+        }
     }
 }
 
@@ -202,7 +192,7 @@ struct zj_imex_global {
     zj_solver* s = nullptr;
     int N = 0, n = 0, dim = 2, m = 20;
     double alpha = 0.0;
-    double *aux = nullptr, *auxv = nullptr, *R = nullptr, *W = nullptr, *Vb = nullptr,
+    double *aux = nullptr, *kf = nullptr, *R = nullptr, *W = nullptr, *Vb = nullptr,
            *dX = nullptr, *part = nullptr, *dots = nullptr, *host = nullptr;
     int newton_its = 0, gmres_its = 0, fallbacks = 0;
     double rnorm = 0.0;
@@ -228,11 +218,11 @@ int norm2(zj_imex_global* g, const double* w, double* out, hipStream_t st) {
     return ZJ_OK;
 }
 
-// R(Q) into g->R and its norm (aux of Q refreshed into g->aux)
+// R(Q) into g->R and its norm (g->aux holds aux(Q), whose dh/dx_d rows do
+// not change during the stage)
 int residual(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* Qe,
              double* rnorm, hipStream_t st) {
-    if (int rc = zj_update_aux(g->s, Q, g->aux, st)) return rc;
-    k_src_residual<<<IGRID, IB, 0, st>>>(p, Q, Qe, g->aux, g->R, g->part);
+    k_src_residual<<<IGRID, IB, 0, st>>>(p, Q, Qe, g->aux, g->kf, g->R, g->part);
     k_fold<<<1, IB, 0, st>>>(IGRID, g->part, g->dots);
     IHIPCHK(hipMemcpyAsync(g->host, g->dots, sizeof(double), hipMemcpyDeviceToHost, st));
     IHIPCHK(hipStreamSynchronize(st));
@@ -240,11 +230,10 @@ int residual(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* 
     return ZJ_OK;
 }
 
-// J v at the current Q (g->aux holds aux(Q))
+// J v at the current Q
 int jvp(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* v, double* out,
         hipStream_t st) {
-    if (int rc = zj_update_aux(g->s, v, g->auxv, st)) return rc;
-    k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->aux, v, g->auxv, out);
+    k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, v, out);
     IHIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -357,13 +346,13 @@ int zj_imex_global_create(zj_solver* s, double alpha, int32_t restart, zj_imex_g
     if (!g) return ZJ_ENOMEM;
     g->s = s;
     g->N = zj::solver_cells(s);
-    g->n = 4 * g->N;
     g->dim = zj::solver_dim(s);
+    g->n = g->dim * g->N;  // the momentum rows
     g->m = restart;
     g->alpha = alpha;
     const size_t naux = (size_t)(3 * g->dim + 1) * g->N;
     if (hipMalloc((void**)&g->aux, sizeof(double) * naux) != hipSuccess ||
-        hipMalloc((void**)&g->auxv, sizeof(double) * naux) != hipSuccess ||
+        hipMalloc((void**)&g->kf, sizeof(double) * g->N) != hipSuccess ||
         hipMalloc((void**)&g->R, sizeof(double) * g->n) != hipSuccess ||
         hipMalloc((void**)&g->W, sizeof(double) * g->n) != hipSuccess ||
         hipMalloc((void**)&g->dX, sizeof(double) * g->n) != hipSuccess ||
@@ -380,7 +369,7 @@ int zj_imex_global_create(zj_solver* s, double alpha, int32_t restart, zj_imex_g
 
 int zj_imex_global_destroy(zj_imex_global* g) {
     if (!g) return ZJ_OK;
-    for (double* p : {g->aux, g->auxv, g->R, g->W, g->Vb, g->dX, g->part, g->dots})
+    for (double* p : {g->aux, g->kf, g->R, g->W, g->Vb, g->dX, g->part, g->dots})
         if (p) (void)hipFree(p);
     if (g->host) (void)hipHostFree(g->host);
     delete g;
@@ -396,8 +385,11 @@ int zj_imex_global_solve(zj_imex_global* g, double dt, const double* Qexp, doubl
     hipStream_t st = (hipStream_t)stream;
     const zj_params prm = zj::solver_params(g->s);
     SrcPrm p{prm.g, prm.manning_n * prm.manning_n, prm.eps_wet, g->alpha, dt, g->N, g->dim};
-    double* corr = g->dX;  // the Newton correction
-    IHIPCHK(hipMemcpyAsync(Qout, Qexp, sizeof(double) * g->n, hipMemcpyDeviceToDevice, st));
+    double* corr = g->dX;  // the Newton correction (momentum rows)
+    double* Qmom = Qout + (size_t)2 * g->N;
+    IHIPCHK(hipMemcpyAsync(Qout, Qexp, sizeof(double) * 4 * g->N, hipMemcpyDeviceToDevice, st));
+    if (int rc = zj_update_aux(g->s, Qout, g->aux, st)) return rc;
+    k_friction<<<IGRID, IB, 0, st>>>(p, Qout, g->kf);
     double rn = 0.0;
     if (int rc = residual(g, p, Qout, Qexp, &rn, st)) return rc;
     g->newton_its = 0;
@@ -417,7 +409,7 @@ int zj_imex_global_solve(zj_imex_global* g, double dt, const double* Qexp, doubl
         }
         Coefs c{};
         c.a[0] = 1.0;
-        k_maxpy<<<IGRID, IB, 0, st>>>(g->n, 1, c, 1.0, corr, Qout);
+        k_maxpy<<<IGRID, IB, 0, st>>>(g->n, 1, c, 1.0, corr, Qmom);
         if ((rc = residual(g, p, Qout, Qexp, &rn, st))) break;
         g->newton_its += 1;
     }
