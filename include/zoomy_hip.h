@@ -231,6 +231,15 @@ int zj_reconstruct(zj_solver *s, const void *Q, const uint8_t *force_o1,
 int zj_flux_pass(zj_solver *s, const void *Q, const void *hinv_aux,
                  const void *QL, const void *QR, void *Fface, void *Dp,
                  void *Dm, void *stream);
+/* The runtime slots numerical_flux / numerical_fluctuations
+ * (transformation/jax_runtime.py:789-806): the solver's Riemann solver on n
+ * given face states qL, qR (4, n) with the aux of the faces' cells qauxL,
+ * qauxR (3 dim + 1, n; the hinv row is read; NULL: the hinv of the face
+ * state) and FP64 unit normals (dim, n).  Writes any of F (4, n) and the
+ * fluctuations Dp, Dm (4, n) (both or neither). */
+int zj_face_numerics(zj_solver *s, int32_t n, const void *qL, const void *qR,
+                     const void *qauxL, const void *qauxR, const double *normals,
+                     void *Fface, void *Dp, void *Dm, void *stream);
 /* gather_update (blocks.py:241-301): Qcand = Q0 + a dt (-div(F + D) +
  * cell term + source) gathered per cell over its faces; the cell term is
  * cell_term (4, N) or, when that is NULL and lim_grad (4, dim, N) is given,

@@ -59,6 +59,8 @@ def test_argument_checks_before_any_device_work():
                               null) == _lib.ZJ_EINVAL
     assert L.zj_flux_operator_o1(null, null, null, null,
                                  null) == _lib.ZJ_EINVAL
+    assert L.zj_face_numerics(null, 4, null, null, null, null, null, null,
+                              null, null, null) == _lib.ZJ_EINVAL
 
 
 def test_dirichlet_descriptor_validation():

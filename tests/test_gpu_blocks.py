@@ -337,3 +337,57 @@ def test_flux_operator_force_o1(recon):
         assert np.linalg.norm(got[r] - want[r]) <= 1e-12 * qs
     if recon == "eta":
         assert rel_l2(want[1], o.flux(Q)[1]) > 1e-6   # the mask acted
+
+
+@pytest.mark.parametrize("riemann", ["hll", "rusanov", "rusanov_nc"])
+@pytest.mark.parametrize("dim", [1, 2])
+def test_runtime_numerical_flux_slots(riemann, dim):
+    """The runtime slots numerical_flux / numerical_fluctuations on random
+    face states, cell aux and unit normals against the oracle's face
+    numerics (with the aux 1/h, and with the face states' 1/h when no aux is
+    given), FP64 to round-off."""
+    from oracle.oracle import face_flux, lib
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.mesh import Mesh
+    mesh = (Mesh.quad(8, 8) if dim == 2 else Mesh.line(16)).set_bc("wall")
+    s = HyperbolicSolver(time_end=np.inf)
+    s.setup_simulation(mesh, ShallowWater(dimension=dim, order=2,
+                                          limiter="minmod", riemann=riemann))
+    rng = np.random.default_rng(17)
+    n = 300
+    nv = 2 + dim
+    qL = np.zeros((nv, n))
+    qR = np.zeros((nv, n))
+    for q in (qL, qR):
+        q[0] = rng.uniform(0.0, 0.3, n)
+        q[1] = rng.uniform(0.0, 2.0, n) * (rng.uniform(size=n) > 0.1)
+        q[2:] = rng.normal(0.0, 0.5, (dim, n)) * q[1]
+    ang = rng.uniform(0, 2 * np.pi, n)
+    nrm = (np.array([np.cos(ang), np.sin(ang)]) if dim == 2
+           else np.sign(rng.uniform(-1, 1, (1, n))))
+    naux = 3 * dim + 1
+    xL = rng.normal(0.0, 1.0, (naux, n))
+    xR = rng.normal(0.0, 1.0, (naux, n))
+    xL[-1] = [lib().zjo_hinv(h, 1e-8) for h in rng.uniform(0.1, 2.0, n)]
+    xR[-1] = [lib().zjo_hinv(h, 1e-8) for h in rng.uniform(0.1, 2.0, n)]
+    rt = s.runtime
+    for aux in (True, False):
+        F = rt.numerical_flux(qL, qR, xL if aux else None,
+                              xR if aux else None, None, nrm)
+        D = rt.numerical_fluctuations(qL, qR, xL if aux else None,
+                                      xR if aux else None, None, nrm)
+        F, D = F.cpu().numpy(), D.cpu().numpy()
+        assert F.shape == (nv, n) and D.shape == (2, nv, n)
+        for f in range(n):
+            a = np.zeros(4)
+            b = np.zeros(4)
+            a[:nv], b[:nv] = qL[:, f], qR[:, f]
+            hA = xL[-1, f] if aux else lib().zjo_hinv(a[1], 1e-8)
+            hB = xR[-1, f] if aux else lib().zjo_hinv(b[1], 1e-8)
+            nn = np.array([nrm[0, f], nrm[1, f] if dim == 2 else 0.0])
+            Fo, Dpo, Dmo = face_flux(a, b, hA, hB, nn, riemann=riemann)
+            sc = max(np.abs(Fo).max(), np.abs(Dpo).max(), np.abs(Dmo).max(),
+                     1.0)
+            assert np.abs(F[:, f] - Fo[:nv]).max() <= 1e-12 * sc
+            assert np.abs(D[0, :, f] - Dpo[:nv]).max() <= 1e-12 * sc
+            assert np.abs(D[1, :, f] - Dmo[:nv]).max() <= 1e-12 * sc

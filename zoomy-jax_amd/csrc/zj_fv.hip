@@ -2267,6 +2267,38 @@ __global__ __launch_bounds__(BLOCK) void k_flux_pass(DevMesh m, Opts o, const T*
     }
 }
 
+// The runtime's numerical_flux / numerical_fluctuations slots: the
+// solver's Riemann solver on n given face states.  The hinv of each side is
+// the last aux row of the face's cell (qauxL / qauxR, (3 dim + 1, n)), or
+// the hinv of the face state when no aux is given; normals (dim, n).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_face_numerics(
+    Opts o, int dim, int n, const T* __restrict__ qL, const T* __restrict__ qR,
+    const T* __restrict__ qauxL, const T* __restrict__ qauxR, const double* __restrict__ nrm,
+    T* __restrict__ Fo, T* __restrict__ Dpo, T* __restrict__ Dmo) {
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= n) return;
+    T a[4], b[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        a[v] = qL[(size_t)v * n + f];
+        b[v] = qR[(size_t)v * n + f];
+    }
+    const T eps = T(o.eps);
+    const size_t hrow = (size_t)(3 * dim) * n;
+    const T hA = qauxL ? qauxL[hrow + f] : hinv_of(a[1], eps);
+    const T hB = qauxR ? qauxR[hrow + f] : hinv_of(b[1], eps);
+    const T nx = T(nrm[f]), ny = dim == 2 ? T(nrm[(size_t)n + f]) : T(0);
+    T Fx[4], Dp[4], Dm[4];
+    face_split(o.rs, a, b, hA, hB, nx, ny, T(o.g), eps, Fx, Dp, Dm);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if (Fo) Fo[(size_t)v * n + f] = Fx[v];
+        if (Dpo) Dpo[(size_t)v * n + f] = Dp[v];
+        if (Dmo) Dmo[(size_t)v * n + f] = Dm[v];
+    }
+}
+
 // gather_update: Q_cand = Q0 + a dt (-div(F + D) + cell_term + source) per
 // cell over its faces (blocks.py:241-287); the cell-interior term is the
 // given array or, from a limited gradient, -B(Q0) . grad with B[q_d, b] =
@@ -2496,6 +2528,27 @@ int zj_flux_pass(zj_solver* s, const void* Q, const void* hinv_aux, const void* 
                            s->o, (const float*)Q, (const float*)hinv_aux, (const float*)QL,
                            (const float*)QR, M->fc, M->fs, M->nfaces, (const double2*)M->slot_n,
                            (float*)Fface, (float*)Dp, (float*)Dm);
+    HIPCHK(hipGetLastError());
+    return ZJ_OK;
+}
+
+int zj_face_numerics(zj_solver* s, int32_t n, const void* qL, const void* qR,
+                     const void* qauxL, const void* qauxR, const double* normals, void* Fface,
+                     void* Dp, void* Dm, void* stream) {
+    if (!s || n < 0 || !qL || !qR || !normals || (!Fface && !Dp && !Dm) || (!Dp != !Dm))
+        return ZJ_EINVAL;
+    if (n == 0) return ZJ_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int dim = s->m->d.dim;
+    if (s->p.dtype == ZJ_F64)
+        hipLaunchKernelGGL(k_face_numerics<double>, dim3(fblocks(n)), dim3(BLOCK), 0, st, s->o,
+                           dim, n, (const double*)qL, (const double*)qR, (const double*)qauxL,
+                           (const double*)qauxR, normals, (double*)Fface, (double*)Dp,
+                           (double*)Dm);
+    else
+        hipLaunchKernelGGL(k_face_numerics<float>, dim3(fblocks(n)), dim3(BLOCK), 0, st, s->o,
+                           dim, n, (const float*)qL, (const float*)qR, (const float*)qauxL,
+                           (const float*)qauxR, normals, (float*)Fface, (float*)Dp, (float*)Dm);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }

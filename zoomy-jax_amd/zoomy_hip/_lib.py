@@ -84,6 +84,8 @@ _SIGS = {
     "zj_rcm_from": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_int32,
                                    ctypes.c_void_p]),
+    "zj_face_numerics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]
+                         + [ctypes.c_void_p] * 9),
     "zj_patch_plan": (ctypes.c_int, [ctypes.POINTER(MeshDesc),
                                      ctypes.c_void_p]),
     "zj_patch_check": (ctypes.c_int, [ctypes.POINTER(MeshDesc),

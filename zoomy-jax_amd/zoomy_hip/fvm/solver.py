@@ -177,6 +177,58 @@ class _Runtime:
                                      ctypes.byref(h)), "zj_solver_create")
         self.handle = h
 
+    def _face_args(self, qL, qR, qauxL, qauxR, normal):
+        dev = _device()
+        n = qL.shape[1]
+        dim = self.dmesh.mesh.dim
+
+        def rows4(q):
+            q = torch.as_tensor(q, dtype=self.dtype, device=dev)
+            if q.shape[0] == 4:
+                return q.contiguous()
+            out = torch.zeros((4, n), dtype=self.dtype, device=dev)
+            out[:q.shape[0]] = q
+            return out
+
+        def aux(a):
+            if a is None:
+                return None
+            a = torch.as_tensor(a, dtype=self.dtype, device=dev).contiguous()
+            if a.shape != (3 * dim + 1, n):
+                raise ValueError(f"qaux must be ({3 * dim + 1}, {n}), got "
+                                 f"{tuple(a.shape)}")
+            return a
+        a, b = rows4(qL), rows4(qR)
+        if a.shape != (4, n) or b.shape != (4, n):
+            raise ValueError("qL and qR must be (n_var, n) face states")
+        nrm = torch.as_tensor(normal, dtype=torch.float64,
+                              device=dev)[:dim].contiguous()
+        if nrm.shape != (dim, n):
+            raise ValueError(f"normal must be ({dim}, {n})")
+        return n, a, b, aux(qauxL), aux(qauxR), nrm
+
+    def numerical_flux(self, qL, qR, qauxL, qauxR, parameters, normal):
+        """The runtime slot ``numerical_flux(qL, qR, qauxL, qauxR, p, n)``
+        (``jax_runtime.py:789-795``): the conservative face flux (n_var, n)
+        of the solver's Riemann solver on given face states."""
+        n, a, b, xa, xb, nrm = self._face_args(qL, qR, qauxL, qauxR, normal)
+        F = torch.empty_like(a)
+        L.check(L.lib().zj_face_numerics(
+            self.handle, n, _p(a), _p(b), _p(xa), _p(xb), _p(nrm), _p(F),
+            None, None, _stream()), "zj_face_numerics")
+        return F[:self.model.n_state]
+
+    def numerical_fluctuations(self, qL, qR, qauxL, qauxR, parameters,
+                               normal):
+        """The runtime slot ``numerical_fluctuations`` (``jax_runtime.py:
+        797-803``): the fluctuations (Dp, Dm) stacked as (2, n_var, n)."""
+        n, a, b, xa, xb, nrm = self._face_args(qL, qR, qauxL, qauxR, normal)
+        D = torch.empty((2, 4, n), dtype=self.dtype, device=a.device)
+        L.check(L.lib().zj_face_numerics(
+            self.handle, n, _p(a), _p(b), _p(xa), _p(xb), _p(nrm), None,
+            _p(D[0]), _p(D[1]), _stream()), "zj_face_numerics")
+        return D[:, :self.model.n_state]
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and L is not None and L._lib is not None:
@@ -343,6 +395,13 @@ class HyperbolicSolver:
         reference (``solver_jax.py:580-582``, ``reconstruction_jax.py:
         1058-1062``)."""
         return self.reconstruction_variables in ("eta", "eta_hu")
+
+    @property
+    def runtime(self):
+        """The solver's runtime: its ``numerical_flux`` /
+        ``numerical_fluctuations`` slots evaluate the Riemann solver on
+        given face states (``transformation/jax_runtime.py:789-806``)."""
+        return self._rt
 
     def get_flux_operator(self, mesh=None, runtime=None):
         lib = L.lib()
