@@ -84,7 +84,9 @@ typedef struct {
 } zj_params;
 
 /* Replaces convert_mesh_to_jax (zoomy_jax/mesh/mesh.py:186-264) and
- * prepare_mesh (zoomy_jax/fvm/solver2/context.py:99-163). */
+ * prepare_mesh (zoomy_jax/fvm/solver2/context.py:99-163).  Meshes of more
+ * than 268,435,455 cells (8 rows of 32-bit offsets) are refused with
+ * ZJ_EUNSUP. */
 int zj_mesh_create(const zj_mesh_desc *desc, zj_mesh **out);
 int zj_mesh_destroy(zj_mesh *m);
 

@@ -63,6 +63,17 @@ def test_argument_checks_before_any_device_work():
                               null, null, null) == _lib.ZJ_EINVAL
 
 
+def test_mesh_size_limit():
+    """Meshes beyond 32-bit row offsets (8 rows of N cells) are refused
+    before any array is read."""
+    import ctypes
+    d = _lib.MeshDesc()
+    d.dim, d.n_cells, d.n_verts, d.max_faces = 2, 300_000_000, 4, 4
+    h = ctypes.c_void_p()
+    assert _lib.lib().zj_mesh_create(ctypes.byref(d),
+                                     ctypes.byref(h)) == _lib.ZJ_EUNSUP
+
+
 def test_dirichlet_descriptor_validation():
     """A Dirichlet face needs its tables, and only state rows 0-3 may be
     prescribed (host-side checks of zj_mesh_create / zj_patch_check)."""

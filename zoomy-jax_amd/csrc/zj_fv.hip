@@ -1437,6 +1437,11 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
         d->max_faces > MAXF || d->max_extra < 0 || d->n_bf < 0)
         return ZJ_EINVAL;
     if (d->dim == 1 && d->max_faces != 2) return ZJ_EINVAL;
+    // the kernels index rows of (rows, N) arrays with 32-bit offsets (up to
+    // 8 rows: the state, the aux rows, the slot tables)
+    if ((int64_t)d->n_cells * 8 > (int64_t)INT32_MAX || (int64_t)d->n_cells * d->max_extra >
+                                                             (int64_t)INT32_MAX)
+        return ZJ_EUNSUP;
     const int N = d->n_cells, mf = d->max_faces, mx = d->max_extra;
     // host-side index validation: every gather in the kernels stays in bounds
     for (size_t i = 0; i < (size_t)mf * N; ++i) {
