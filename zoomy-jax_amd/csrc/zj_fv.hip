@@ -681,9 +681,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     }
     using T2 = typename Vec2<T>::type;
     // extra-stencil ids fetched per batch on triangle meshes (a vertex
-    // stencil of up to 12 cells); the batch of the run-time numerics is
-    // smaller, for its register budget
-    constexpr int MXB = SPEC == 3 ? 12 : (SPEC == 2 || SPEC == 4) ? 6 : SPEC == 1 ? 4 : 0;
+    // stencil of up to 12 cells); the batch of the run-time numerics
+    // (SPEC = 2) is smaller, for its register budget
+    constexpr int MXB = (SPEC == 3 || SPEC == 4) ? 12 : SPEC == 2 ? 6 : SPEC == 1 ? 4 : 0;
     // dynamic LDS, 16-byte aligned (pairs of doubles)
     extern __shared__ double2 smem_v2[];
     double* smem_d = reinterpret_cast<double*>(smem_v2);
