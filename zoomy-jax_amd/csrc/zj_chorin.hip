@@ -227,7 +227,9 @@ constexpr int TC = 8;                // coarse cells per tile side (8: 4x the
 constexpr int WD = 2 * TC + 6;       // down-kernel fine window side (halo 3)
 constexpr int WU = 2 * TC + 4;       // up-kernel fine window side (halo 2)
 constexpr int WC = TC + 4;           // up-kernel coarse window side
-constexpr int TAIL_CELLS = 1400;     // cells of all tail levels together
+constexpr int TAIL_CELLS = 1365;     // cells of all tail levels together
+// the tail's LDS: five arrays per level and the residual of one level
+static_assert(6 * TAIL_CELLS * sizeof(double) <= 64 * 1024, "tail LDS");
 constexpr int TAIL_MAXL = 10;
 constexpr int TB = 512;              // tail kernel threads
 
@@ -480,8 +482,10 @@ __global__ __launch_bounds__(TB) void k_mg_tail(TailDesc<T> d, int coarse_sweeps
         }
     }
     __syncthreads();
-    // arrays of level l: diag, cE, cN, x, f at base + soff[l] + k n
+    // arrays of level l: diag, cE, cN, x, f at base + soff[l] + k n; the
+    // residual of the level being restricted after the last level
     auto arr = [&](int l, int k) { return base + soff[l] + k * snx[l] * sny[l]; };
+    T* SR = base + soff[nl - 1] + 5 * snx[nl - 1] * sny[nl - 1];
     auto od = [&](int nx, int ny, const T* SE, const T* SN, int c, int i, int j, const T* xv) {
         T s = T(0);
         if (i + 1 < nx) s += SE[c] * xv[c + ny];
@@ -505,6 +509,12 @@ __global__ __launch_bounds__(TB) void k_mg_tail(TailDesc<T> d, int coarse_sweeps
             if ((i + j) & 1) SX[c] = (SF[c] - od(nx, ny, SE, SN, c, i, j, SX)) / SD[c];
         }
         __syncthreads();
+        // the residual once per cell, then its restriction
+        for (int c = tid; c < n; c += TB) {
+            const int i = c / ny, j = c - i * ny;
+            SR[c] = SF[c] - (SD[c] * SX[c] + od(nx, ny, SE, SN, c, i, j, SX));
+        }
+        __syncthreads();
         const int nxc = snx[l + 1], nyc = sny[l + 1];
         T* SFc = arr(l + 1, 4);
         for (int cc = tid; cc < nxc * nyc; cc += TB) {
@@ -518,9 +528,7 @@ __global__ __launch_bounds__(TB) void k_mg_tail(TailDesc<T> d, int coarse_sweeps
                 for (int dj = -1; dj <= 2; ++dj) {
                     double wy = rw(J, dj, nyc);
                     if (wy == 0.0) continue;
-                    const int j = 2 * J + dj, c = i * ny + j;
-                    const T r = SF[c] - (SD[c] * SX[c] + od(nx, ny, SE, SN, c, i, j, SX));
-                    sr += wy * (double)r;
+                    sr += wy * (double)SR[i * ny + 2 * J + dj];
                 }
                 s += wx * sr;
             }
@@ -592,7 +600,7 @@ struct Red {
     int op;             // FIN_* step
 };
 
-__device__ void scalar_step(const Red& R, double v, double v2) {
+__device__ void scalar_step(const Red& R, double v) {
     int* fl = R.fl;
     double* sc = R.sc;
     if (R.op != FIN_B2 && fl[0]) return;
@@ -600,7 +608,6 @@ __device__ void scalar_step(const Red& R, double v, double v2) {
         case FIN_B2:
             sc[CG_B2] = v;
             sc[CG_RR] = v;
-            sc[CG_BETA] = 0.0;
             fl[1] = 0;
             fl[0] = !(v > 0.0);
             break;
@@ -608,26 +615,10 @@ __device__ void scalar_step(const Red& R, double v, double v2) {
             sc[CG_RR] = v;
             if (v <= R.tol2 * sc[CG_B2]) fl[0] = 1;
             break;
-        case FIN_RZ0:
-            sc[CG_RZ] = v;
-            sc[CG_BETA] = 0.0;
-            break;
-        case FIN_PQ:
-            sc[CG_ALPHA] = sc[CG_RZ] / v;
-            if (!(v > 0.0)) fl[0] = 2;  // breakdown: not positive definite
-            break;
         case FIN_RR:
             sc[CG_RR] = v;
             fl[1] += 1;
             if (v <= R.tol2 * sc[CG_B2] || fl[1] >= R.maxit) fl[0] = 1;
-            break;
-        case FIN_RZ:
-            sc[CG_BETA] = v / sc[CG_RZ];
-            sc[CG_RZ] = v;
-            break;
-        case FIN_RZ_FLEX:  // Polak-Ribiere: beta = z.(r - r_old) / rz_old
-            sc[CG_BETA] = (v - v2) / sc[CG_RZ];
-            sc[CG_RZ] = v;
             break;
     }
 }
@@ -646,20 +637,15 @@ __device__ void reduce_finish(const Red& R, double v, double v2, bool two) {
 }
 
 // Sums the np partials of the preceding reduction in a fixed order and
-// performs the scalar step of CG.
-__global__ __launch_bounds__(RED_MAX) void k_fin(Red R, int np, int two) {
+// performs the scalar step R.op.
+__global__ __launch_bounds__(RED_MAX) void k_fin(Red R, int np) {
     if (R.op != FIN_B2 && R.fl[0]) return;
     __shared__ double lds[RED_MAX / 64];
     auto add = [](double a, double b) { return a + b; };
-    double v = 0.0, v2 = 0.0;
+    double v = 0.0;
     for (int i = threadIdx.x; i < np; i += RED_MAX) v += R.part[i];
     v = block_reduce<double, decltype(add), RED_MAX>(v, add, lds);
-    if (two) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < np; i += RED_MAX) v2 += R.part[np + i];
-        v2 = block_reduce<double, decltype(add), RED_MAX>(v2, add, lds);
-    }
-    if (threadIdx.x == 0) scalar_step(R, v, v2);
+    if (threadIdx.x == 0) scalar_step(R, v);
 }
 
 template <typename T>
@@ -689,39 +675,6 @@ __global__ __launch_bounds__(CB) void k_dot(int n, const T* __restrict__ a, cons
     reduce_finish(R, acc, 0.0, false);
 }
 
-// x += alpha d, r -= alpha q (r_old keeps r before the update when given);
-// sum of r.r
-template <typename T>
-__global__ __launch_bounds__(CB) void k_axpy2(int n, T* __restrict__ x, T* __restrict__ r,
-                                              T* __restrict__ r_old, const T* __restrict__ d,
-                                              const T* __restrict__ q, Red R,
-                                              const int* __restrict__ done) {
-    if (is_done(done)) return;
-    const T alpha = (T)R.sc[CG_ALPHA];
-    double acc = 0.0;
-    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
-        x[c] += alpha * d[c];
-        T rv = r[c];
-        if (r_old) r_old[c] = rv;
-        rv -= alpha * q[c];
-        r[c] = rv;
-        acc += (double)rv * (double)rv;
-    }
-    reduce_finish(R, acc, 0.0, false);
-}
-
-// d = z + beta d; first != 0: d = z (d holds no value yet: 0 * d could
-// be NaN)
-template <typename T>
-__global__ __launch_bounds__(CB) void k_xpby(int n, const T* __restrict__ z, T* __restrict__ d,
-                                             const double* __restrict__ sc, int first,
-                                             const int* __restrict__ done) {
-    if (is_done(done)) return;
-    const T beta = (T)sc[CG_BETA];
-    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB)
-        d[c] = first ? z[c] : z[c] + beta * d[c];
-}
-
 // r = rhs - S x (zero_x: zero initial guess, x is zeroed); sum of rhs.rhs
 template <typename T>
 __global__ __launch_bounds__(CB) void k_cg_init(int nx, int ny, const T* __restrict__ diag,
@@ -745,19 +698,111 @@ __global__ __launch_bounds__(CB) void k_cg_init(int nx, int ny, const T* __restr
     reduce_finish(R, acc, 0.0, false);
 }
 
-// Two sums (a.b, a.c) in one pass (flexible CG: z.r and z.r_old).
+// Sum of the np partials of a preceding reduction, folded in a fixed order
+// that every block of a launch repeats: all blocks, and all their threads,
+// hold the same value.
+__device__ double fold_all(const double* __restrict__ part, int np) {
+    __shared__ double lds[CB / 64 + 1];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += CB) v += part[i];
+    auto add = [](double a, double b) { return a + b; };
+    v = block_reduce<double, decltype(add), CB>(v, add, lds);
+    if (threadIdx.x == 0) lds[CB / 64] = v;
+    __syncthreads();
+    v = lds[CB / 64];
+    __syncthreads();
+    return v;
+}
+
+// CG iteration k, first half: rz_k = r.z folded from the partials prz
+// (flexible CG: the Polak-Ribiere numerator z.(r - r_old), the second half
+// of prz), beta = rz_k / rz_{k-1}; the direction d = z + beta d_prev (k = 0:
+// d = z) of the cell and, redundantly, of its four neighbours; q = S d and
+// the p.q partial.  rz_k goes to slot CG_RZ (k even) or CG_RZ2 (k odd), and
+// rz_{k-1} is read from the other one, so no block reads a slot that another
+// block of the launch writes.
 template <typename T>
-__global__ __launch_bounds__(CB) void k_dot2(int n, const T* __restrict__ a,
-                                             const T* __restrict__ b, const T* __restrict__ c2,
-                                             Red R, const int* __restrict__ done) {
+__global__ __launch_bounds__(CB) void k_spmv_dir(int nx, int ny, const T* __restrict__ diag,
+                                                 const T* __restrict__ cE,
+                                                 const T* __restrict__ cN,
+                                                 const T* __restrict__ z,
+                                                 const T* __restrict__ dprev, T* __restrict__ d,
+                                                 T* __restrict__ q,
+                                                 const double* __restrict__ prz, int np,
+                                                 int flex, int k, Red R,
+                                                 const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const double rz = fold_all(prz, np);
+    double num = rz;
+    if (flex && k > 0) num -= fold_all(prz + np, np);
+    const T beta = k == 0 ? T(0) : (T)(num / R.sc[(k & 1) ? CG_RZ : CG_RZ2]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) R.sc[(k & 1) ? CG_RZ2 : CG_RZ] = rz;
+    const int n = nx * ny;
+    auto dir = [&](int e) { return k == 0 ? z[e] : z[e] + beta * dprev[e]; };
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        const int i = c / ny, j = c - i * ny;
+        const T dc = dir(c);
+        T s = T(0);
+        if (i + 1 < nx) s += cE[c] * dir(c + ny);
+        if (i > 0) s += cE[c - ny] * dir(c - ny);
+        if (j + 1 < ny) s += cN[c] * dir(c + 1);
+        if (j > 0) s += cN[c - 1] * dir(c - 1);
+        const T v = diag[c] * dc + s;
+        d[c] = dc;
+        q[c] = v;
+        acc += (double)dc * (double)v;
+    }
+    reduce_finish(R, acc, 0.0, false);
+}
+
+// CG iteration k, second half: alpha = rz_k / p.q (p.q folded from the
+// partials ppq; p.q <= 0 is a breakdown: the flag becomes 2 and nothing is
+// updated); x += alpha d, r -= alpha q (r_old keeps r before the update when
+// given, r32 receives r rounded to FP32 when given) and the r.r partial.
+template <typename T>
+__global__ __launch_bounds__(CB) void k_axpy2_fold(int n, T* __restrict__ x, T* __restrict__ r,
+                                                   T* __restrict__ r_old, float* __restrict__ r32,
+                                                   const T* __restrict__ d,
+                                                   const T* __restrict__ q,
+                                                   const double* __restrict__ ppq, int np, int k,
+                                                   Red R, const int* __restrict__ done) {
+    if (is_done(done)) return;
+    const double pq = fold_all(ppq, np);
+    if (!(pq > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) R.fl[0] = 2;  // not positive definite
+        return;
+    }
+    const T alpha = (T)(R.sc[(k & 1) ? CG_RZ2 : CG_RZ] / pq);
+    double acc = 0.0;
+    for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
+        x[c] += alpha * d[c];
+        T rv = r[c];
+        if (r_old) r_old[c] = rv;
+        rv -= alpha * q[c];
+        r[c] = rv;
+        if (r32) r32[c] = (float)rv;
+        acc += (double)rv * (double)rv;
+    }
+    reduce_finish(R, acc, 0.0, false);
+}
+
+// z = the FP32 preconditioner output widened to FP64, with the partials of
+// z.r and (when r_old is given) z.r_old.
+__global__ __launch_bounds__(CB) void k_widen_dot(int n, const float* __restrict__ z32,
+                                                  double* __restrict__ z,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ r_old, Red R,
+                                                  const int* __restrict__ done) {
     if (is_done(done)) return;
     double s1 = 0.0, s2 = 0.0;
     for (int c = blockIdx.x * CB + threadIdx.x; c < n; c += gridDim.x * CB) {
-        double av = (double)a[c];
-        s1 += av * (double)b[c];
-        s2 += av * (double)c2[c];
+        const double zv = (double)z32[c];
+        z[c] = zv;
+        s1 += zv * r[c];
+        if (r_old) s2 += zv * r_old[c];
     }
-    reduce_finish(R, s1, s2, true);
+    reduce_finish(R, s1, s2, r_old != nullptr);
 }
 
 // ───────────────────────── Chorin stage kernels ────────────────────────────
@@ -1042,7 +1087,9 @@ static bool vcycle_fused(Hier& H, const T* r0, T* z0, int coarse_sweeps, const i
     }
     d.f = F(lt);
     d.x = X(lt);
-    k_mg_tail<T><<<1, TB, (size_t)5 * cells_of(H, lt) * sizeof(T), st>>>(d, coarse_sweeps, done);
+    const size_t tail_lds =
+        (size_t)(5 * cells_of(H, lt) + H.levels[lt].nx * H.levels[lt].ny) * sizeof(T);
+    k_mg_tail<T><<<1, TB, tail_lds, st>>>(d, coarse_sweeps, done);
     for (int l = lt - 1; l >= l0; --l) {
         HLevel& L = H.levels[l];
         dim3 g((L.nx / 2 + TC - 1) / TC, (L.ny / 2 + TC - 1) / TC);
@@ -1141,12 +1188,12 @@ struct zj_chorin {
     // mixed-precision predictor: FP32 FV solver of the same cells
     zj_solver* fv32 = nullptr;
     float *Qm32 = nullptr, *dQ32 = nullptr;
-    double *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
+    double *r = nullptr, *z = nullptr, *d = nullptr, *d2 = nullptr, *q = nullptr;
     // the FP32 input and output of the low-precision preconditioner and the
     // previous residual of flexible CG
     float *r32 = nullptr, *z32 = nullptr;
     double* r_old = nullptr;
-    double* part = nullptr;  // reduction partials (2 * RED_MAX)
+    double* part = nullptr;  // reduction partials (4 * RED_MAX)
     double* sc = nullptr;    // CG scalars
     int* fl = nullptr;       // {done, iterations}
     int* host_fl = nullptr;  // pinned copy of fl
@@ -1187,7 +1234,10 @@ static int read_flags(zj_chorin* c, hipStream_t st) {
 // The Krylov vectors, the operator products and the residual stay FP64; the
 // low-precision preconditioners receive the residual rounded to FP32 and
 // return an FP32 correction.  x holds the initial guess when x0 != 0.
-// Host-synchronous once per chunk of iterations.
+// One iteration is k_spmv_dir (beta, direction, S d), k_axpy2_fold (alpha,
+// x and r), k_fin (r.r and the convergence test), the preconditioner and its
+// r.z partials; the scalars of beta and alpha are folded by the kernels that
+// use them.  Host-synchronous once per chunk of iterations.
 static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, int maxit,
                int pmode, hipStream_t st, int* iters, double* relres, int slot = -1) {
     zj::Hier& H = c->H64;
@@ -1195,16 +1245,23 @@ static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, i
     const int n = L.nx * L.ny;
     const int nr = nred(n);
     const double *D = (double*)L.diag, *E = (double*)L.cE, *Nn = (double*)L.cN;
-    double *r = c->r, *z = c->z, *d = c->d, *q = c->q;
+    double *r = c->r, *z = c->z, *q = c->q;
+    double* dbuf[2] = {c->d, c->d2};
     const int flex = pmode != ZJ_PRECOND_MG;
     double* r_old = flex ? c->r_old : nullptr;
     const double tol2 = tol * tol;
-    auto precond = [&](const double* rin, double* zout) -> int {
+    // partial sums: r.z (and z.r_old) | p.q | r.r
+    double* prz = c->part;
+    double* ppq = c->part + 2 * RED_MAX;
+    double* prr = c->part + 3 * RED_MAX;
+    auto red = [&](double* part, int op) { return Red{part, c->sc, c->fl, tol2, maxit, op}; };
+    // z = M r and the r.z partials (flexible: with z.r_old when given)
+    auto precond = [&](const double* ro) -> int {
         if (!flex) {
-            zj::vcycle<double>(H, rin, zout, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
+            zj::vcycle<double>(H, r, z, c->prm.nu, c->prm.coarse_sweeps, c->fl, st);
+            k_dot<double><<<nr, CB, 0, st>>>(n, z, r, red(prz, FIN_NONE), c->fl);
             return ZJ_OK;
         }
-        k_cast<float><<<nblk(n), CB, 0, st>>>(n, rin, c->r32);
         if (pmode == ZJ_PRECOND_LEARNED) {
             if (int rc = zj::learned_vcycle(c->H32, c->net, c->r32, c->z32, c->prm.nu,
                                             c->prm.coarse_sweeps, c->fl, 0, st))
@@ -1213,40 +1270,32 @@ static int pcg(zj_chorin* c, const double* rhs, double* x, int x0, double tol, i
             zj::vcycle<float>(c->H32, c->r32, c->z32, c->prm.nu, c->prm.coarse_sweeps, c->fl,
                               st);
         }
-        k_widen<<<nblk(n), CB, 0, st>>>(n, c->z32, zout);
+        k_widen_dot<<<nr, CB, 0, st>>>(n, c->z32, z, r, ro, red(prz, FIN_NONE), c->fl);
         return ZJ_OK;
     };
-    auto red = [&](int op) { return Red{c->part, c->sc, c->fl, tol2, maxit, op}; };
-    auto fin = [&](int op, int two = 0) { k_fin<<<1, RED_MAX, 0, st>>>(red(op), nr, two); };
-    k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r, red(FIN_B2));
-    fin(FIN_B2);
+    k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, x, !x0, rhs, r,
+                                         red(prz, FIN_B2));
+    k_fin<<<1, RED_MAX, 0, st>>>(red(prz, FIN_B2), nr);
     if (x0) {
-        k_dot<double><<<nr, CB, 0, st>>>(n, r, r, red(FIN_RR0), c->fl);
-        fin(FIN_RR0);
+        k_dot<double><<<nr, CB, 0, st>>>(n, r, r, red(prr, FIN_RR0), c->fl);
+        k_fin<<<1, RED_MAX, 0, st>>>(red(prr, FIN_RR0), nr);
     }
-    if (int rc = precond(r, z)) return rc;
-    k_dot<double><<<nr, CB, 0, st>>>(n, r, z, red(FIN_RZ0), c->fl);
-    fin(FIN_RZ0);
-    k_xpby<double><<<nr, CB, 0, st>>>(n, z, d, c->sc, 1, c->fl);
+    if (flex) k_cast<float><<<nblk(n), CB, 0, st>>>(n, r, c->r32);
+    if (int rc = precond(nullptr)) return rc;
     HIPCHK(hipGetLastError());
     int launched = 0;
     int& last = c->last_iters[slot < 0 ? pmode : slot];
     int chunk = std::max(1, std::min(maxit, last));
     while (true) {
-        for (int k = 0; k < chunk; ++k) {
-            k_spmv<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, d, q, red(FIN_PQ), c->fl);
-            fin(FIN_PQ);
-            k_axpy2<double><<<nr, CB, 0, st>>>(n, x, r, r_old, d, q, red(FIN_RR), c->fl);
-            fin(FIN_RR);
-            if (int rc = precond(r, z)) return rc;
-            if (flex) {
-                k_dot2<double><<<nr, CB, 0, st>>>(n, z, r, r_old, red(FIN_RZ_FLEX), c->fl);
-                fin(FIN_RZ_FLEX, 1);
-            } else {
-                k_dot<double><<<nr, CB, 0, st>>>(n, z, r, red(FIN_RZ), c->fl);
-                fin(FIN_RZ);
-            }
-            k_xpby<double><<<nr, CB, 0, st>>>(n, z, d, c->sc, 0, c->fl);
+        for (int k = launched; k < launched + chunk; ++k) {
+            k_spmv_dir<double><<<nr, CB, 0, st>>>(L.nx, L.ny, D, E, Nn, z, dbuf[(k + 1) & 1],
+                                                  dbuf[k & 1], q, prz, nr, flex, k,
+                                                  red(ppq, FIN_NONE), c->fl);
+            k_axpy2_fold<double><<<nr, CB, 0, st>>>(n, x, r, r_old, flex ? c->r32 : nullptr,
+                                                    dbuf[k & 1], q, ppq, nr, k,
+                                                    red(prr, FIN_RR), c->fl);
+            k_fin<<<1, RED_MAX, 0, st>>>(red(prr, FIN_RR), nr);
+            if (int rc = precond(r_old)) return rc;
         }
         HIPCHK(hipGetLastError());
         launched += chunk;
@@ -1271,9 +1320,9 @@ static int residual64(zj_chorin* c, const double* rhs, const double* x, double* 
     const Red rr{c->part, c->sc, c->fl, 0.0, 1, FIN_RR0};
     k_cg_init<double><<<nr, CB, 0, st>>>(L.nx, L.ny, (double*)L.diag, (double*)L.cE,
                                          (double*)L.cN, (double*)x, 0, rhs, r, rb);
-    k_fin<<<1, RED_MAX, 0, st>>>(rb, nr, 0);
+    k_fin<<<1, RED_MAX, 0, st>>>(rb, nr);
     k_dot<double><<<nr, CB, 0, st>>>(c->N, r, r, rr, nullptr);
-    k_fin<<<1, RED_MAX, 0, st>>>(rr, nr, 0);
+    k_fin<<<1, RED_MAX, 0, st>>>(rr, nr);
     HIPCHK(hipGetLastError());
     if (int rc = read_flags(c, st)) return rc;
     double b2 = c->host_sc[CG_B2];
@@ -1413,11 +1462,12 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
     if (!rc) rc = dalloc(c, &c->r, N);
     if (!rc) rc = dalloc(c, &c->z, N);
     if (!rc) rc = dalloc(c, &c->d, N);
+    if (!rc) rc = dalloc(c, &c->d2, N);
     if (!rc) rc = dalloc(c, &c->q, N);
     if (!rc) rc = dalloc(c, &c->r32, N);
     if (!rc) rc = dalloc(c, &c->z32, N);
     if (!rc) rc = dalloc(c, &c->r_old, N);
-    if (!rc) rc = dalloc(c, &c->part, 2 * RED_MAX);
+    if (!rc) rc = dalloc(c, &c->part, 4 * RED_MAX);
     if (!rc) rc = dalloc(c, &c->sc, CG_NSLOT);
     if (!rc) rc = dalloc(c, &c->fl, 4);
     if (!rc && hipHostMalloc((void**)&c->host_fl, 4 * sizeof(int)) != hipSuccess) rc = ZJ_ENOMEM;

@@ -10,9 +10,11 @@
 
 namespace zj {
 
-// CG scalar slots and the scalar steps of k_cg_fin
-enum { CG_B2 = 0, CG_RZ, CG_ALPHA, CG_BETA, CG_RR, CG_NSLOT = 8 };
-enum { FIN_B2 = 0, FIN_RR0, FIN_RZ0, FIN_PQ, FIN_RR, FIN_RZ, FIN_RZ_FLEX };
+// CG scalar slots (CG_RZ and CG_RZ2 hold r.z of even and odd iterations)
+// and the scalar steps of k_fin (FIN_NONE: a reduction folded by the kernel
+// that uses it)
+enum { CG_B2 = 0, CG_RZ, CG_RR, CG_RZ2, CG_NSLOT = 8 };
+enum { FIN_B2 = 0, FIN_RR0, FIN_RR, FIN_NONE };
 
 // One level of the multigrid hierarchy (arrays of element size esz; h and b
 // are FP64 averages on levels >= 1).

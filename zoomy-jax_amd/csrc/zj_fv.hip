@@ -603,6 +603,40 @@ __device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const double* fx, cons
     return hmin < T(0) ? clip01(hbar / den) : T(1);
 }
 
+// The 2-D least-squares gradient from the normal-matrix sums of the
+// neighbour rows (S) and of the ghost rows that enter gradient component d
+// (G[d]): inverse rows (c0, c1) of system 0 (x component) and (c2, c3) of
+// system 1 (y component).  These helpers and face_increment spell out their
+// fused multiply-adds, so that every pass that computes a cell's slope (the
+// own-cell and the face-ring passes of the stage kernel) rounds it alike.
+template <typename T>
+__device__ __forceinline__ void lsq_inverse_rows(double Sxx, double Sxy, double Syy,
+                                                 const double* Gxx, const double* Gxy,
+                                                 const double* Gyy, T& c0, T& c1, T& c2,
+                                                 T& c3) {
+    const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
+    const double di0 = 1.0 / fma(axx, ayy, -(axy * axy));
+    const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
+    const double di1 = 1.0 / fma(bxx, byy, -(bxy * bxy));
+    c0 = T(ayy * di0);
+    c1 = T(-axy * di0);
+    c2 = T(-bxy * di1);
+    c3 = T(bxx * di1);
+}
+
+template <typename T>
+__device__ __forceinline__ void lsq_solve(T c0, T c1, T c2, T c3, T bx0, T by0, T bx1, T by1,
+                                          T& gx, T& gy) {
+    gx = fma(c0, bx0, c1 * by0);
+    gy = fma(c2, bx1, c3 * by1);
+}
+
+// Increment of a slope (gx, gy) over the offset (rx, ry).
+template <typename T>
+__device__ __forceinline__ T face_increment(T gx, T gy, T rx, T ry) {
+    return fma(gx, rx, gy * ry);
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
@@ -777,10 +811,20 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     __syncthreads();
 
     // ── phase 2: limited slopes of own + face-ring cells ─────────────
+    // With the numerics fixed to conservative variables and the Barth-
+    // Jespersen / minmod limiter (SPEC 1, 3) the variables' slopes are
+    // independent: in FP64 (latency-bound at 3 waves per SIMD) the own cells
+    // take one 4-variable pass, and the face-ring cells (64 of a 16 x 16
+    // quad patch, which as a second pass would keep one wave busy and three
+    // waiting) one variable per wave.  FP32 is bound by VALU issue, and
+    // there the per-variable pass's repeated geometry costs more than the
+    // idle waves.
+    constexpr bool SPLIT = (SPEC == 1 || SPEC == 3) && AUXONLY == 0 && sizeof(T) == 8;
     if (use_slopes) {
         // AUXONLY (post_step aux rows only, launched as mode 3): the own
         // cells' gradients, no limiter, no slopes, no face pass
-        for (int ls = tid; ls < nsl; ls += SBLOCK) {
+        const int nsl4 = SPLIT ? (nsl < nown ? nsl : nown) : nsl;
+        for (int ls = tid; ls < nsl4; ls += SBLOCK) {
             if (!PRE_TPL || ls != tid) row.load(tn, tv, te, m.mf, mxh, MS, ls);
             const int gi = (ls < nown && !ex) ? c0 + ls : lids[ls];
             const uint16_t* nls = row.nbr;
@@ -896,16 +940,12 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     ax[vv] = T(inv) * bx[vv]; ay[vv] = T(0);
                 }
             } else {
-                const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
-                const double di0 = 1.0 / (axx * ayy - axy * axy);
-                const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
-                const double di1 = 1.0 / (bxx * byy - bxy * bxy);
-                const T c0 = T(ayy * di0), c1 = T(-axy * di0);
-                const T c2 = T(-bxy * di1), c3 = T(bxx * di1);
+                T c0, c1, c2, c3;
+                lsq_inverse_rows(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) {
-                    gx[vv] = c0 * (bx[vv] + gbx[0][vv]) + c1 * (by[vv] + gby[0][vv]);
-                    gy[vv] = c2 * (bx[vv] + gbx[1][vv]) + c3 * (by[vv] + gby[1][vv]);
+                    lsq_solve(c0, c1, c2, c3, bx[vv] + gbx[0][vv], by[vv] + gby[0][vv],
+                              bx[vv] + gbx[1][vv], by[vv] + gby[1][vv], gx[vv], gy[vv]);
                     ax[vv] = c0 * bx[vv] + c1 * by[vv];
                     ay[vv] = c2 * bx[vv] + c3 * by[vv];
                 }
@@ -929,7 +969,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
-                        T d = gx[vv] * rx + gy[vv] * ry;
+                        const T d = face_increment(gx[vv], gy[vv], rx, ry);
                         dpos[vv] = tmax(dpos[vv], d);
                         dneg[vv] = tmin(dneg[vv], d);
                     }
@@ -944,7 +984,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
-                        T d = gx[vv] * rx + gy[vv] * ry;
+                        const T d = face_increment(gx[vv], gy[vv], rx, ry);
                         phi[vv] = tmin(phi[vv], lim_cand(o.lim, d, umax[vv] - w[vv], umin[vv] - w[vv], ev2));
                     }
                 }
@@ -1027,6 +1067,108 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     ro.lim[(size_t)(vv * m.dim) * N + gi] = sx[vv];
                     if (m.dim == 2) ro.lim[(size_t)(vv * 2 + 1) * N + gi] = sy[vv];
                 }
+            }
+        }
+        if (SPLIT) {
+            // face-ring cells: wave vs computes variable vs of 64 cells per
+            // round; the arithmetic of each variable is the 4-variable
+            // pass's, term for term
+            const int vs = __builtin_amdgcn_readfirstlane(tid >> 6);
+            const int half = vs >> 1, hi = vs & 1;
+            auto pick = [&](int i) {
+                const T2 pr = Qs[half * ML + i];
+                return hi ? pr.y : pr.x;
+            };
+            for (int r0 = nown; r0 < nsl; r0 += 64) {
+                const int ls = r0 + (tid & 63);
+                if (ls >= nsl) break;
+                TplRow<MXB / 2> rw;
+                rw.load(tn, tv, te, m.mf, mxh, MS, ls);
+                const int gi = lids[ls];
+                const T w = pick(ls);
+                double2 v[MAXF];
+                double fx[MAXF], fy[MAXF];
+                face_centres(m, Vs, rw.vert, org, v, fx, fy);
+                const double2 cc = c2d(Cs[ls]);
+                double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
+                double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
+                T bx = 0, by = 0, gbx[2] = {0, 0}, gby[2] = {0, 0};
+                T umax = w, umin = w;
+                ZJ_FOR_FACES(k) {
+                    const uint16_t nl = rw.nbr[k];
+                    if (nl != LNONE) {
+                        const T wn = pick(nl);
+                        const double2 cn = c2d(Cs[nl]);
+                        const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                        Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+                        const T dv = wn - w;
+                        bx += T(rx) * dv;
+                        by += T(ry) * dv;
+                        umax = tmax(umax, wn);
+                        umin = tmin(umin, wn);
+                    } else {
+                        double nx, ny, len;
+                        face_normal(m, v, k, nx, ny, len);
+                        const int bf = -1 - m.nbr[k * N + gi];
+                        const int kind = m.bf_kind[bf];
+                        T q[4], qg[4];
+                        lds_q4(Qs, ML, ls, q);
+                        ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
+                        const T wg = vs == 0 ? qg[0] : vs == 1 ? qg[1] : vs == 2 ? qg[2] : qg[3];
+                        const double rx = 2.0 * (fx[k] - cc.x), ry = 2.0 * (fy[k] - cc.y);
+                        const T bv = (kind == BC_PERIODIC) ? wg : T(2) * wg - w;
+                        umax = tmax(umax, bv);
+                        umin = tmin(umin, bv);
+                        const double an[2] = {fabs(nx), fabs(ny)};
+#pragma unroll
+                        for (int d = 0; d < 2; ++d) {
+                            if (an[d] > 0.5) {
+                                Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry;
+                                const T dv = wg - w;
+                                gbx[d] += T(rx) * dv;
+                                gby[d] += T(ry) * dv;
+                            }
+                        }
+                    }
+                }
+                auto extra_row = [&](uint16_t el) {
+                    const T wn = pick(el);
+                    const double2 cn = c2d(Cs[el]);
+                    const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                    Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
+                    const T dv = wn - w;
+                    bx += T(rx) * dv;
+                    by += T(ry) * dv;
+                };
+                for (int b2 = 0; b2 < mxh; b2 += MXB / 2) {
+                    uint32_t els[MXB > 0 ? MXB / 2 : 1];
+#pragma unroll
+                    for (int u = 0; u < MXB / 2; ++u)
+                        els[u] = b2 == 0 ? rw.ex[u] : b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
+#pragma unroll
+                    for (int j = 0; j < MXB; ++j) {
+                        const uint16_t el = (uint16_t)(els[j >> 1] >> (16 * (j & 1)));
+                        if (el != LNONE) extra_row(el);
+                    }
+                }
+                T c0, c1, c2, c3, gx, gy;
+                lsq_inverse_rows(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
+                lsq_solve(c0, c1, c2, c3, bx + gbx[0], by + gby[0], bx + gbx[1], by + gby[1],
+                          gx, gy);
+                T dpos = 0, dneg = 0;
+                ZJ_FOR_FACES(k) {
+                    const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
+                    const T d = face_increment(gx, gy, rx, ry);
+                    dpos = tmax(dpos, d);
+                    dneg = tmin(dneg, d);
+                }
+                T phi = bj_phi(dpos, dneg, umax - w, umin - w);
+                if (o1_mask && o1_mask[gi]) phi = T(0);
+                const T pv = clip01(phi);
+                T2 sv;
+                sv.x = pv * gx;
+                sv.y = pv * gy;
+                Ss[vs * MS + ls] = sv;
             }
         }
     }
