@@ -250,14 +250,34 @@ __device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
 // dp >= 0 >= dn) the smaller is picked by cross-multiplication,
 // dn / dmin < dp / dmax  <=>  dn dmax > dp dmin  (dmax > 0 > dmin),
 // which can pick the other one only where the two agree to rounding.
+// Returns false when no face increment exceeds eps (no limiting).
 template <typename T>
-__device__ __forceinline__ T bj_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
+__device__ __forceinline__ bool min_face_ratio(T dmax_pos, T dmin_neg, T dp, T dn, T& y) {
     const T eps = T(1e-30);
     const bool a1 = dmax_pos > eps, a2 = dmin_neg < -eps;
-    if (!a1 && !a2) return T(1);
+    if (!a1 && !a2) return false;
     const bool use2 = a2 && (!a1 || dn * dmax_pos > dp * dmin_neg);
     const T num = use2 ? dn : dp, den = use2 ? dmin_neg : dmax_pos;
-    return clip01(num / den);
+    y = num / den;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T bj_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
+    T y;
+    return min_face_ratio(dmax_pos, dmin_neg, dp, dn, y) ? clip01(y) : T(1);
+}
+
+// van Leer over all faces with one division: the per-face candidate
+// y (4 - y) / 4 (0 for y <= 0, 1 for y >= 2) is non-decreasing in the face
+// ratio y, so its minimum over the faces is its value at the smallest ratio,
+// which min_face_ratio gives.
+template <typename T>
+__device__ __forceinline__ T vl_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
+    T y;
+    if (!min_face_ratio(dmax_pos, dmin_neg, dp, dn, y)) return T(1);
+    if (y <= T(0)) return T(0);
+    return y >= T(2) ? T(1) : y * (T(4) - y) * T(0.25);
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs of the chip; remap the

@@ -963,7 +963,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
             if (AUXONLY == 1) continue;
             T phi[4] = {1, 1, 1, 1};
-            if (o.lim == LIM_BJ || o.lim == LIM_MINMOD) {
+            if (o.lim == LIM_BJ || o.lim == LIM_MINMOD || o.lim == LIM_VANLEER) {
+                // the extreme face increments, then one division per variable
                 T dpos[4] = {0, 0, 0, 0}, dneg[4] = {0, 0, 0, 0};
                 ZJ_FOR_FACES(k) {
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
@@ -976,7 +977,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv)
-                    phi[vv] = bj_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv]);
+                    phi[vv] = o.lim == LIM_VANLEER
+                                  ? vl_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv])
+                                  : bj_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv]);
             } else if (o.lim != LIM_NONE) {
                 const double vol = cell_volume(m, v);
                 const T ev2 = T(m.dim == 1 ? vol * vol : fabs(vol));
