@@ -527,49 +527,53 @@ struct TplRow {
 };
 
 
-// Face centres of a cell's slots from the staged vertices (cheap: no sqrt).
-template <typename V>
-__device__ __forceinline__ double2 c2d(V a) {
-    return make_double2((double)a.x, (double)a.y);
-}
-
 // Face centres of a cell with local vertex ids vl, relative to org (the
-// frame of the staged centres).
+// frame of the staged centres), in the precision T of the stage: formed in
+// FP64 from the staged FP64 vertices, then rounded once.
+template <typename T>
 __device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs,
                                              const uint16_t* vl, double2 org, double2* v,
-                                             double* fx, double* fy) {
+                                             T* fx, T* fy) {
 #pragma unroll
     for (int k = 0; k < MAXF; ++k) v[k] = k < m.mf ? Vs[vl[k]] : make_double2(0.0, 0.0);
     if (m.dim == 1) {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            fx[k] = v[k].x - org.x;
-            fy[k] = 0.0;
+            fx[k] = T(v[k].x - org.x);
+            fy[k] = T(0);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             const double2 b = (k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0];
-            fx[k] = 0.5 * (v[k].x + b.x) - org.x;
-            fy[k] = 0.5 * (v[k].y + b.y) - org.y;
+            fx[k] = T(0.5 * (v[k].x + b.x) - org.x);
+            fy[k] = T(0.5 * (v[k].y + b.y) - org.y);
         }
     }
 }
 
-// Outward unit normal and length of slot k.
-__device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, int k,
-                                            double& nx, double& ny, double& len) {
+// Outward unit normal and length of slot k in the precision T of the stage.
+// The edge vector is formed in FP64 from the global vertices, so the two
+// cells of a face (which run the edge in opposite directions) get exactly
+// negated normals and the same length.
+template <typename T>
+__device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, int k, T& nx,
+                                            T& ny, T& len) {
     if (m.dim == 1) {
-        nx = k == 0 ? -1.0 : 1.0;
-        ny = 0.0;
-        len = 1.0;
+        nx = k == 0 ? T(-1) : T(1);
+        ny = T(0);
+        len = T(1);
         return;
     }
     const double2 a = v[k];
     const double2 b = ((k + 1 < m.mf) ? v[k + 1 < MAXF ? k + 1 : 0] : v[0]);
-    const double ex = b.x - a.x, ey = b.y - a.y;
-    const double l2 = ex * ex + ey * ey;
-    const double il = rsqrt(l2);
+    const T ex = T(b.x - a.x), ey = T(b.y - a.y);
+    const T l2 = ex * ex + ey * ey;
+    T il;
+    if constexpr (sizeof(T) == 4)
+        il = rsqrtf(l2);
+    else
+        il = rsqrt(l2);
     len = l2 * il;
     nx = ey * il;
     ny = -ex * il;
@@ -592,13 +596,13 @@ __device__ __forceinline__ double cell_volume(const DevMesh& m, const double2* v
 // (sx, sy): theta = clip(hbar / max(hbar - min_f h_f, 1e-14), 0, 1) when the
 // face-midpoint depth h_f = hbar + s . (x_f - x_c) is negative on some face,
 // else 1 (reconstruction_jax.py:840-850).
-template <typename T>
-__device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const double* fx, const double* fy,
-                                      double2 cc, int mf) {
+template <typename T, typename T2>
+__device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const T* fx, const T* fy, T2 cc,
+                                      int mf) {
     T hmin = T(INFINITY);
 #pragma unroll
     for (int k = 0; k < MAXF; ++k)
-        if (k < mf) hmin = tmin(hmin, hbar + (sx * T(fx[k] - cc.x) + sy * T(fy[k] - cc.y)));
+        if (k < mf) hmin = tmin(hmin, hbar + (sx * (fx[k] - cc.x) + sy * (fy[k] - cc.y)));
     const T den = tmax(hbar - hmin, T(1e-14));
     return hmin < T(0) ? clip01(hbar / den) : T(1);
 }
@@ -610,18 +614,17 @@ __device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const double* fx, cons
 // fused multiply-adds, so that every pass that computes a cell's slope (the
 // own-cell and the face-ring passes of the stage kernel) rounds it alike.
 template <typename T>
-__device__ __forceinline__ void lsq_inverse_rows(double Sxx, double Sxy, double Syy,
-                                                 const double* Gxx, const double* Gxy,
-                                                 const double* Gyy, T& c0, T& c1, T& c2,
-                                                 T& c3) {
-    const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
-    const double di0 = 1.0 / fma(axx, ayy, -(axy * axy));
-    const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
-    const double di1 = 1.0 / fma(bxx, byy, -(bxy * bxy));
-    c0 = T(ayy * di0);
-    c1 = T(-axy * di0);
-    c2 = T(-bxy * di1);
-    c3 = T(bxx * di1);
+__device__ __forceinline__ void lsq_inverse_rows(T Sxx, T Sxy, T Syy, const T* Gxx,
+                                                 const T* Gxy, const T* Gyy, T& c0, T& c1,
+                                                 T& c2, T& c3) {
+    const T axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
+    const T di0 = T(1) / fma(axx, ayy, -(axy * axy));
+    const T bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
+    const T di1 = T(1) / fma(bxx, byy, -(bxy * bxy));
+    c0 = ayy * di0;
+    c1 = -axy * di0;
+    c2 = -bxy * di1;
+    c3 = bxx * di1;
 }
 
 template <typename T>
@@ -835,11 +838,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             T w[4];
             to_w(o.recon, q, hic, w);
             double2 v[MAXF];
-            double fx[MAXF], fy[MAXF];
+            T fx[MAXF], fy[MAXF];
             face_centres(m, Vs, row.vert, org, v, fx, fy);
-            const double2 cc = c2d(Cs[ls]);
-            double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
-            double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
+            const T2 cc = Cs[ls];
+            T Sxx = 0, Sxy = 0, Syy = 0;
+            T Gxx[2] = {0, 0}, Gxy[2] = {0, 0}, Gyy[2] = {0, 0};
             T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
             T gbx[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
             T gby[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -852,8 +855,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     T qn[4], wn[4];
                     lds_q4(Qs, ML, nl, qn);
                     to_w(o.recon, qn, Hs[nl], wn);
-                    const double2 cn = c2d(Cs[nl]);
-                    const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                    const T2 cn = Cs[nl];
+                    const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
@@ -864,21 +867,21 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         umin[vv] = tmin(umin[vv], wn[vv]);
                     }
                 } else {
-                    double nx, ny, len;
+                    T nx, ny, len;
                     face_normal(m, v, k, nx, ny, len);
                     const int bf = -1 - m.nbr[k * N + gi];
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
                     ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
                     to_w(o.recon, qg, hinv_of(qg[1], eps), wg);
-                    const double rx = 2.0 * (fx[k] - cc.x), ry = 2.0 * (fy[k] - cc.y);
+                    const T rx = T(2) * (fx[k] - cc.x), ry = T(2) * (fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
                         T bv = (kind == BC_PERIODIC) ? wg[vv] : T(2) * wg[vv] - w[vv];
                         umax[vv] = tmax(umax[vv], bv);
                         umin[vv] = tmin(umin[vv], bv);
                     }
-                    const double an[2] = {fabs(nx), fabs(ny)};
+                    const T an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
                     for (int d = 0; d < 2; ++d) {
                         if (an[d] > 0.5) {
@@ -897,8 +900,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 T qn[4], wn[4];
                 lds_q4(Qs, ML, el, qn);
                 to_w(o.recon, qn, Hs[el], wn);
-                const double2 cn = c2d(Cs[el]);
-                const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                const T2 cn = Cs[el];
+                const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                 Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) {
@@ -933,11 +936,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
             T gx[4], gy[4], ax[4], ay[4];
             if (m.dim == 1) {
-                const double inv = 1.0 / (Sxx + Gxx[0]);
+                const T inv = T(1) / (Sxx + Gxx[0]);
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) {
-                    gx[vv] = T(inv) * (bx[vv] + gbx[0][vv]); gy[vv] = T(0);
-                    ax[vv] = T(inv) * bx[vv]; ay[vv] = T(0);
+                    gx[vv] = inv * (bx[vv] + gbx[0][vv]); gy[vv] = T(0);
+                    ax[vv] = inv * bx[vv]; ay[vv] = T(0);
                 }
             } else {
                 T c0, c1, c2, c3;
@@ -1090,19 +1093,19 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 const int gi = lids[ls];
                 const T w = pick(ls);
                 double2 v[MAXF];
-                double fx[MAXF], fy[MAXF];
+                T fx[MAXF], fy[MAXF];
                 face_centres(m, Vs, rw.vert, org, v, fx, fy);
-                const double2 cc = c2d(Cs[ls]);
-                double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
-                double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
+                const T2 cc = Cs[ls];
+                T Sxx = 0, Sxy = 0, Syy = 0;
+                T Gxx[2] = {0, 0}, Gxy[2] = {0, 0}, Gyy[2] = {0, 0};
                 T bx = 0, by = 0, gbx[2] = {0, 0}, gby[2] = {0, 0};
                 T umax = w, umin = w;
                 ZJ_FOR_FACES(k) {
                     const uint16_t nl = rw.nbr[k];
                     if (nl != LNONE) {
                         const T wn = pick(nl);
-                        const double2 cn = c2d(Cs[nl]);
-                        const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                        const T2 cn = Cs[nl];
+                        const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                         Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
                         const T dv = wn - w;
                         bx += T(rx) * dv;
@@ -1110,7 +1113,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         umax = tmax(umax, wn);
                         umin = tmin(umin, wn);
                     } else {
-                        double nx, ny, len;
+                        T nx, ny, len;
                         face_normal(m, v, k, nx, ny, len);
                         const int bf = -1 - m.nbr[k * N + gi];
                         const int kind = m.bf_kind[bf];
@@ -1118,11 +1121,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         lds_q4(Qs, ML, ls, q);
                         ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
                         const T wg = vs == 0 ? qg[0] : vs == 1 ? qg[1] : vs == 2 ? qg[2] : qg[3];
-                        const double rx = 2.0 * (fx[k] - cc.x), ry = 2.0 * (fy[k] - cc.y);
+                        const T rx = T(2) * (fx[k] - cc.x), ry = T(2) * (fy[k] - cc.y);
                         const T bv = (kind == BC_PERIODIC) ? wg : T(2) * wg - w;
                         umax = tmax(umax, bv);
                         umin = tmin(umin, bv);
-                        const double an[2] = {fabs(nx), fabs(ny)};
+                        const T an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
                         for (int d = 0; d < 2; ++d) {
                             if (an[d] > 0.5) {
@@ -1136,8 +1139,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
                 auto extra_row = [&](uint16_t el) {
                     const T wn = pick(el);
-                    const double2 cn = c2d(Cs[el]);
-                    const double rx = cn.x - cc.x, ry = cn.y - cc.y;
+                    const T2 cn = Cs[el];
+                    const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
                     const T dv = wn - w;
                     bx += T(rx) * dv;
@@ -1212,9 +1215,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
             }
             double2 v[MAXF];
-            double fx[MAXF], fy[MAXF];
+            T fx[MAXF], fy[MAXF];
             face_centres(m, Vs, row3.vert, org, v, fx, fy);
-            const double2 cc = c2d(Cs[tid]);
+            const T2 cc = Cs[tid];
             ZJ_FOR_FACES(k) {
                 T qf[4] = {q[0], q[1], q[2], q[3]};
                 if (use_slopes)
@@ -1226,7 +1229,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 for (int vv = 0; vv < 4; ++vv) dst[(size_t)vv * ro.F + f] = qf[vv];
                 const int nb = m.nbr[(size_t)k * N + gi];
                 if (nb < 0) {
-                    double nxd, nyd, len;
+                    T nxd, nyd, len;
                     face_normal(m, v, k, nxd, nyd, len);
                     T qg[4];
                     ghost_state(m, Qin, -1 - nb, q, T(nxd), T(nyd), qg);
@@ -1265,13 +1268,13 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
         }
         double2 v[MAXF];
-        double fx[MAXF], fy[MAXF];
+        T fx[MAXF], fy[MAXF];
         face_centres(m, Vs, row3.vert, org, v, fx, fy);
-        const double2 cc = c2d(Cs[ls]);
-        double nxs[MAXF], nys[MAXF];
+        const T2 cc = Cs[ls];
+        T nxs[MAXF], nys[MAXF];
         T acc[4] = {0, 0, 0, 0};
         ZJ_FOR_FACES(k) {
-            double nxd, nyd, len;
+            T nxd, nyd, len;
             face_normal(m, v, k, nxd, nyd, len);
             nxs[k] = nxd;
             nys[k] = nyd;
@@ -1295,7 +1298,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         sn[2 * vv] = sv.x;
                         sn[2 * vv + 1] = sv.y;
                     }
-                    const double2 cn = c2d(Cs[nl]);
+                    const T2 cn = Cs[nl];
                     recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
                 } else {
                     qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
