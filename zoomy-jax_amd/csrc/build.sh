@@ -21,8 +21,9 @@ for src in "$HERE"/*.hip; do
 done
 "$HIPCC" -O2 -std=c++17 -fPIC -c "$HERE/zj_rcm.cpp" -o "$HERE/_build/zj_rcm.o"
 "$HIPCC" -O2 -std=c++17 -fPIC -c "$HERE/zj_patches.cpp" -o "$HERE/_build/zj_patches.o"
+"$HIPCC" -O2 -std=c++17 -fPIC -c "$HERE/zj_io.cpp" -o "$HERE/_build/zj_io.o"
 for pid in "${PIDS[@]}"; do wait "$pid"; done
-OBJS+=("$HERE/_build/zj_rcm.o" "$HERE/_build/zj_patches.o")
-"$HIPCC" -shared -fPIC --offload-arch="$ARCH" -o "$OUT/libzoomy_hip.so.tmp" "${OBJS[@]}"
+OBJS+=("$HERE/_build/zj_rcm.o" "$HERE/_build/zj_patches.o" "$HERE/_build/zj_io.o")
+"$HIPCC" -shared -fPIC --offload-arch="$ARCH" -o "$OUT/libzoomy_hip.so.tmp" "${OBJS[@]}" -ldl -lpthread
 mv -f "$OUT/libzoomy_hip.so.tmp" "$OUT/libzoomy_hip.so"
 echo "built $OUT/libzoomy_hip.so"

@@ -12,6 +12,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libzoomy_hip.so"
 
 ZJ_OK, ZJ_EINVAL, ZJ_EHIP, ZJ_ENOMEM, ZJ_EUNSUP, ZJ_ESOLVE = 0, 1, 2, 3, 4, 5
+ZJ_EIO, ZJ_EEXIST = 6, 7
 PRECOND = {"mg": 0, "learned": 1, "mixed": 2, "learned_guess": 3}
 LIMITERS = {"none": 0, "barth_jespersen": 1, "minmod": 2,
             "venkatakrishnan": 3, "vanleer": 4}
@@ -229,8 +230,22 @@ _SIGS_POISSON = {
 }
 
 
+_I64 = ctypes.c_int64
+# the HDF5 snapshot writer (include/zoomy_io.h)
+_SIGS_IO = {
+    "zj_io_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32,
+                                  ctypes.POINTER(_V)]),
+    "zj_io_write": (ctypes.c_int, [_V, _I64, _D, _V, ctypes.c_int32, _I64,
+                                   _I64, _V, _I64]),
+    "zj_io_write_async": (ctypes.c_int, [_V, _I64, _D, _V, ctypes.c_int32,
+                                         _I64, _I64, _V, _I64, _V]),
+    "zj_io_flush": (ctypes.c_int, [_V]),
+    "zj_io_close": (ctypes.c_int, [_V]),
+}
+
+
 def exported_symbols():
-    return sorted(list(_SIGS) + list(_SIGS_POISSON))
+    return sorted(list(_SIGS) + list(_SIGS_POISSON) + list(_SIGS_IO))
 
 
 def lib():
@@ -242,7 +257,8 @@ def lib():
                 f"{LIB_PATH} not found: build it with "
                 f"`bash zoomy-jax_amd/csrc/build.sh` (or __graft_entry__.build())")
         L = ctypes.CDLL(str(LIB_PATH))
-        for name, (res, args) in {**_SIGS, **_SIGS_POISSON}.items():
+        for name, (res, args) in {**_SIGS, **_SIGS_POISSON,
+                                  **_SIGS_IO}.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -255,5 +271,7 @@ def check(rc, what=""):
         names = {ZJ_EINVAL: "invalid argument", ZJ_EHIP: "HIP runtime error",
                  ZJ_ENOMEM: "out of device memory",
                  ZJ_EUNSUP: "unsupported option",
-                 ZJ_ESOLVE: "solve missed its tolerance or accept gate"}
+                 ZJ_ESOLVE: "solve missed its tolerance or accept gate",
+                 ZJ_EIO: "HDF5 library error",
+                 ZJ_EEXIST: "the snapshot group exists"}
         raise ZoomyHipError(f"{what}: {names.get(rc, rc)} (code {rc})")

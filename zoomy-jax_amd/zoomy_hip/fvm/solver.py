@@ -249,7 +249,8 @@ class HyperbolicSolver:
                  positivity_method: str = "",
                  front_theta_tol: Optional[float] = None,
                  stage_aux: str = "refresh", patch_cells: int = 0,
-                 patch_order: str = "auto"):
+                 patch_order: str = "auto", output_directory: str = "output",
+                 output_filename: str = "simulation", snapshots: int = 10):
         """``reconstruction_variables``: ``"conservative"``, ``"eta"``
         (well-balanced wet/dry recipe on (b, eta, u)), ``"eta_hu"`` (the
         same on (b, eta, hu) with the 1e-3 depth threshold of the class it
@@ -262,7 +263,10 @@ class HyperbolicSolver:
         ``solver_jax.py:1026-1041``) or ``"frozen"`` (the step-start aux,
         with which the reference's order-2 goldens were recorded).
         ``patch_cells`` / ``patch_order`` steer the device layout only (the
-        stage kernel's patches; results do not depend on them)."""
+        stage kernel's patches; results do not depend on them).
+        ``output_directory``, ``output_filename`` and ``snapshots`` are the
+        reference's ``settings.output`` fields read by
+        ``run_simulation(write_output=True)`` (``solver_jax.py:1264-1292``)."""
         if stage_aux not in L.STAGE_AUX:
             raise ValueError(
                 f"stage_aux must be one of {sorted(L.STAGE_AUX)}")
@@ -290,6 +294,11 @@ class HyperbolicSolver:
         # a-posteriori MOOD (solver_jax.py:1055-1071): redo a step with the
         # troubled cells (h < 0 or non-finite) at first order
         self.mood = bool(mood)
+        if int(snapshots) < 1:
+            raise ValueError("snapshots must be >= 1")
+        self.output_directory = output_directory
+        self.output_filename = output_filename
+        self.snapshots = int(snapshots)
 
     # ── setup ─────────────────────────────────────────────────────────
     def initialize(self, mesh: Mesh, model: ShallowWater):
@@ -561,17 +570,36 @@ class HyperbolicSolver:
     def run_simulation(self, Q, Qaux, write_output=False, parameters=None,
                        max_steps: int = 10 ** 9):
         """Adaptive march to ``time_end``; the dt guard reads dt each step
-        (the solver2 march's host guard)."""
+        (the solver2 march's host guard).  With ``write_output``, HDF5
+        snapshots go to ``<output_directory>/<output_filename>.h5`` at the
+        reference's schedule (``solver_jax.py:1264-1322``): the initial
+        state, then after every step that reaches the next multiple of
+        ``time_end / (snapshots - 1)``."""
+        save = None
+        if write_output:
+            import os
+            from ..io import get_save_fields
+            os.makedirs(self.output_directory, exist_ok=True)
+            save = get_save_fields(os.path.join(
+                self.output_directory, f"{self.output_filename}.h5"))
+            dt_snapshot = self.time_end / max(self.snapshots - 1, 1)
+            i_snap = save(0.0, 0.0, 0.0, Q, Qaux)
         t = 0.0
         it = 0
-        while t < self.time_end and it < max_steps:
-            dt = self.compute_timestep(Q, Qaux,
-                                       t_remaining=self.time_end - t)
-            d = self.assert_dt_admissible(dt, t, it)
-            Qn = self.step(dt, t, Q, Qaux)
-            Q, Qaux = self.post_step(t + d, d, Qn, Q, Qaux)
-            t += d
-            it += 1
+        try:
+            while t < self.time_end and it < max_steps:
+                dt = self.compute_timestep(Q, Qaux,
+                                           t_remaining=self.time_end - t)
+                d = self.assert_dt_admissible(dt, t, it)
+                Qn = self.step(dt, t, Q, Qaux)
+                Q, Qaux = self.post_step(t + d, d, Qn, Q, Qaux)
+                t += d
+                it += 1
+                if save is not None:
+                    i_snap = save(t, i_snap * dt_snapshot, i_snap, Q, Qaux)
+        finally:
+            if save is not None:
+                save.writer.close()
         self.n_steps_taken = it
         return Q, Qaux
 
