@@ -38,10 +38,25 @@ template <typename T> __device__ __forceinline__ T clip01(T x) {
     return x < T(0) ? T(0) : (x > T(1) ? T(1) : x);
 }
 
+// Reciprocal, square root and quotient.  FP64: the correctly rounded
+// operations.  FP32: the hardware's v_rcp_f32 and v_sqrt_f32 (1 ulp) in
+// place of the correctly rounded multi-instruction sequences -- the FP32
+// kernels are bound by VALU issue, and an ulp of FP32 lies far inside what
+// FP32 results are held to.
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <typename T> __device__ __forceinline__ T hinv_of(T h, T eps) {
     T m = tmax(h, eps);
     T m2 = m * m, h2 = h * h;
-    return T(1.4142135623730951) * h / sqrt(h2 * h2 + m2 * m2);
+    if constexpr (sizeof(T) == 4)
+        return T(1.4142135623730951) * h * __builtin_amdgcn_rsqf(h2 * h2 + m2 * m2);
+    else
+        return T(1.4142135623730951) * h / sqrt(h2 * h2 + m2 * m2);
 }
 
 // Ghost state at a boundary face from the inner state q (Wall mirrors the
@@ -92,7 +107,7 @@ __device__ __forceinline__ T lim_cand(int lim, T d, T dp, T dn, T ev2) {
 template <typename T>
 __device__ __forceinline__ T max_abs_eig(const T* q, T hinv, T nx, T ny, T g) {
     T un = (q[2] * nx + q[3] * ny) * hinv;
-    return fabs(un) + sqrt(g * tmax(q[1], T(0)));
+    return fabs(un) + fsqrt(g * tmax(q[1], T(0)));
 }
 
 // F + Dm for the evaluating cell (left state qL, neighbour qR, outward n).
@@ -110,7 +125,7 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
         T uL = qL[2] * hiL, vL = qL[3] * hiL;
         T uR = qR[2] * hiR, vR = qR[3] * hiR;
         T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
-        T cL = sqrt(g * hLs), cR = sqrt(g * hRs);
+        T cL = fsqrt(g * hLs), cR = fsqrt(g * hRs);
         T sL = tmin(unL - cL, unR - cR);
         T sR = tmax(unL + cL, unR + cR);
         T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
@@ -122,7 +137,7 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
         } else if (sR <= T(0)) {
             F0 = FR0; F1 = FR1; F2 = FR2;
         } else {
-            T inv = T(1) / (sR - sL);
+            T inv = frcp(sR - sL);
             T ss = sL * sR;
             F0 = (sR * FL0 - sL * FR0 + ss * (hRs - hLs)) * inv;
             F1 = (sR * FL1 - sL * FR1 + ss * (hRs * uR - hLs * uL)) * inv;
@@ -139,8 +154,8 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
     T unL = (qL[2] * nx + qL[3] * ny) * hiL;
     T unR = (qR[2] * nx + qR[3] * ny) * hiR;
     T pL = T(0.5) * g * qL[1] * qL[1], pR = T(0.5) * g * qR[1] * qR[1];
-    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + sqrt(g * tmax(qL[1], T(0))),
-               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + sqrt(g * tmax(qR[1], T(0))));
+    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + fsqrt(g * tmax(qL[1], T(0))),
+               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + fsqrt(g * tmax(qR[1], T(0))));
     T d0 = qR[0] - qL[0], d1 = qR[1] - qL[1], d2 = qR[2] - qL[2], d3 = qR[3] - qL[3];
     T Bd = g * (T(0.5) * (qL[1] + qR[1])) * d0;
     T diss0, diss1;
@@ -171,7 +186,7 @@ __device__ __forceinline__ void face_split(int rs, const T* qL, const T* qR, T h
         T uL = qL[2] * hiL, vL = qL[3] * hiL;
         T uR = qR[2] * hiR, vR = qR[3] * hiR;
         T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
-        T cL = sqrt(g * hLs), cR = sqrt(g * hRs);
+        T cL = fsqrt(g * hLs), cR = fsqrt(g * hRs);
         T sL = tmin(unL - cL, unR - cR);
         T sR = tmax(unL + cL, unR + cR);
         T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
@@ -183,7 +198,7 @@ __device__ __forceinline__ void face_split(int rs, const T* qL, const T* qR, T h
         for (int k = 0; k < 3; ++k) {
             if (sL >= T(0)) F[k + 1] = FL[k];
             else if (sR <= T(0)) F[k + 1] = FR[k];
-            else F[k + 1] = (sR * FL[k] - sL * FR[k] + sL * sR * (UR[k] - UL[k])) / (sR - sL);
+            else F[k + 1] = fdiv(sR * FL[k] - sL * FR[k] + sL * sR * (UR[k] - UL[k]), sR - sL);
         }
         F[0] = T(0);
         // hydrostatic pressure corrections g/2 (h^2 - h*^2) n
@@ -200,8 +215,8 @@ __device__ __forceinline__ void face_split(int rs, const T* qL, const T* qR, T h
     T unL = (qL[2] * nx + qL[3] * ny) * hiL;
     T unR = (qR[2] * nx + qR[3] * ny) * hiR;
     T pL = T(0.5) * g * qL[1] * qL[1], pR = T(0.5) * g * qR[1] * qR[1];
-    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + sqrt(g * tmax(qL[1], T(0))),
-               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + sqrt(g * tmax(qR[1], T(0))));
+    T s = tmax(fabs((qL[2] * nx + qL[3] * ny) * hinv_of(qL[1], eps)) + fsqrt(g * tmax(qL[1], T(0))),
+               fabs((qR[2] * nx + qR[3] * ny) * hinv_of(qR[1], eps)) + fsqrt(g * tmax(qR[1], T(0))));
     T d[4] = {qR[0] - qL[0], qR[1] - qL[1], qR[2] - qL[2], qR[3] - qL[3]};
     T Bd = g * (T(0.5) * (qL[1] + qR[1])) * d[0];
     F[0] = T(0);
@@ -258,7 +273,7 @@ __device__ __forceinline__ bool min_face_ratio(T dmax_pos, T dmin_neg, T dp, T d
     if (!a1 && !a2) return false;
     const bool use2 = a2 && (!a1 || dn * dmax_pos > dp * dmin_neg);
     const T num = use2 ? dn : dp, den = use2 ? dmin_neg : dmax_pos;
-    y = num / den;
+    y = fdiv(num, den);
     return true;
 }
 

@@ -1319,7 +1319,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 acc[3] += ce[3] * L;
             }
         }
-        const T ivol = T(1.0 / cell_volume(m, v));
+        const T ivol = frcp(T(cell_volume(m, v)));
         T out[4];
         if (mode == 2) {
 #pragma unroll
@@ -1348,16 +1348,17 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             if (bad_out) bad_out[gi] = (uint8_t)bad;
             if (dt_part || (ENS && ea.part2)) {
                 // min over faces of r_f / a_f without a division per face
+                // (in the stage precision; the products of the comparison
+                // are exact in FP64 for FP32 operands)
                 const T hn = hinv_of(out[1], eps);
-                const double c = sqrt(o.g * fmax((double)out[1], 0.0));
-                const double rc = Rs[ls];
+                const T c = fsqrt(g * tmax(out[1], T(0)));
+                const T rc = Rs[ls];
                 ZJ_FOR_FACES(k) {
                     const uint16_t nl = nls[k];
-                    const double rf = nl != LNONE ? fmin(rc, Rs[nl]) : rc;
-                    const double un = (double)((out[2] * T(nxs[k]) + out[3] * T(nys[k])) * hn);
-                    const double a = fabs(un) + c;
+                    const T rf = nl != LNONE ? tmin(rc, Rs[nl]) : rc;
+                    const T a = fabs((out[2] * nxs[k] + out[3] * nys[k]) * hn) + c;
                     if (!(a == a)) { best_r = 0.0; best_a = 1.0; }
-                    else if (rf * best_a < best_r * a) { best_r = rf; best_a = a; }
+                    else if ((double)rf * best_a < best_r * (double)a) { best_r = rf; best_a = a; }
                 }
             }
         }
