@@ -38,25 +38,81 @@ template <typename T> __device__ __forceinline__ T clip01(T x) {
     return x < T(0) ? T(0) : (x > T(1) ? T(1) : x);
 }
 
-// Reciprocal, square root and quotient.  FP64: the correctly rounded
-// operations.  FP32: the hardware's v_rcp_f32 and v_sqrt_f32 (1 ulp) in
-// place of the correctly rounded multi-instruction sequences -- the FP32
-// kernels are bound by VALU issue, and an ulp of FP32 lies far inside what
-// FP32 results are held to.
-__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// FP64 square root, reciprocal and quotient by Newton-Raphson from the
+// hardware estimates v_rsq_f64 and v_rcp_f64, for normal finite operands.
+// lean_sqrt refines the Goldschmidt pair (g, h) ~ (sqrt x, 1 / (2 sqrt x))
+// -- one coupled step, then two corrections of g -- which is the correctly
+// rounded square root for x >= 2^-767; smaller, zero, negative and NaN
+// operands take sqrt() itself (a branch that no lane normally enters), so
+// lean_sqrt equals sqrt() everywhere and only drops the scaling and
+// special-value steps from the common path.  lean_rcp (two corrections),
+// lean_div (lean_rcp and one residual correction) and lean_rsqrt (two
+// corrections) are within an ulp of the correctly rounded results.
+__device__ __forceinline__ double lean_sqrt(double x) {
+    if (!(x >= 0x1p-767)) return sqrt(x);
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+__device__ __forceinline__ double lean_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double lean_div(double a, double b) {
+    const double r = lean_rcp(b);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double lean_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return fma(y, fma(-hx * y, y, 0.5), y);
+}
 
-template <typename T> __device__ __forceinline__ T hinv_of(T h, T eps) {
+// Reciprocal, square root and quotient of the kernels.  FP32: the
+// hardware's v_rcp_f32 and v_sqrt_f32 (1 ulp) in place of the correctly
+// rounded multi-instruction sequences -- the FP32 kernels are bound by VALU
+// issue, and an ulp of FP32 lies far inside what FP32 results are held to.
+// FP64: the correctly rounded operations (square root through lean_sqrt);
+// with LEAN (the quad and triangle instantiations with the numerics fixed
+// at compile time) the reciprocal and the quotient through lean_rcp and
+// lean_div, which shortens the latency chains of the FP64 stage kernel.
+template <bool LEAN = false> __device__ __forceinline__ double frcp(double x) {
+    if constexpr (LEAN) return lean_rcp(x);
+    else return 1.0 / x;
+}
+template <bool LEAN = false> __device__ __forceinline__ float frcp(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ double fsqrt(double x) { return lean_sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <bool LEAN = false> __device__ __forceinline__ double fdiv(double a, double b) {
+    if constexpr (LEAN) return lean_div(a, b);
+    else return a / b;
+}
+template <bool LEAN = false> __device__ __forceinline__ float fdiv(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
+template <typename T, bool LEAN = false> __device__ __forceinline__ T hinv_of(T h, T eps) {
     T m = tmax(h, eps);
     T m2 = m * m, h2 = h * h;
     if constexpr (sizeof(T) == 4)
         return T(1.4142135623730951) * h * __builtin_amdgcn_rsqf(h2 * h2 + m2 * m2);
+    else if constexpr (LEAN)
+        return T(1.4142135623730951) * h * lean_rsqrt(h2 * h2 + m2 * m2);
     else
-        return T(1.4142135623730951) * h / sqrt(h2 * h2 + m2 * m2);
+        return T(1.4142135623730951) * h / lean_sqrt(h2 * h2 + m2 * m2);
 }
 
 // Ghost state at a boundary face from the inner state q (Wall mirrors the
@@ -114,7 +170,7 @@ __device__ __forceinline__ T max_abs_eig(const T* q, T hinv, T nx, T ny, T g) {
 // hiL/hiR are the cell aux 1/h used by the fluxes; the Rusanov speed s_M
 // takes 1/h of each face state itself (what the reference's periodic-seam
 // golden pins: there the ghost is the partner cell, the aux the inner one's).
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
                                                T hiL, T hiR, T nx, T ny, T g,
                                                T eps, T* out) {
@@ -137,7 +193,7 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
         } else if (sR <= T(0)) {
             F0 = FR0; F1 = FR1; F2 = FR2;
         } else {
-            T inv = frcp(sR - sL);
+            T inv = frcp<LEAN>(sR - sL);
             T ss = sL * sR;
             F0 = (sR * FL0 - sL * FR0 + ss * (hRs - hLs)) * inv;
             F1 = (sR * FL1 - sL * FR1 + ss * (hRs * uR - hLs * uL)) * inv;
@@ -266,31 +322,31 @@ __device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
 // dn / dmin < dp / dmax  <=>  dn dmax > dp dmin  (dmax > 0 > dmin),
 // which can pick the other one only where the two agree to rounding.
 // Returns false when no face increment exceeds eps (no limiting).
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ bool min_face_ratio(T dmax_pos, T dmin_neg, T dp, T dn, T& y) {
     const T eps = T(1e-30);
     const bool a1 = dmax_pos > eps, a2 = dmin_neg < -eps;
     if (!a1 && !a2) return false;
     const bool use2 = a2 && (!a1 || dn * dmax_pos > dp * dmin_neg);
     const T num = use2 ? dn : dp, den = use2 ? dmin_neg : dmax_pos;
-    y = fdiv(num, den);
+    y = fdiv<LEAN>(num, den);
     return true;
 }
 
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ T bj_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
     T y;
-    return min_face_ratio(dmax_pos, dmin_neg, dp, dn, y) ? clip01(y) : T(1);
+    return min_face_ratio<T, LEAN>(dmax_pos, dmin_neg, dp, dn, y) ? clip01(y) : T(1);
 }
 
 // van Leer over all faces with one division: the per-face candidate
 // y (4 - y) / 4 (0 for y <= 0, 1 for y >= 2) is non-decreasing in the face
 // ratio y, so its minimum over the faces is its value at the smallest ratio,
 // which min_face_ratio gives.
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ T vl_phi(T dmax_pos, T dmin_neg, T dp, T dn) {
     T y;
-    if (!min_face_ratio(dmax_pos, dmin_neg, dp, dn, y)) return T(1);
+    if (!min_face_ratio<T, LEAN>(dmax_pos, dmin_neg, dp, dn, y)) return T(1);
     if (y <= T(0)) return T(0);
     return y >= T(2) ? T(1) : y * (T(4) - y) * T(0.25);
 }

@@ -717,6 +717,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         o.recon = RECON_ETA;
     }
     using T2 = typename Vec2<T>::type;
+    // the lean FP64 reciprocal and quotient (zj_device.hpp) in the
+    // instantiations with the numerics fixed at compile time
+    constexpr bool LEAN = SPEC == 1 || SPEC == 3;
     // extra-stencil ids fetched per batch on triangle meshes (a vertex
     // stencil of up to 12 cells); the batch of the run-time numerics
     // (SPEC = 2) is smaller, for its register budget
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 if (i < nload) {
                     Qs[i] = a[u];
                     Qs[ML + i] = b[u];
-                    Hs[i] = hinv_of(a[u].y, eps);
+                    Hs[i] = hinv_of<T, LEAN>(a[u].y, eps);
                     T2 cl;
                     cl.x = T(cc[u].x - org.x);
                     cl.y = T(cc[u].y - org.y);
@@ -873,7 +876,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
                     ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
-                    to_w(o.recon, qg, hinv_of(qg[1], eps), wg);
+                    to_w(o.recon, qg, hinv_of<T, LEAN>(qg[1], eps), wg);
                     const T rx = T(2) * (fx[k] - cc.x), ry = T(2) * (fy[k] - cc.y);
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
@@ -981,8 +984,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv)
                     phi[vv] = o.lim == LIM_VANLEER
-                                  ? vl_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv])
-                                  : bj_phi(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv]);
+                                  ? vl_phi<T, LEAN>(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv])
+                                  : bj_phi<T, LEAN>(dpos[vv], dneg[vv], umax[vv] - w[vv], umin[vv] - w[vv]);
             } else if (o.lim != LIM_NONE) {
                 const double vol = cell_volume(m, v);
                 const T ev2 = T(m.dim == 1 ? vol * vol : fabs(vol));
@@ -1168,7 +1171,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     dpos = tmax(dpos, d);
                     dneg = tmin(dneg, d);
                 }
-                T phi = bj_phi(dpos, dneg, umax - w, umin - w);
+                T phi = bj_phi<T, LEAN>(dpos, dneg, umax - w, umin - w);
                 if (o1_mask && o1_mask[gi]) phi = T(0);
                 const T pv = clip01(phi);
                 T2 sv;
@@ -1257,7 +1260,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         // the fluxes' 1/h: this stage's state, or in a frozen second stage
         // the step state's (the face reconstruction keeps this stage's)
         const bool frz = SPEC != 1 && SPEC != 3 && o.stage_aux && mode == 1;
-        const T hif = frz ? hinv_of(q0[1], eps) : hic;
+        const T hif = frz ? hinv_of<T, LEAN>(q0[1], eps) : hic;
         T sc[8];
         if (use_slopes) {
 #pragma unroll
@@ -1289,7 +1292,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 lds_q4(Qs, ML, nl, qn);
                 hiR = Hs[nl];
                 hfR = hiR;
-                if (frz) hfR = hinv_of(Q0[(size_t)N + ((nl < nown && !ex) ? c0 + nl : lids[nl])], eps);
+                if (frz) hfR = hinv_of<T, LEAN>(Q0[(size_t)N + ((nl < nown && !ex) ? c0 + nl : lids[nl])], eps);
                 if (use_slopes) {
                     T sn[8];
 #pragma unroll
@@ -1308,7 +1311,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 ghost_state(m, Qin, bf, qL, nx, ny, qR);
             }
             T f[4];
-            face_flux_left(o.rs, qL, qR, hif, hfR, nx, ny, g, T(o.eps), f);
+            face_flux_left<T, LEAN>(o.rs, qL, qR, hif, hfR, nx, ny, g, T(o.eps), f);
             const T L = T(len);
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) acc[vv] += f[vv] * L;
@@ -1319,7 +1322,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 acc[3] += ce[3] * L;
             }
         }
-        const T ivol = frcp(T(cell_volume(m, v)));
+        const T ivol = frcp<LEAN>(T(cell_volume(m, v)));
         T out[4];
         if (mode == 2) {
 #pragma unroll
@@ -1334,7 +1337,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
             if (final_stage && o.manning > 0.0) {
                 const T h0 = mode == 1 ? q0[1] : q[1];
-                const T hs = hinv_of(h0, eps);
+                const T hs = hinv_of<T, LEAN>(h0, eps);
                 const T u = out[2] * hs, vv2 = out[3] * hs;
                 const T sp = sqrt(u * u + vv2 * vv2);
                 const T kk = T(o.g * o.manning * o.manning) * cbrt(hs);
@@ -1350,7 +1353,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 // min over faces of r_f / a_f without a division per face
                 // (in the stage precision; the products of the comparison
                 // are exact in FP64 for FP32 operands)
-                const T hn = hinv_of(out[1], eps);
+                const T hn = hinv_of<T, LEAN>(out[1], eps);
                 const T c = fsqrt(g * tmax(out[1], T(0)));
                 const T rc = Rs[ls];
                 ZJ_FOR_FACES(k) {
