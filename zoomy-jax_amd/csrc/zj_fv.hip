@@ -556,7 +556,7 @@ __device__ __forceinline__ void face_centres(const DevMesh& m, const double2* Vs
 // The edge vector is formed in FP64 from the global vertices, so the two
 // cells of a face (which run the edge in opposite directions) get exactly
 // negated normals and the same length.
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, int k, T& nx,
                                             T& ny, T& len) {
     if (m.dim == 1) {
@@ -572,6 +572,8 @@ __device__ __forceinline__ void face_normal(const DevMesh& m, const double2* v, 
     T il;
     if constexpr (sizeof(T) == 4)
         il = rsqrtf(l2);
+    else if constexpr (LEAN)
+        il = lean_rsqrt(l2);
     else
         il = rsqrt(l2);
     len = l2 * il;
@@ -613,14 +615,14 @@ __device__ __forceinline__ T pp_theta(T hbar, T sx, T sy, const T* fx, const T* 
 // system 1 (y component).  These helpers and face_increment spell out their
 // fused multiply-adds, so that every pass that computes a cell's slope (the
 // own-cell and the face-ring passes of the stage kernel) rounds it alike.
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ void lsq_inverse_rows(T Sxx, T Sxy, T Syy, const T* Gxx,
                                                  const T* Gxy, const T* Gyy, T& c0, T& c1,
                                                  T& c2, T& c3) {
     const T axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
-    const T di0 = T(1) / fma(axx, ayy, -(axy * axy));
+    const T di0 = frcp<LEAN>(fma(axx, ayy, -(axy * axy)));
     const T bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
-    const T di1 = T(1) / fma(bxx, byy, -(bxy * bxy));
+    const T di1 = frcp<LEAN>(fma(bxx, byy, -(bxy * bxy)));
     c0 = ayy * di0;
     c1 = -axy * di0;
     c2 = -bxy * di1;
@@ -871,7 +873,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     }
                 } else {
                     T nx, ny, len;
-                    face_normal(m, v, k, nx, ny, len);
+                    face_normal<T, LEAN>(m, v, k, nx, ny, len);
                     const int bf = -1 - m.nbr[k * N + gi];
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
             } else {
                 T c0, c1, c2, c3;
-                lsq_inverse_rows(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
+                lsq_inverse_rows<T, LEAN>(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
 #pragma unroll
                 for (int vv = 0; vv < 4; ++vv) {
                     lsq_solve(c0, c1, c2, c3, bx[vv] + gbx[0][vv], by[vv] + gby[0][vv],
@@ -1117,7 +1119,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         umin = tmin(umin, wn);
                     } else {
                         T nx, ny, len;
-                        face_normal(m, v, k, nx, ny, len);
+                        face_normal<T, LEAN>(m, v, k, nx, ny, len);
                         const int bf = -1 - m.nbr[k * N + gi];
                         const int kind = m.bf_kind[bf];
                         T q[4], qg[4];
@@ -1161,7 +1163,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     }
                 }
                 T c0, c1, c2, c3, gx, gy;
-                lsq_inverse_rows(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
+                lsq_inverse_rows<T, LEAN>(Sxx, Sxy, Syy, Gxx, Gxy, Gyy, c0, c1, c2, c3);
                 lsq_solve(c0, c1, c2, c3, bx + gbx[0], by + gby[0], bx + gbx[1], by + gby[1],
                           gx, gy);
                 T dpos = 0, dneg = 0;
@@ -1233,7 +1235,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 const int nb = m.nbr[(size_t)k * N + gi];
                 if (nb < 0) {
                     T nxd, nyd, len;
-                    face_normal(m, v, k, nxd, nyd, len);
+                    face_normal<T, LEAN>(m, v, k, nxd, nyd, len);
                     T qg[4];
                     ghost_state(m, Qin, -1 - nb, q, T(nxd), T(nyd), qg);
 #pragma unroll
@@ -1278,7 +1280,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         T acc[4] = {0, 0, 0, 0};
         ZJ_FOR_FACES(k) {
             T nxd, nyd, len;
-            face_normal(m, v, k, nxd, nyd, len);
+            face_normal<T, LEAN>(m, v, k, nxd, nyd, len);
             nxs[k] = nxd;
             nys[k] = nyd;
             const T nx = T(nxd), ny = T(nyd);
