@@ -788,7 +788,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 b[u].x = Qin[2 * N + gi];
                 b[u].y = Qin[3 * N + gi];
                 cc[u] = pt.cen2[gi];
-                rr[u] = final_stage ? m.rin[gi] : 0.0;
+                // inradii: the slope cells' (own + face ring) enter the dt law
+                rr[u] = (final_stage && i0 + u * SBLOCK < nslope) ? m.rin[gi] : 0.0;
                 vx[u] = m.verts[gv[u]];
             }
 #pragma unroll
@@ -797,7 +798,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 if (i < nload) {
                     Qs[i] = a[u];
                     Qs[ML + i] = b[u];
-                    Hs[i] = hinv_of<T, LEAN>(a[u].y, eps);
+                    // with conservative variables only the fluxes read 1/h:
+                    // own and face-ring (slope) cells
+                    if (!LEAN || i < nslope) Hs[i] = hinv_of<T, LEAN>(a[u].y, eps);
                     T2 cl;
                     cl.x = T(cc[u].x - org.x);
                     cl.y = T(cc[u].y - org.y);
@@ -1375,7 +1378,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // per-wave slot that k_reduce_part folds (deterministic, no contention).
     if (final_stage && mode != 2 && (troubled || dt_part)) {
         double loc_dt = best_r == INFINITY ? INFINITY
-                                           : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
+                                           : fdiv<LEAN>(o.cfl * 2.0 * best_r, (double)m.dim * best_a);
         if (loc_dt != loc_dt) loc_dt = 0.0;
         for (int off = 32; off > 0; off >>= 1) {
             bad += __shfl_xor(bad, off, 64);
@@ -1388,7 +1391,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     }
     if (ENS && final_stage && mode != 2 && ea.part2) {
         double loc_dt = best_r == INFINITY ? INFINITY
-                                           : o.cfl * 2.0 * best_r / ((double)m.dim * best_a);
+                                           : fdiv<LEAN>(o.cfl * 2.0 * best_r, (double)m.dim * best_a);
         if (loc_dt != loc_dt) loc_dt = 0.0;
         // own cells are a prefix of the patch: lane 0 of a wave with cells
         // has one, and names the wave's first instance
