@@ -119,3 +119,40 @@ def test_api_entry_points_agree_across_layouts(recon, kw):
     assert a.keys() == b.keys()
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_renumbered_march_keeps_the_callers_order(dtype):
+    """A strip-ordered mesh (RCM) with the automatic layout marches on its
+    renumbered twin (cells in the patch clusters' order) and hands the state
+    back in the caller's order: equal to the march on patches of
+    consecutive ids -- state, aux rows of the final state, time, dt and the
+    MOOD statistics -- bit for bit in FP64 (FP32: to 2e-5 of the scales)."""
+    from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
+    from zoomy_hip.fvm.solver import patch_plan
+    mesh, Q0 = rcm_wetdry_triangles(100, 50)
+    rc, st = patch_plan(mesh, 0, "auto")
+    assert rc == 0 and st["clustered"] == 1
+    kw = dict(reconstruction_variables="eta", mood=True, front_theta_tol=0.5)
+    mk = dict(order=2, limiter="vanleer", riemann="hll")
+    res = {}
+    for order in ("auto", "consecutive"):
+        s = HyperbolicSolver(time_end=np.inf, cfl=0.9, dtype=dtype,
+                             patch_order=order, **kw)
+        s.setup_simulation(mesh, ShallowWater(dimension=2, **mk))
+        Q = torch.as_tensor(Q0, dtype=dtype, device="cuda").contiguous()
+        Qaux = torch.zeros((7, mesh.n_cells), dtype=dtype, device="cuda")
+        _, t, dt = s.march_device(Q, 13, Qaux=Qaux)
+        res[order] = (Q.double().cpu().numpy(), Qaux.double().cpu().numpy(),
+                      float(t), float(dt), tuple(s.mood_stats()))
+        print(order, res[order][2:], flush=True)
+    a, b = res["auto"], res["consecutive"]
+    assert np.isfinite(a[0]).all() and a[0][1].min() >= 0.0
+    if dtype == torch.float64:
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert a[2:] == b[2:]
+    else:
+        for r in (1, 2, 3):
+            scale = max(np.abs(b[0][r]).max(), 1e-30)
+            assert np.abs(a[0][r] - b[0][r]).max() <= 2e-5 * scale, r
+        assert abs(a[2] - b[2]) <= 1e-5 * b[2]

@@ -322,6 +322,23 @@ __global__ __launch_bounds__(BLOCK) void k_dt(DevMesh m, Opts o, const T* __rest
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// rows x N state between the caller's cell order and a renumbered one:
+// scatter == 0: out[r, k] = in[r, ord[k]]; scatter != 0: out[r, ord[k]] =
+// in[r, k].
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_permute_rows(int N, int rows, const int* __restrict__ ord,
+                                                        const T* __restrict__ in,
+                                                        T* __restrict__ out, int scatter) {
+    const long i = (long)xcd_block() * BLOCK + threadIdx.x;
+    if (i >= (long)rows * N) return;
+    const int r = (int)(i / N), k = (int)(i - (long)r * N);
+    const size_t o = (size_t)r * N;
+    if (scatter)
+        out[o + ord[k]] = in[o + k];
+    else
+        out[o + k] = in[o + ord[k]];
+}
+
 __global__ void k_dt_init(unsigned long long* red, int* troubled) {
     if (red) *red = (unsigned long long)__double_as_longlong(INFINITY);
     if (troubled) *troubled = 0;
@@ -1437,6 +1454,13 @@ struct zj_mesh {
     int* fc = nullptr;          // (2, F) owner A, neighbour B (-1: boundary)
     int* fs = nullptr;          // (2, F) slot of the face in A, in B
     int* cell_face = nullptr;   // (mf, N) 2 f + side of each cell slot
+    // When the automatic layout cuts the patches from clusters of the Morton
+    // order (a strip-like cell order, such as RCM), the march runs on this
+    // mesh: the same cells renumbered in the cluster order, whose patches
+    // are consecutive ids (own cells implicit, own-cell loads and stores in
+    // runs).  ren_order[k] is the caller's id of renumbered cell k.
+    zj_mesh* ren = nullptr;
+    int* ren_order = nullptr;
 };
 
 // centres (2 words) + state and hinv (5 words) per loaded cell, inradius (1
@@ -1453,7 +1477,8 @@ static constexpr size_t LDS_LIMIT = 80 * 1024;
 // than 3x a patch's size of ring cells (orders whose id runs are thin
 // strips, such as RCM) and clusters do better, or when asked -- from
 // clusters of the Morton order of the cell centres.
-static int plan_patches(const zj_mesh_desc* d, zj::PatchHost& ph) {
+static int plan_patches(const zj_mesh_desc* d, zj::PatchHost& ph,
+                        std::vector<int32_t>* order_out = nullptr) {
     const int N = d->n_cells, mf = d->max_faces, mx = d->max_extra;
     std::vector<int32_t> ex_dummy(N, -1);
     const int32_t* exh = mx ? d->cell_extra : ex_dummy.data();
@@ -1484,6 +1509,7 @@ static int plan_patches(const zj_mesh_desc* d, zj::PatchHost& ph) {
                              (pc.P == ph.P && 5 * pc.max_load < 4 * ph.max_load))) {
             ph = std::move(pc);
             rc = ZJ_OK;
+            if (order_out) *order_out = order;
         } else if (d->patch_order == 2) {
             rc = rcc;
         }
@@ -1516,6 +1542,10 @@ struct zj_solver {
     // aux: that stage reads the neighbours' step-start depth from Q0, which
     // other workgroups of the same launch would be overwriting
     void* Qfrz = nullptr;
+    // the march on the mesh's renumbered twin (zj_mesh::ren): a solver of
+    // the same numerics and options on it, and the (4, N) state in its order
+    zj_solver* ren = nullptr;
+    void* Qp = nullptr;
 };
 
 namespace zj {
@@ -1588,6 +1618,49 @@ int zj_patch_plan(const zj_mesh_desc* d, int32_t* stats_out) {
     return ZJ_OK;
 }
 
+// The mesh of d with its cells renumbered: new cell k is the caller's cell
+// order[k]; neighbour and stencil ids, the boundary faces' inner cells and
+// the per-cell geometry follow, and the patches are consecutive ids.
+static int build_renumbered(const zj_mesh_desc* d, const std::vector<int32_t>& order,
+                            zj_mesh* m) {
+    const int N = d->n_cells, mf = d->max_faces, mx = d->max_extra;
+    std::vector<int32_t> inv(N);
+    for (int k = 0; k < N; ++k) inv[order[k]] = k;
+    std::vector<int32_t> cv((size_t)mf * N), nbr((size_t)mf * N), ex((size_t)mx * N);
+    std::vector<double> cen((size_t)2 * N), rin(N);
+    for (int k = 0; k < N; ++k) {
+        const int c = order[k];
+        for (int s = 0; s < mf; ++s) {
+            cv[(size_t)s * N + k] = d->cell_vertices[(size_t)s * N + c];
+            const int nb = d->cell_nbr[(size_t)s * N + c];
+            nbr[(size_t)s * N + k] = nb >= 0 ? inv[nb] : nb;
+        }
+        for (int j = 0; j < mx; ++j) {
+            const int e = d->cell_extra[(size_t)j * N + c];
+            ex[(size_t)j * N + k] = e >= 0 ? inv[e] : e;
+        }
+        cen[k] = d->cell_center[c];
+        cen[(size_t)N + k] = d->cell_center[(size_t)N + c];
+        rin[k] = d->cell_inradius[c];
+    }
+    std::vector<int32_t> bcell(d->n_bf > 0 ? d->n_bf : 1);
+    for (int b = 0; b < d->n_bf; ++b) bcell[b] = inv[d->bface_cell[b]];
+    zj_mesh_desc d2 = *d;
+    d2.cell_vertices = cv.data();
+    d2.cell_nbr = nbr.data();
+    d2.cell_extra = mx ? ex.data() : nullptr;
+    d2.cell_center = cen.data();
+    d2.cell_inradius = rin.data();
+    d2.bface_cell = bcell.data();
+    d2.patch_order = 1;  // consecutive ids: the clusters themselves
+    int rc = zj_mesh_create(&d2, &m->ren);
+    if (rc) return rc;
+    void* p = nullptr;
+    if (dev_upload(m, order.data(), sizeof(int32_t) * N, &p)) return ZJ_ENOMEM;
+    m->ren_order = (int*)p;
+    return ZJ_OK;
+}
+
 int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     if (!d || !out) return ZJ_EINVAL;
     if (d->dim < 1 || d->dim > 2 || d->n_cells <= 0 || d->max_faces < 2 ||
@@ -1630,6 +1703,7 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     }
     zj_mesh* m = new (std::nothrow) zj_mesh();
     if (!m) return ZJ_ENOMEM;
+    std::vector<int32_t> cluster_order;
     m->d.dim = d->dim; m->d.N = N; m->d.nverts = d->n_verts;
     m->d.mf = mf; m->d.mx = mx; m->d.nbf = d->n_bf;
     void* p;
@@ -1668,7 +1742,7 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
     {
         std::vector<int32_t> ex_dummy(N, -1);
         zj::PatchHost ph;
-        const int prc = plan_patches(d, ph);
+        const int prc = plan_patches(d, ph, &cluster_order);
         if (prc != ZJ_OK) {
             zj_mesh_destroy(m);
             return prc;
@@ -1741,12 +1815,19 @@ int zj_mesh_create(const zj_mesh_desc* d, zj_mesh** out) {
             return ZJ_EHIP;
         }
     }
+    if (d->patch_order == 0 && m->pt.expl && cluster_order.size() == (size_t)N) {
+        if (int rrc = build_renumbered(d, cluster_order, m)) {
+            zj_mesh_destroy(m);
+            return rrc;
+        }
+    }
     *out = m;
     return ZJ_OK;
 }
 
 int zj_mesh_destroy(zj_mesh* m) {
     if (!m) return ZJ_OK;
+    if (m->ren) zj_mesh_destroy(m->ren);
     for (void* p : m->allocs) (void)hipFree(p);
     for (void* p : {(void*)m->fc, (void*)m->fs, (void*)m->cell_face})
         if (p) (void)hipFree(p);
@@ -1785,6 +1866,14 @@ int zj_solver_create(zj_mesh* m, const zj_params* p, zj_solver** out) {
         zj_solver_destroy(s);
         return grc;
     }
+    if (m->ren) {
+        int rc = zj_solver_create(m->ren, p, &s->ren);
+        if (!rc && hipMalloc(&s->Qp, esz * 4 * N) != hipSuccess) rc = ZJ_ENOMEM;
+        if (rc) {
+            zj_solver_destroy(s);
+            return rc;
+        }
+    }
     *out = s;
     return ZJ_OK;
 }
@@ -1802,6 +1891,8 @@ int zj_solver_destroy(zj_solver* s) {
     if (s->mood_cnt) (void)hipFree(s->mood_cnt);
     if (s->mood_stats) (void)hipFree(s->mood_stats);
     if (s->Qfrz) (void)hipFree(s->Qfrz);
+    if (s->ren) zj_solver_destroy(s->ren);
+    if (s->Qp) (void)hipFree(s->Qp);
     delete s;
     return ZJ_OK;
 }
@@ -1989,6 +2080,8 @@ int zj_step(zj_solver* s, const double* dt, const void* Q, void* Qout, int32_t* 
 
 int zj_solver_set_mood(zj_solver* s, int32_t on) {
     if (!s) return ZJ_EINVAL;
+    if (s->ren)
+        if (int rc = zj_solver_set_mood(s->ren, on)) return rc;
     if (on && !s->Qc) {
         const size_t N = s->m->d.N;
         const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
@@ -2017,11 +2110,14 @@ int zj_solver_set_positivity(zj_solver* s, int32_t method, double front_tol) {
         return ZJ_EINVAL;
     s->o.pos = method;
     s->o.front_tol = front_tol > 0.0 ? front_tol : -1.0;
+    if (s->ren) return zj_solver_set_positivity(s->ren, method, front_tol);
     return ZJ_OK;
 }
 
 int zj_solver_set_stage_aux(zj_solver* s, int32_t mode) {
     if (!s || mode < ZJ_STAGE_AUX_REFRESH || mode > ZJ_STAGE_AUX_FROZEN) return ZJ_EINVAL;
+    if (s->ren)
+        if (int rc = zj_solver_set_stage_aux(s->ren, mode)) return rc;
     if (mode == ZJ_STAGE_AUX_FROZEN && !s->Qfrz) {
         const size_t esz = s->p.dtype == ZJ_F64 ? 8 : 4;
         if (hipMalloc(&s->Qfrz, esz * 4 * (size_t)s->m->d.N) != hipSuccess) {
@@ -2035,12 +2131,18 @@ int zj_solver_set_stage_aux(zj_solver* s, int32_t mode) {
 
 int zj_solver_mood_stats(zj_solver* s, int32_t* out2) {
     if (!s || !out2) return ZJ_EINVAL;
-    if (!s->mood_stats) {
-        out2[0] = out2[1] = 0;
-        return ZJ_OK;
+    out2[0] = out2[1] = 0;
+    if (s->mood_stats) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(out2, s->mood_stats, 2 * sizeof(int), hipMemcpyDeviceToHost));
     }
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpy(out2, s->mood_stats, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (s->ren) {
+        // the steps the march ran on the renumbered twin
+        int32_t r[2];
+        if (int rc = zj_solver_mood_stats(s->ren, r)) return rc;
+        out2[0] += r[0];
+        out2[1] += r[1];
+    }
     return ZJ_OK;
 }
 
@@ -2076,6 +2178,32 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     if (!s || !Q || !t_dev || !dt_dev || n_steps < 0) return ZJ_EINVAL;
     hipStream_t st = (hipStream_t)stream;
     const DevMesh& m = s->m->d;
+    if (s->ren) {
+        // the march on the renumbered twin: the state in its order on the way
+        // in, back in the caller's order on the way out; the aux rows of the
+        // final state from the caller's mesh
+        const int nb = nblocks(4 * m.N);
+        const int* ord = s->m->ren_order;
+        if (s->p.dtype == ZJ_F64)
+            hipLaunchKernelGGL(k_permute_rows<double>, dim3(nb), dim3(BLOCK), 0, st, m.N, 4, ord,
+                               (const double*)Q, (double*)s->Qp, 0);
+        else
+            hipLaunchKernelGGL(k_permute_rows<float>, dim3(nb), dim3(BLOCK), 0, st, m.N, 4, ord,
+                               (const float*)Q, (float*)s->Qp, 0);
+        HIPCHK(hipGetLastError());
+        int rc = zj_march(s->ren, s->Qp, nullptr, n_steps, t_end, t_dev, dt_dev, troubled_out,
+                          stream);
+        if (rc) return rc;
+        if (s->p.dtype == ZJ_F64)
+            hipLaunchKernelGGL(k_permute_rows<double>, dim3(nb), dim3(BLOCK), 0, st, m.N, 4, ord,
+                               (const double*)s->Qp, (double*)Q, 1);
+        else
+            hipLaunchKernelGGL(k_permute_rows<float>, dim3(nb), dim3(BLOCK), 0, st, m.N, 4, ord,
+                               (const float*)s->Qp, (float*)Q, 1);
+        HIPCHK(hipGetLastError());
+        if (Qaux && n_steps > 0) return zj_update_aux(s, Q, Qaux, stream);
+        return ZJ_OK;
+    }
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, s->dt_red, troubled_out);
     if (s->p.dtype == ZJ_F64)
         launch_dt<double>(s, (const double*)Q, st);
