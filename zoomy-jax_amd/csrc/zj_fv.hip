@@ -365,15 +365,25 @@ __global__ void k_dt_finalize_rem(unsigned long long* red, double* dt, double t_
 }
 
 // The march's per-step head: t += the previous step's dt (advance != 0),
-// then dt of this step as k_dt_finalize (one launch instead of two).
-__global__ void k_dt_next(unsigned long long* red, double* dt, double* t, double t_end,
-                          double dt_max, int advance) {
+// then dt of this step from the reduction scalar (capped by dt_max and
+// t_end - t; the scalar is reset), then the MOOD bookkeeping of the
+// previous step (stats += (1, troubled cells) when it had troubled cells;
+// prev != 0) and the troubled count cleared for this step's candidate.
+__global__ void k_step_head(unsigned long long* red, double* dt, double* t, double t_end,
+                            double dt_max, int advance, int* cnt, int* stats, int prev) {
     if (advance) *t += *dt;
     double d = __longlong_as_double((long long)*red);
     if (dt_max > 0.0) d = fmin(d, dt_max);
     if (t_end >= 0.0) d = fmin(d, t_end - *t);
     *dt = d;
     *red = (unsigned long long)__double_as_longlong(INFINITY);
+    if (cnt) {
+        if (prev && stats && *cnt != 0) {
+            stats[0] += 1;
+            stats[1] += *cnt;
+        }
+        *cnt = 0;
+    }
 }
 
 // Folds the per-wave dt minima of a stage launch into the reduction scalar.
@@ -1546,6 +1556,11 @@ struct zj_solver {
     // the same numerics and options on it, and the (4, N) state in its order
     zj_solver* ren = nullptr;
     void* Qp = nullptr;
+    // set by zj_march around its steps: the stage-2 launches leave their dt
+    // minima, which one fold at the end of the step reduces (with MOOD: the
+    // redo's when a redo ran -- it overwrites the candidate's -- so no reset
+    // is needed), and the MOOD bookkeeping moves into the next step's head
+    int defer_dt = 0;
 };
 
 namespace zj {
@@ -1949,7 +1964,7 @@ void launch_stage(zj_solver* s, const T* Qin, const T* Q0, T* Qout, const double
                        lds, st, M->d, M->pt, s->o, Qin, Q0, Qout, dt, aux, mode, final_stage,
                        write_b, use_slopes, troubled, dt_red ? s->dt_part : (double*)nullptr,
                        o1_mask, bad_out, gate, ro, ens ? *ens : EnsArgs{});
-    if (dt_red)
+    if (dt_red && !s->defer_dt)
         hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(M->pt.npatch * (SBLOCK / 64))),
                            dim3(1024), 0, st, (const double*)s->dt_part,
                            M->pt.npatch * (SBLOCK / 64), dt_red, gate);
@@ -2003,19 +2018,21 @@ int step_impl(zj_solver* s, const double* dt, const T* Q, T* Qout, int* troubled
     // kernels read the count and return at once when it is 0).
     const bool in_place = (const void*)Qout == (const void*)Q;
     T* Qc = in_place ? (T*)s->Qc : Qout;
-    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, (unsigned long long*)nullptr,
-                       s->mood_cnt);
+    if (!s->defer_dt)
+        hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, st, (unsigned long long*)nullptr,
+                           s->mood_cnt);
     launch_stage<T>(s, (const T*)Q1, Q, Qc, dt, (T*)nullptr, 1, 1, write_b, 1, s->mood_cnt, dt_red,
                     st, (const uint8_t*)nullptr, s->mood_mask);
-    if (dt_red)
+    if (dt_red && !s->defer_dt)
         hipLaunchKernelGGL(k_dt_reset_if, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt, dt_red);
     launch_stage<T>(s, Q, (const T*)nullptr, Q1, dt, (T*)nullptr, 0, 0, write_b, 1, (int*)nullptr,
                     (unsigned long long*)nullptr, st, s->mood_mask, (uint8_t*)nullptr,
                     s->mood_cnt);
     launch_stage<T>(s, (const T*)Q1, Q, Qc, dt, (T*)nullptr, 1, 1, write_b, 1, troubled, dt_red, st,
                     s->mood_mask, (uint8_t*)nullptr, s->mood_cnt);
-    hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt,
-                       s->mood_stats);
+    if (!s->defer_dt)
+        hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt,
+                           s->mood_stats);
     if (in_place)
         HIPCHK(hipMemcpyAsync(Qout + (write_b ? 0 : m.N), Qc + (write_b ? 0 : m.N),
                               sizeof(T) * (write_b ? 4 : 3) * m.N, hipMemcpyDeviceToDevice, st));
@@ -2224,11 +2241,15 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
     if (ping && !write_b && n_steps > 0)
         HIPCHK(hipMemcpyAsync(s->Qc, Q, esz * m.N, hipMemcpyDeviceToDevice, st));
     void* buf[2] = {Q, ping ? s->Qc : Q};
+    const bool mood2 = s->mood && s->o.order >= 2;
+    const int npart = s->m->pt.npatch * (SBLOCK / 64);
     for (int k = 0; k < n_steps; ++k) {
         void* in = buf[k & 1];
         void* out = buf[(k + 1) & 1];
-        hipLaunchKernelGGL(k_dt_next, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev, t_dev, t_end,
-                           s->p.dt_max, k > 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_step_head, dim3(1), dim3(1), 0, st, s->dt_red, dt_dev, t_dev, t_end,
+                           s->p.dt_max, k > 0 ? 1 : 0, mood2 ? s->mood_cnt : (int*)nullptr,
+                           s->mood_stats, k > 0 ? 1 : 0);
+        s->defer_dt = 1;
         int rc = s->p.dtype == ZJ_F64
                      ? step_impl(s, dt_dev, (const double*)in, (double*)out, troubled_out,
                                  s->dt_red, st, fuse && k > 0 ? (double*)Qaux : (double*)nullptr,
@@ -2236,12 +2257,21 @@ int zj_march(zj_solver* s, void* Q, void* Qaux, int32_t n_steps, double t_end, d
                      : step_impl(s, dt_dev, (const float*)in, (float*)out, troubled_out,
                                  s->dt_red, st, fuse && k > 0 ? (float*)Qaux : (float*)nullptr,
                                  k == 0, !ping);
+        s->defer_dt = 0;
         if (rc) return rc;
+        // the step's dt minima: the last stage-2 launch's (the redo's, when
+        // a MOOD redo ran), for the next step's head
+        hipLaunchKernelGGL(k_reduce_part, dim3(reduce_blocks(npart)), dim3(1024), 0, st,
+                           (const double*)s->dt_part, npart, s->dt_red, (const int*)nullptr);
         if (Qaux && !fuse) {
             rc = zj_update_aux(s, out, Qaux, stream);
             if (rc) return rc;
         }
     }
+    // the MOOD bookkeeping of the last step
+    if (mood2 && n_steps > 0)
+        hipLaunchKernelGGL(k_mood_finish, dim3(1), dim3(1), 0, st, (const int*)s->mood_cnt,
+                           s->mood_stats);
     if (ping && (n_steps & 1))
         HIPCHK(hipMemcpyAsync((char*)Q + (write_b ? 0 : esz * m.N),
                               (const char*)s->Qc + (write_b ? 0 : esz * m.N),
