@@ -41,61 +41,79 @@ constexpr int RED_MAX = 1024;
 __device__ __forceinline__ bool is_done(const int* done) { return done && *done; }
 
 // ───────────────────────── hierarchy: averages and stencils ─────────────────
-__global__ __launch_bounds__(CB) void k_hb_avg(int nxc, int nyc, const double* __restrict__ hf,
-                                               const double* __restrict__ bf,
-                                               double* __restrict__ hc, double* __restrict__ bc) {
-    int c = blockIdx.x * CB + threadIdx.x;
-    if (c >= nxc * nyc) return;
-    int I = c / nyc, J = c - I * nyc;
-    int nyf = 2 * nyc;
-    int f00 = (2 * I) * nyf + 2 * J;
-    int f10 = f00 + nyf;
-    hc[c] = 0.25 * (((hf[f00] + hf[f10]) + hf[f00 + 1]) + hf[f10 + 1]);
-    bc[c] = 0.25 * (((bf[f00] + bf[f10]) + bf[f00 + 1]) + bf[f10 + 1]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(CB) void k_coef(int nx, int ny, double dx, double dy, double eps,
-                                             const double* __restrict__ h,
-                                             const double* __restrict__ b, T* __restrict__ diag,
-                                             T* __restrict__ cE, T* __restrict__ cN) {
+// One level of the hierarchy.  On level 0 (avg == 0) h and b are the
+// level's own depth and bed; on coarser levels (avg != 0) they are the finer
+// level's, and each cell forms the 4-cell averages 0.25 (((f00 + f10) + f01)
+// + f11) of itself and of its four neighbours (the same sums, so the same
+// values, as a separate averaging pass would store), storing its own in
+// (hc, bc) for the next level.  The stencil coefficients (diag, cE, cN) go
+// to the FP64 arrays and, when given, rounded to the FP32 arrays of the
+// low-precision hierarchy.
+__global__ __launch_bounds__(CB) void k_level(int nx, int ny, double dx, double dy, double eps,
+                                              const double* __restrict__ h,
+                                              const double* __restrict__ b, int avg,
+                                              double* __restrict__ hc_out,
+                                              double* __restrict__ bc_out,
+                                              double* __restrict__ diag, double* __restrict__ cE,
+                                              double* __restrict__ cN, float* __restrict__ diag32,
+                                              float* __restrict__ cE32, float* __restrict__ cN32) {
     int c = blockIdx.x * CB + threadIdx.x;
     if (c >= nx * ny) return;
     int i = c / ny, j = c - i * ny;
-    double hc = h[c], bc = b[c];
+    const int nyf = 2 * ny;
+    auto val = [&](const double* __restrict__ a, int q) {
+        if (!avg) return a[q];
+        const int I = q / ny, J = q - I * ny;
+        const int f00 = (2 * I) * nyf + 2 * J, f10 = f00 + nyf;
+        return 0.25 * (((a[f00] + a[f10]) + a[f00 + 1]) + a[f10 + 1]);
+    };
+    double hc = val(h, c), bc = val(b, c);
+    if (avg) {
+        hc_out[c] = hc;
+        bc_out[c] = bc;
+    }
     double d = 4.0 * dx * dy * hinv_of(hc, eps);
     double e = 0.0, n = 0.0;
     const double rx = dy / dx, ry = dx / dy;
     // faces to the four neighbours: (offset, ratio |f|/d_f, is the E/N face)
     if (i + 1 < nx) {
         int o = c + ny;
-        double cf = rx * hinv_of(0.5 * (hc + h[o]), eps);
-        double ac = hc + bc - b[o], ae = h[o] + b[o] - bc;
+        const double ho = val(h, o), bo = val(b, o);
+        double cf = rx * hinv_of(0.5 * (hc + ho), eps);
+        double ac = hc + bc - bo, ae = ho + bo - bc;
         d += cf * ac * ac;
         e = -cf * ac * ae;
     }
     if (i > 0) {
         int o = c - ny;
-        double cf = rx * hinv_of(0.5 * (h[o] + hc), eps);
-        double ac = hc + bc - b[o];
+        const double ho = val(h, o), bo = val(b, o);
+        double cf = rx * hinv_of(0.5 * (ho + hc), eps);
+        double ac = hc + bc - bo;
         d += cf * ac * ac;
     }
     if (j + 1 < ny) {
         int o = c + 1;
-        double cf = ry * hinv_of(0.5 * (hc + h[o]), eps);
-        double ac = hc + bc - b[o], ae = h[o] + b[o] - bc;
+        const double ho = val(h, o), bo = val(b, o);
+        double cf = ry * hinv_of(0.5 * (hc + ho), eps);
+        double ac = hc + bc - bo, ae = ho + bo - bc;
         d += cf * ac * ac;
         n = -cf * ac * ae;
     }
     if (j > 0) {
         int o = c - 1;
-        double cf = ry * hinv_of(0.5 * (h[o] + hc), eps);
-        double ac = hc + bc - b[o];
+        const double ho = val(h, o), bo = val(b, o);
+        double cf = ry * hinv_of(0.5 * (ho + hc), eps);
+        double ac = hc + bc - bo;
         d += cf * ac * ac;
     }
-    diag[c] = (T)d;
-    cE[c] = (T)e;
-    cN[c] = (T)n;
+    diag[c] = d;
+    cE[c] = e;
+    cN[c] = n;
+    if (diag32) {
+        diag32[c] = (float)d;
+        cE32[c] = (float)e;
+        cN32[c] = (float)n;
+    }
 }
 
 // sum_{e != c} S_ce x_e
@@ -1021,23 +1039,21 @@ void Hier::release() {
     mem = nullptr;
 }
 
-template <typename T>
-static void build_levels(Hier& H, const double* h0, const double* b0, double eps, hipStream_t st) {
+// The FP64 hierarchy H from (h0, b0), and with H32 the FP32 coefficients of
+// the same levels: one k_level launch per level.
+static void build_levels(Hier& H, Hier* H32, const double* h0, const double* b0, double eps,
+                         hipStream_t st) {
     for (size_t l = 0; l < H.levels.size(); ++l) {
         HLevel& L = H.levels[l];
         const int n = L.nx * L.ny;
-        const double* h = h0;
-        const double* b = b0;
-        if (l) {
-            const HLevel& F = H.levels[l - 1];
-            const double* hf = l == 1 ? h0 : F.h;
-            const double* bf = l == 1 ? b0 : F.b;
-            k_hb_avg<<<nblk(n), CB, 0, st>>>(L.nx, L.ny, hf, bf, L.h, L.b);
-            h = L.h;
-            b = L.b;
-        }
-        k_coef<T><<<nblk(n), CB, 0, st>>>(L.nx, L.ny, L.dx, L.dy, eps, h, b, (T*)L.diag,
-                                          (T*)L.cE, (T*)L.cN);
+        const double* hf = l == 0 ? h0 : (l == 1 ? h0 : H.levels[l - 1].h);
+        const double* bf = l == 0 ? b0 : (l == 1 ? b0 : H.levels[l - 1].b);
+        HLevel* S = H32 ? &H32->levels[l] : nullptr;
+        k_level<<<nblk(n), CB, 0, st>>>(L.nx, L.ny, L.dx, L.dy, eps, hf, bf, l ? 1 : 0, L.h, L.b,
+                                         (double*)L.diag, (double*)L.cE, (double*)L.cN,
+                                         S ? (float*)S->diag : nullptr,
+                                         S ? (float*)S->cE : nullptr,
+                                         S ? (float*)S->cN : nullptr);
     }
 }
 
@@ -1196,7 +1212,7 @@ struct zj_chorin {
     double* part = nullptr;  // reduction partials (4 * RED_MAX)
     double* sc = nullptr;    // CG scalars
     int* fl = nullptr;       // {done, iterations}
-    int* host_fl = nullptr;  // pinned copy of fl
+    int* host_fl = nullptr;  // pinned copy of fl (inside host_sc)
     double* host_sc = nullptr;
     zj_mpnn* net = nullptr;  // learned preconditioner (may be NULL)
     // iterations of the last solve per preconditioner (ZJ_PRECOND_*): the
@@ -1219,9 +1235,9 @@ static int dalloc(zj_chorin* c, T** p, size_t n) {
 }
 
 static int read_flags(zj_chorin* c, hipStream_t st) {
-    HIPCHK(hipMemcpyAsync(c->host_fl, c->fl, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(c->host_sc, c->sc, CG_NSLOT * sizeof(double), hipMemcpyDeviceToHost,
-                          st));
+    // the scalars and the flags are one block: one copy
+    HIPCHK(hipMemcpyAsync(c->host_sc, c->sc, (CG_NSLOT + 2) * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return ZJ_OK;
 }
@@ -1389,8 +1405,8 @@ static int solve(zj_chorin* c, const double* rhs, double* p, int precond, hipStr
 
 static int setup_operator(zj_chorin* c, const double* b, const double* h, hipStream_t st) {
     zj::mpnn_invalidate_phi(c->net);
-    zj::build_levels<double>(c->H64, h, b, c->eps, st);
-    if (c->prm.precond != ZJ_PRECOND_MG) zj::build_levels<float>(c->H32, h, b, c->eps, st);
+    zj::build_levels(c->H64, c->prm.precond != ZJ_PRECOND_MG ? &c->H32 : nullptr, h, b, c->eps,
+                     st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
@@ -1468,11 +1484,13 @@ int zj_chorin_create(zj_solver* fv, int32_t nx, int32_t ny, double dx, double dy
     if (!rc) rc = dalloc(c, &c->z32, N);
     if (!rc) rc = dalloc(c, &c->r_old, N);
     if (!rc) rc = dalloc(c, &c->part, 4 * RED_MAX);
-    if (!rc) rc = dalloc(c, &c->sc, CG_NSLOT);
-    if (!rc) rc = dalloc(c, &c->fl, 4);
-    if (!rc && hipHostMalloc((void**)&c->host_fl, 4 * sizeof(int)) != hipSuccess) rc = ZJ_ENOMEM;
-    if (!rc && hipHostMalloc((void**)&c->host_sc, CG_NSLOT * sizeof(double)) != hipSuccess)
+    // CG scalars and the flags {done, iterations} in one block (the host
+    // reads both with one copy)
+    if (!rc) rc = dalloc(c, &c->sc, CG_NSLOT + 2);
+    if (!rc) c->fl = (int*)(c->sc + CG_NSLOT);
+    if (!rc && hipHostMalloc((void**)&c->host_sc, (CG_NSLOT + 2) * sizeof(double)) != hipSuccess)
         rc = ZJ_ENOMEM;
+    if (!rc) c->host_fl = (int*)(c->host_sc + CG_NSLOT);
     if (rc) {
         zj_chorin_destroy(c);
         return rc;
@@ -1487,7 +1505,6 @@ int zj_chorin_destroy(zj_chorin* c) {
     for (void* p : c->allocs) (void)hipFree(p);
     c->H64.release();
     c->H32.release();
-    if (c->host_fl) (void)hipHostFree(c->host_fl);
     if (c->host_sc) (void)hipHostFree(c->host_sc);
     delete c;
     return ZJ_OK;
