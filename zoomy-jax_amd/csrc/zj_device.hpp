@@ -60,6 +60,22 @@ __device__ __forceinline__ double lean_sqrt(double x) {
     d = fma(-g, g, x);
     return fma(d, h, g);
 }
+// The same iteration for operands known to be >= 0 (wave speeds, depths),
+// branch-free: 0 (and NaN) pass through; positive operands below 2^-767,
+// depths of no physical meaning, lose the last bits instead of taking the
+// scaled path.
+__device__ __forceinline__ double lean_sqrt_nonneg(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x > 0.0 ? g : x;
+}
 __device__ __forceinline__ double lean_rcp(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
@@ -96,6 +112,14 @@ template <bool LEAN = false> __device__ __forceinline__ float frcp(float x) {
 }
 __device__ __forceinline__ double fsqrt(double x) { return lean_sqrt(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// square root of an operand >= 0: with LEAN branch-free (lean_sqrt_nonneg)
+template <bool LEAN = false> __device__ __forceinline__ double fsqrt_nn(double x) {
+    if constexpr (LEAN) return lean_sqrt_nonneg(x);
+    else return lean_sqrt(x);
+}
+template <bool LEAN = false> __device__ __forceinline__ float fsqrt_nn(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+}
 template <bool LEAN = false> __device__ __forceinline__ double fdiv(double a, double b) {
     if constexpr (LEAN) return lean_div(a, b);
     else return a / b;
@@ -181,7 +205,7 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
         T uL = qL[2] * hiL, vL = qL[3] * hiL;
         T uR = qR[2] * hiR, vR = qR[3] * hiR;
         T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
-        T cL = fsqrt(g * hLs), cR = fsqrt(g * hRs);
+        T cL = fsqrt_nn<LEAN>(g * hLs), cR = fsqrt_nn<LEAN>(g * hRs);
         T sL = tmin(unL - cL, unR - cR);
         T sR = tmax(unL + cL, unR + cR);
         T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;

@@ -1386,7 +1386,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 // (in the stage precision; the products of the comparison
                 // are exact in FP64 for FP32 operands)
                 const T hn = hinv_of<T, LEAN>(out[1], eps);
-                const T c = fsqrt(g * tmax(out[1], T(0)));
+                const T c = fsqrt_nn<LEAN>(g * tmax(out[1], T(0)));
                 const T rc = Rs[ls];
                 ZJ_FOR_FACES(k) {
                     const uint16_t nl = nls[k];
