@@ -142,9 +142,9 @@ def test_reference_long_o2_goldens(golden, name, n, L, t_end, ic, riemann):
 @pytest.mark.parametrize("recon", ["conservative", "xz", "eta", "eta_hu"])
 @pytest.mark.parametrize("kind", ["quad", "tri", "line"])
 def test_post_step_aux_every_reconstruction(recon, kind):
-    """post_step's aux rows (the patch kernel's gradient pass for the
-    conservative reconstructions, the per-cell kernel for the eta ones)
-    against the oracle."""
+    """post_step's aux rows (the patch kernel's gradient pass, in
+    conservative variables whatever the reconstruction) against the
+    oracle."""
     from zoomy_hip.mesh import Mesh
     if kind == "quad":
         mesh = Mesh.quad(37, 29).set_bc({"left": "wall", "right": "extrapolation",

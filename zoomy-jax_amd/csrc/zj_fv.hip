@@ -12,9 +12,9 @@
 // recomputed from the staged vertices instead of streamed from HBM.  An
 // SSP-RK2 step is two launches (stage 2 adds the RK average, the explicit
 // source, the troubled flag and the per-wave dt minima) and one small
-// reduction.  Separate kernels: the post_step aux refresh (k_aux), the dt
-// law (k_dt), the compensated mass (k_mass_*) and the local implicit source
-// (k_imex).
+// reduction.  The post_step aux refresh is the kernel's aux-only
+// instantiation; separate kernels: the dt law (k_dt), the compensated mass
+// (k_mass_*) and the local implicit source (k_imex).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -180,99 +180,6 @@ __device__ __forceinline__ void recon_face(int recon, const T* q, T hi, const T*
 #pragma unroll
         for (int v = 0; v < 4; ++v) out[v] = q[v] + s[2 * v] * rx + s[2 * v + 1] * ry;
     }
-}
-
-// ─────────────────────────────────────────────────────────────────────────
-// aux: post_step derivative rows (LSQ gradient of the conservative state
-// with zero ghost deltas; the ghost rows stay in the normal matrices) and
-// hinv.  Rows per direction d: [d q_d/dx_d, d h/dx_d], then d b/dx_d, then
-// hinv (solver_jax.py:284-337).
-// ─────────────────────────────────────────────────────────────────────────
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_aux(DevMesh m, Opts o, const T* __restrict__ Q,
-                                               T* __restrict__ Qaux) {
-    const int c = xcd_block() * BLOCK + threadIdx.x;
-    if (c >= m.N) return;
-    const int N = m.N;
-    const T eps = T(o.eps);
-    T q[4];
-    load_q(Q, N, c, q);
-    const T hic = hinv_of(q[1], eps);
-    // the face geometry enters only through the ghost rows: interior cells
-    // read their centre alone
-    bool bnd = false;
-    ZJ_FOR_FACES(k) bnd |= m.nbr[k * N + c] < 0;
-    Geo G;
-    if (bnd) {
-        load_geo(m, c, G);
-    } else {
-        G.cx = m.cen[c];
-        G.cy = m.cen[N + c];
-    }
-    double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
-    double Gxx[2] = {0.0, 0.0}, Gxy[2] = {0.0, 0.0}, Gyy[2] = {0.0, 0.0};
-    T bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0};
-    ZJ_FOR_FACES(k) {
-        const int nb = m.nbr[k * N + c];
-        if (nb >= 0) {
-            T qn[4];
-            load_q(Q, N, nb, qn);
-            const double rx = m.cen[nb] - G.cx, ry = m.cen[N + nb] - G.cy;
-            Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                T dv = qn[v] - q[v];
-                bx[v] += T(rx) * dv;
-                by[v] += T(ry) * dv;
-            }
-        } else {
-            const double rx = 2.0 * (G.fx[k] - G.cx), ry = 2.0 * (G.fy[k] - G.cy);
-            const double an[2] = {fabs(G.nx[k]), fabs(G.ny[k])};
-#pragma unroll
-            for (int d = 0; d < 2; ++d)
-                if (an[d] > 0.5) { Gxx[d] += rx * rx; Gxy[d] += rx * ry; Gyy[d] += ry * ry; }
-        }
-    }
-    for (int j = 0; j < m.mx; ++j) {
-        const int e = m.extra[j * N + c];
-        if (e < 0) break;
-        T qn[4];
-        load_q(Q, N, e, qn);
-        const double rx = m.cen[e] - G.cx, ry = m.cen[N + e] - G.cy;
-        Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            T dv = qn[v] - q[v];
-            bx[v] += T(rx) * dv;
-            by[v] += T(ry) * dv;
-        }
-    }
-    T ax[4], ay[4];
-    if (m.dim == 1) {
-        const double inv = 1.0 / (Sxx + Gxx[0]);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) { ax[v] = T(inv) * bx[v]; ay[v] = T(0); }
-    } else {
-        // component x: normal matrix with the x-ghost rows only (and y alike)
-        const double axx = Sxx + Gxx[0], axy = Sxy + Gxy[0], ayy = Syy + Gyy[0];
-        const double dinv0 = 1.0 / (axx * ayy - axy * axy);
-        const double bxx = Sxx + Gxx[1], bxy = Sxy + Gxy[1], byy = Syy + Gyy[1];
-        const double dinv1 = 1.0 / (bxx * byy - bxy * bxy);
-        const T c0 = T(ayy * dinv0), c1 = T(-axy * dinv0);
-        const T c2 = T(-bxy * dinv1), c3 = T(bxx * dinv1);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            ax[v] = c0 * bx[v] + c1 * by[v];
-            ay[v] = c2 * bx[v] + c3 * by[v];
-        }
-    }
-    int row = 0;
-    for (int d = 0; d < m.dim; ++d) {
-        Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[2] : ay[3];
-        Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[1] : ay[1];
-    }
-    for (int d = 0; d < m.dim; ++d) Qaux[(size_t)(row++) * N + c] = d == 0 ? ax[0] : ay[0];
-    Qaux[(size_t)row * N + c] = hic;
 }
 
 // Per (slot, cell): the outward unit normal and r_f = min(r_cell, r_nbr)
@@ -745,6 +652,9 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         o.rs = RS_HLL;
         o.recon = RECON_ETA;
     }
+    // the post_step aux rows are gradients of the conservative state, whatever
+    // the reconstruction variables (solver_jax.py:284-337)
+    if (AUXONLY == 1) o.recon = RECON_CONS;
     using T2 = typename Vec2<T>::type;
     // the lean FP64 reciprocal and quotient (zj_device.hpp) in the
     // instantiations with the numerics fixed at compile time
@@ -2166,26 +2076,18 @@ int zj_solver_mood_stats(zj_solver* s, int32_t* out2) {
 int zj_update_aux(zj_solver* s, const void* Q, void* Qaux, void* stream) {
     if (!s || !Q || !Qaux) return ZJ_EINVAL;
     hipStream_t st = (hipStream_t)stream;
-    const DevMesh& m = s->m->d;
     const int write_b = s->o.rs == RS_RUSANOV_NC ? 1 : 0;
-    if (s->o.recon == RECON_CONS || s->o.recon == RECON_XZ) {
-        // the patch kernel's gradient pass alone (mode 3): staged neighbours
-        // instead of per-cell gathers
-        if (s->p.dtype == ZJ_F64)
-            launch_stage<double>(s, (const double*)Q, (const double*)nullptr, (double*)nullptr,
-                                 (const double*)nullptr, (double*)Qaux, 3, 0, write_b, 1,
-                                 (int*)nullptr, (unsigned long long*)nullptr, st);
-        else
-            launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)nullptr,
-                                (const double*)nullptr, (float*)Qaux, 3, 0, write_b, 1,
-                                (int*)nullptr, (unsigned long long*)nullptr, st);
-    } else if (s->p.dtype == ZJ_F64) {
-        hipLaunchKernelGGL(k_aux<double>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const double*)Q, (double*)Qaux);
-    } else {
-        hipLaunchKernelGGL(k_aux<float>, dim3(nblocks(m.N)), dim3(BLOCK), 0, st, m, s->o,
-                           (const float*)Q, (float*)Qaux);
-    }
+    // the patch kernel's gradient pass alone (mode 3): staged neighbours
+    // instead of per-cell gathers, conservative variables for every
+    // reconstruction
+    if (s->p.dtype == ZJ_F64)
+        launch_stage<double>(s, (const double*)Q, (const double*)nullptr, (double*)nullptr,
+                             (const double*)nullptr, (double*)Qaux, 3, 0, write_b, 1,
+                             (int*)nullptr, (unsigned long long*)nullptr, st);
+    else
+        launch_stage<float>(s, (const float*)Q, (const float*)nullptr, (float*)nullptr,
+                            (const double*)nullptr, (float*)Qaux, 3, 0, write_b, 1,
+                            (int*)nullptr, (unsigned long long*)nullptr, st);
     HIPCHK(hipGetLastError());
     return ZJ_OK;
 }
