@@ -342,9 +342,11 @@ def imex_global_leg(mesh, Q0, reps=3):
     0.03) and the depth-gradient source -alpha g h dh/dx_d (alpha = 0.5),
     which reads the LSQ derivative aux rows; seeded velocities of 0.5 so
     that the friction acts; Newton to 1e-8, GMRES(20) to 1e-7, FP64.  A
-    GMRES iteration is one exact Jacobian-vector product (a patch-kernel
-    gradient pass of v and the source linearisation) plus the classical
-    Gram-Schmidt work."""
+    GMRES iteration is one exact Jacobian-vector product (the source
+    linearisation on the momentum rows) plus the classical Gram-Schmidt
+    work; us_per_gmres_iteration is the whole stage's time over its GMRES
+    iterations (the aux refresh, the residuals and the Newton updates
+    included)."""
     import numpy as np
     import torch
     from zoomy_hip.fvm import IMEXSourceSolver, ShallowWater

@@ -17,8 +17,13 @@
 // Krylov vector live in the dim momentum rows: v_h = 0, hence the aux(v)
 // rows the source reads (dh/dx_d) vanish and aux(Q) is the same for every
 // Newton step -- it is refreshed once per solve.  GMRES is restarted
-// GMRES(m) with classical Gram-Schmidt and one reorthogonalisation, Givens
-// rotations on the host.  Every reduction stores one partial per block and
+// GMRES(m) with classical Gram-Schmidt and one reorthogonalisation.  The
+// dot products, the Hessenberg columns, the Givens rotations, the
+// back-substitution and the convergence test stay on the device (one-thread
+// kernels on a GmresDev block); the host enqueues Arnoldi iterations in
+// chunks (the first as long as the previous solve took) and reads the
+// block's flags once per chunk, and the Arnoldi kernels of a chunk return at
+// once after convergence.  Every reduction stores one partial per block and
 // a fold kernel sums them in a fixed order, so the stage is deterministic.
 #include <hip/hip_runtime.h>
 
@@ -122,7 +127,9 @@ __global__ __launch_bounds__(IB) void k_src_residual(SrcPrm p, const double* __r
 __global__ __launch_bounds__(IB) void k_src_jvp(SrcPrm p, const double* __restrict__ Q,
                                                 const double* __restrict__ kf,
                                                 const double* __restrict__ V,
-                                                double* __restrict__ out) {
+                                                double* __restrict__ out,
+                                                const int* __restrict__ stop) {
+    if (stop && *stop) return;
     const int N = p.N;
     for (int c = blockIdx.x * IB + threadIdx.x; c < N; c += gridDim.x * IB) {
         double q[4] = {0.0, 0.0, 0.0, 0.0}, v[2] = {0.0, 0.0};
@@ -148,7 +155,9 @@ struct Coefs {
 // part[i * gridDim + block] = <w, v_i> over the block's entries, i <= k
 __global__ __launch_bounds__(IB) void k_mdot(int n, int k, const double* __restrict__ w,
                                              const double* __restrict__ Vb,
-                                             double* __restrict__ part) {
+                                             double* __restrict__ part,
+                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
     __shared__ double lds[IB / 64];
     for (int i = 0; i <= k; ++i) {
         const double* vi = Vb + (size_t)i * n;
@@ -161,7 +170,9 @@ __global__ __launch_bounds__(IB) void k_mdot(int n, int k, const double* __restr
 
 // out[i] = sum of the nb partials of dot i, in a fixed order
 __global__ __launch_bounds__(IB) void k_fold(int nb, const double* __restrict__ part,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ out,
+                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
     __shared__ double lds[IB / 64];
     const double* p = part + (size_t)blockIdx.x * nb;
     double acc = 0.0;
@@ -186,6 +197,101 @@ __global__ __launch_bounds__(IB) void k_scale(int n, double a, const double* __r
     for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) y[j] = a * x[j];
 }
 
+// GMRES state on the device (one restart cycle): the Hessenberg columns
+// H[i * MAXK + k], the rotations, the rotated right-hand side e, the
+// solution coefficients y, the two Gram-Schmidt passes' dot products d1, d2
+// and the norm dot dn; stop ends the Arnoldi kernels of the cycle, k counts
+// its iterations, its every iteration of the solve, conv marks convergence.
+struct GmresDev {
+    double H[(MAXK + 1) * MAXK];
+    double cs[MAXK], sn[MAXK], e[MAXK + 1], y[MAXK];
+    double d1[MAXK + 1], d2[MAXK + 1];
+    double dn, hn, beta, target;
+    int stop, k, its, conv;
+};
+
+// w += sign sum_i coef_i v_i over m columns (m from *mdev when given)
+__global__ __launch_bounds__(IB) void k_maxpy_dev(int n, int m, const int* __restrict__ mdev,
+                                                  const double* __restrict__ coef, double sign,
+                                                  const double* __restrict__ Vb,
+                                                  double* __restrict__ w,
+                                                  const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    if (mdev) m = *mdev;
+    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) {
+        double x = w[j];
+        for (int i = 0; i < m; ++i) x += (sign * coef[i]) * Vb[(size_t)i * n + j];
+        w[j] = x;
+    }
+}
+
+// x *= 1 / s with s = beta (which == 0) or hn (which == 1), unless stopped
+// or s == 0
+__global__ __launch_bounds__(IB) void k_gm_unit(int n, const GmresDev* __restrict__ G, int which,
+                                                double* __restrict__ x) {
+    if (G->stop) return;
+    const double sv = which ? G->hn : G->beta;
+    if (!(sv > 0.0)) return;
+    const double a = 1.0 / sv;
+    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) x[j] = a * x[j];
+}
+
+// Start of a cycle: beta = |r| from dn, the target of the solve on the first
+// cycle (tol |b|), e = (beta, 0, ...); converged (stop) when |r| <= target
+// (first cycle: when b = 0).
+__global__ void k_gm_cycle(GmresDev* G, double tol, int first) {
+    const double beta = sqrt(fmax(G->dn, 0.0));
+    G->beta = beta;
+    if (first) {
+        G->target = tol * beta;
+        G->its = 0;
+    }
+    for (int i = 0; i <= MAXK; ++i) G->e[i] = 0.0;
+    G->e[0] = beta;
+    G->k = 0;
+    G->conv = first ? beta == 0.0 : beta <= G->target;
+    G->stop = G->conv;
+}
+
+// Arnoldi iteration k, after both Gram-Schmidt passes and the norm dot: the
+// Hessenberg column (h = d1 + d2, h_{k+1} = |v_{k+1}|), the previous
+// rotations applied, a new one, e rotated; converged when |e_{k+1}| <=
+// target, stop then or on breakdown (|v_{k+1}| = 0).
+__global__ void k_gm_givens(GmresDev* G, int k) {
+    if (G->stop) return;
+    double h[MAXK + 1];
+    for (int i = 0; i <= k; ++i) h[i] = G->d1[i] + G->d2[i];
+    const double hn = sqrt(fmax(G->dn, 0.0));
+    G->hn = hn;
+    h[k + 1] = hn;
+    for (int i = 0; i < k; ++i) {
+        const double t = G->cs[i] * h[i] + G->sn[i] * h[i + 1];
+        h[i + 1] = -G->sn[i] * h[i] + G->cs[i] * h[i + 1];
+        h[i] = t;
+    }
+    const double r = hypot(h[k], h[k + 1]);
+    G->cs[k] = r > 0.0 ? h[k] / r : 1.0;
+    G->sn[k] = r > 0.0 ? h[k + 1] / r : 0.0;
+    h[k] = r;
+    G->e[k + 1] = -G->sn[k] * G->e[k];
+    G->e[k] = G->cs[k] * G->e[k];
+    for (int i = 0; i <= k; ++i) G->H[(size_t)i * MAXK + k] = h[i];
+    G->k = k + 1;
+    G->its += 1;
+    G->conv = fabs(G->e[k + 1]) <= G->target;
+    if (G->conv || hn == 0.0) G->stop = 1;
+}
+
+// End of a cycle: y = H^-1 e over its k iterations (upper triangular).
+__global__ void k_gm_finish(GmresDev* G) {
+    const int k = G->k;
+    for (int i = k - 1; i >= 0; --i) {
+        double sv = G->e[i];
+        for (int j = i + 1; j < k; ++j) sv -= G->H[(size_t)i * MAXK + j] * G->y[j];
+        G->y[i] = sv / G->H[(size_t)i * MAXK + i];
+    }
+}
+
 }  // namespace
 
 struct zj_imex_global {
@@ -194,144 +300,101 @@ struct zj_imex_global {
     double alpha = 0.0;
     double *aux = nullptr, *kf = nullptr, *R = nullptr, *W = nullptr, *Vb = nullptr,
            *dX = nullptr, *part = nullptr, *dots = nullptr, *host = nullptr;
+    GmresDev* gd = nullptr;       // the GMRES block on the device
+    GmresDev* gd_host = nullptr;  // its pinned copy (flags read per chunk)
+    int last_its = 4;             // the first chunk of the next solve
     int newton_its = 0, gmres_its = 0, fallbacks = 0;
     double rnorm = 0.0;
 };
 
 namespace {
 
-// dot products <w, v_i>, i <= k, on the host (synchronising)
-int mdot(zj_imex_global* g, const double* w, const double* Vb, int k, double* out,
-         hipStream_t st) {
-    k_mdot<<<IGRID, IB, 0, st>>>(g->n, k, w, Vb, g->part);
-    k_fold<<<k + 1, IB, 0, st>>>(IGRID, g->part, g->dots);
-    IHIPCHK(hipMemcpyAsync(g->host, g->dots, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, st));
-    IHIPCHK(hipStreamSynchronize(st));
-    std::copy(g->host, g->host + k + 1, out);
-    return ZJ_OK;
-}
-
-int norm2(zj_imex_global* g, const double* w, double* out, hipStream_t st) {
-    double d = 0.0;
-    if (int rc = mdot(g, w, w, 0, &d, st)) return rc;
-    *out = std::sqrt(std::max(d, 0.0));
-    return ZJ_OK;
-}
-
 // R(Q) into g->R and its norm (g->aux holds aux(Q), whose dh/dx_d rows do
 // not change during the stage)
 int residual(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* Qe,
              double* rnorm, hipStream_t st) {
     k_src_residual<<<IGRID, IB, 0, st>>>(p, Q, Qe, g->aux, g->kf, g->R, g->part);
-    k_fold<<<1, IB, 0, st>>>(IGRID, g->part, g->dots);
+    k_fold<<<1, IB, 0, st>>>(IGRID, g->part, g->dots, nullptr);
     IHIPCHK(hipMemcpyAsync(g->host, g->dots, sizeof(double), hipMemcpyDeviceToHost, st));
     IHIPCHK(hipStreamSynchronize(st));
     *rnorm = std::sqrt(std::max(g->host[0], 0.0));
     return ZJ_OK;
 }
 
-// J v at the current Q
-int jvp(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* v, double* out,
-        hipStream_t st) {
-    k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, v, out);
-    IHIPCHK(hipGetLastError());
-    return ZJ_OK;
-}
-
 // GMRES(m) for J x = b (b = -R in g->R's negation), x accumulated into dx
-// (zero start), relative tolerance tol, at most maxres restart cycles.
-// Returns 0 when the tolerance was met.
+// (zero start), relative tolerance tol, at most maxres restart cycles; the
+// iteration state lives in g->gd (see GmresDev).  Sets *converged when the
+// tolerance was met.
 int gmres(zj_imex_global* g, const SrcPrm& p, const double* Q, double* dx, double tol,
           int maxres, int* its, int* converged, hipStream_t st) {
     const int n = g->n, m = g->m;
     double* V = g->Vb;                     // (m + 1) basis vectors
     double* w = g->W;
-    // b = -R
-    double bnorm = 0.0;
-    if (int rc = norm2(g, g->R, &bnorm, st)) return rc;
+    GmresDev* G = g->gd;
+    const int* stop = &G->stop;
     IHIPCHK(hipMemsetAsync(dx, 0, sizeof(double) * n, st));
-    *converged = bnorm == 0.0;
+    *converged = 0;
     *its = 0;
-    if (*converged) return ZJ_OK;
-    const double target = tol * bnorm;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), e(m + 1), h(m + 1), h2(m + 1);
+    auto read = [&]() -> int {
+        IHIPCHK(hipMemcpyAsync(g->gd_host, G, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
+        IHIPCHK(hipStreamSynchronize(st));
+        return ZJ_OK;
+    };
+    // one Arnoldi iteration k (the kernels return at once after a stop)
+    auto arnoldi = [&](int k) {
+        double* vk1 = V + (size_t)(k + 1) * n;
+        k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, V + (size_t)k * n, vk1, stop);
+        // classical Gram-Schmidt, twice
+        k_mdot<<<IGRID, IB, 0, st>>>(n, k, vk1, V, g->part, stop);
+        k_fold<<<k + 1, IB, 0, st>>>(IGRID, g->part, G->d1, stop);
+        k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, k + 1, nullptr, G->d1, -1.0, V, vk1, stop);
+        k_mdot<<<IGRID, IB, 0, st>>>(n, k, vk1, V, g->part, stop);
+        k_fold<<<k + 1, IB, 0, st>>>(IGRID, g->part, G->d2, stop);
+        k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, k + 1, nullptr, G->d2, -1.0, V, vk1, stop);
+        k_mdot<<<IGRID, IB, 0, st>>>(n, 0, vk1, vk1, g->part, stop);
+        k_fold<<<1, IB, 0, st>>>(IGRID, g->part, &G->dn, stop);
+        k_gm_givens<<<1, 1, 0, st>>>(G, k);
+        k_gm_unit<<<IGRID, IB, 0, st>>>(n, G, 1, vk1);
+    };
     for (int cyc = 0; cyc < maxres; ++cyc) {
-        // r = b - J dx
-        double beta;
-        if (cyc == 0) {
-            k_scale<<<IGRID, IB, 0, st>>>(n, -1.0, g->R, V);
-            beta = bnorm;
-        } else {
-            // V0 = -R - J dx
-            if (int rc = jvp(g, p, Q, dx, w, st)) return rc;
-            k_scale<<<IGRID, IB, 0, st>>>(n, -1.0, g->R, V);
+        // V0 = r = b - J dx
+        k_scale<<<IGRID, IB, 0, st>>>(n, -1.0, g->R, V);
+        if (cyc > 0) {
+            k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, dx, w, nullptr);
             Coefs c1{};
             c1.a[0] = -1.0;
             k_maxpy<<<IGRID, IB, 0, st>>>(n, 1, c1, 1.0, w, V);
-            if (int rc = norm2(g, V, &beta, st)) return rc;
-            if (beta <= target) {
-                *converged = 1;
-                return ZJ_OK;
-            }
         }
-        k_scale<<<IGRID, IB, 0, st>>>(n, 1.0 / beta, V, V);
-        std::fill(e.begin(), e.end(), 0.0);
-        e[0] = beta;
-        int k = 0;
-        for (; k < m; ++k) {
-            double* vk1 = V + (size_t)(k + 1) * n;
-            if (int rc = jvp(g, p, Q, V + (size_t)k * n, vk1, st)) return rc;
-            // classical Gram-Schmidt, twice
-            if (int rc = mdot(g, vk1, V, k, h.data(), st)) return rc;
-            Coefs c{};
-            for (int i = 0; i <= k; ++i) c.a[i] = -h[i];
-            k_maxpy<<<IGRID, IB, 0, st>>>(n, k + 1, c, 1.0, V, vk1);
-            if (int rc = mdot(g, vk1, V, k, h2.data(), st)) return rc;
-            for (int i = 0; i <= k; ++i) {
-                c.a[i] = -h2[i];
-                h[i] += h2[i];
-            }
-            k_maxpy<<<IGRID, IB, 0, st>>>(n, k + 1, c, 1.0, V, vk1);
-            double hn = 0.0;
-            if (int rc = norm2(g, vk1, &hn, st)) return rc;
-            h[k + 1] = hn;
-            if (hn > 0.0) k_scale<<<IGRID, IB, 0, st>>>(n, 1.0 / hn, vk1, vk1);
-            // apply the previous rotations, then a new one
-            for (int i = 0; i < k; ++i) {
-                const double t = cs[i] * h[i] + sn[i] * h[i + 1];
-                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-                h[i] = t;
-            }
-            const double r = std::hypot(h[k], h[k + 1]);
-            cs[k] = r > 0.0 ? h[k] / r : 1.0;
-            sn[k] = r > 0.0 ? h[k + 1] / r : 0.0;
-            h[k] = r;
-            h[k + 1] = 0.0;
-            e[k + 1] = -sn[k] * e[k];
-            e[k] = cs[k] * e[k];
-            for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = h[i];
-            *its += 1;
-            if (std::fabs(e[k + 1]) <= target || hn == 0.0) {
-                ++k;
-                break;
-            }
+        k_mdot<<<IGRID, IB, 0, st>>>(n, 0, V, V, g->part, nullptr);
+        k_fold<<<1, IB, 0, st>>>(IGRID, g->part, &G->dn, nullptr);
+        k_gm_cycle<<<1, 1, 0, st>>>(G, tol, cyc == 0);
+        k_gm_unit<<<IGRID, IB, 0, st>>>(n, G, 0, V);
+        int launched = 0;
+        int chunk = std::max(1, std::min(m, g->last_its));
+        while (true) {
+            for (int k = launched; k < launched + chunk; ++k) arnoldi(k);
+            IHIPCHK(hipGetLastError());
+            launched += chunk;
+            if (int rc = read()) return rc;
+            if (g->gd_host->stop || launched >= m) break;
+            chunk = std::min(2, m - launched);
         }
-        // y = H^-1 e (upper triangular k x k), dx += V y
-        std::vector<double> y(k);
-        for (int i = k - 1; i >= 0; --i) {
-            double s = e[i];
-            for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
-            y[i] = s / H[(size_t)i * m + i];
+        // the cycle converged on its start (stop with no iteration) or after
+        // its iterations: y and dx += V y
+        if (g->gd_host->k > 0) {
+            k_gm_finish<<<1, 1, 0, st>>>(G);
+            k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, 0, &G->k, G->y, 1.0, V, dx, nullptr);
+            IHIPCHK(hipGetLastError());
         }
-        Coefs c{};
-        for (int i = 0; i < k; ++i) c.a[i] = y[i];
-        k_maxpy<<<IGRID, IB, 0, st>>>(n, k, c, 1.0, V, dx);
-        IHIPCHK(hipGetLastError());
-        if (std::fabs(e[k]) <= target) {
+        *its = g->gd_host->its;
+        // otherwise (the cycle's m iterations, or a breakdown short of the
+        // target) the next cycle restarts from the true residual
+        if (g->gd_host->conv) {
             *converged = 1;
-            return ZJ_OK;
+            break;
         }
     }
+    g->last_its = std::max(1, *its);
     return ZJ_OK;
 }
 
@@ -359,7 +422,9 @@ int zj_imex_global_create(zj_solver* s, double alpha, int32_t restart, zj_imex_g
         hipMalloc((void**)&g->Vb, sizeof(double) * g->n * (size_t)(g->m + 1)) != hipSuccess ||
         hipMalloc((void**)&g->part, sizeof(double) * IGRID * (MAXK + 1)) != hipSuccess ||
         hipMalloc((void**)&g->dots, sizeof(double) * (MAXK + 1)) != hipSuccess ||
-        hipHostMalloc((void**)&g->host, sizeof(double) * (MAXK + 1)) != hipSuccess) {
+        hipHostMalloc((void**)&g->host, sizeof(double) * (MAXK + 1)) != hipSuccess ||
+        hipMalloc((void**)&g->gd, sizeof(GmresDev)) != hipSuccess ||
+        hipHostMalloc((void**)&g->gd_host, sizeof(GmresDev)) != hipSuccess) {
         zj_imex_global_destroy(g);
         return ZJ_ENOMEM;
     }
@@ -372,6 +437,8 @@ int zj_imex_global_destroy(zj_imex_global* g) {
     for (double* p : {g->aux, g->kf, g->R, g->W, g->Vb, g->dX, g->part, g->dots})
         if (p) (void)hipFree(p);
     if (g->host) (void)hipHostFree(g->host);
+    if (g->gd) (void)hipFree(g->gd);
+    if (g->gd_host) (void)hipHostFree(g->gd_host);
     delete g;
     return ZJ_OK;
 }
