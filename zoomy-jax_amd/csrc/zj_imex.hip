@@ -17,14 +17,18 @@
 // Krylov vector live in the dim momentum rows: v_h = 0, hence the aux(v)
 // rows the source reads (dh/dx_d) vanish and aux(Q) is the same for every
 // Newton step -- it is refreshed once per solve.  GMRES is restarted
-// GMRES(m) with classical Gram-Schmidt and one reorthogonalisation.  The
-// dot products, the Hessenberg columns, the Givens rotations, the
-// back-substitution and the convergence test stay on the device (one-thread
-// kernels on a GmresDev block); the host enqueues Arnoldi iterations in
-// chunks (the first as long as the previous solve took) and reads the
-// block's flags once per chunk, and the Arnoldi kernels of a chunk return at
-// once after convergence.  Every reduction stores one partial per block and
-// a fold kernel sums them in a fixed order, so the stage is deterministic.
+// GMRES(m) with classical Gram-Schmidt and one reorthogonalisation.  An
+// Arnoldi iteration is three one-cell-per-thread kernels, each emitting its
+// dot products as block partials: the Jacobian product of v_k (scaled by
+// the norm of the previous iteration on the way, so the normalisation costs
+// no pass of its own) with <w, v_i>; the first Gram-Schmidt update with the
+// second pass's dots; the second update with <w, w>.  Fixed-order fold
+// kernels sum the partials; the Hessenberg column, the Givens rotations and
+// the convergence test run in the last fold's block, and the
+// back-substitution in every block of the solution update.  The host
+// enqueues Arnoldi iterations in chunks (the first as long as the previous
+// solve took) and reads four flags once per chunk; the kernels of a chunk
+// return at once after convergence.  The stage is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +43,6 @@
 namespace {
 
 constexpr int IB = 256;        // threads per block
-constexpr int IGRID = 512;     // blocks of the grid-stride vector kernels
 constexpr int MAXK = 32;       // largest Krylov space (restart length)
 
 #define IHIPCHK(x)                                                    \
@@ -75,40 +78,62 @@ __device__ __forceinline__ void source_at(const SrcPrm& p, const double* q, doub
     S[1] = p.dim == 2 ? -p.alpha * p.g * q[1] * hx[1] - m * q[3] : -m * q[3];
 }
 
-// kf[c] = g n^2 hinv(h_c)^(7/3), once per stage
-__global__ __launch_bounds__(IB) void k_friction(SrcPrm p, const double* __restrict__ Q,
-                                                 double* __restrict__ kf) {
-    for (int c = blockIdx.x * IB + threadIdx.x; c < p.N; c += gridDim.x * IB)
-        kf[c] = p.g * p.n2 * pow(hinv_d(Q[(size_t)p.N + c], p.eps), 7.0 / 3.0);
+// Start of the stage, one cell per thread: Q = Qexp (the 4 rows) and
+// kf[c] = g n^2 hinv(h_c)^(7/3) (h does not change during the stage)
+__global__ __launch_bounds__(IB) void k_start(SrcPrm p, const double* __restrict__ Qe,
+                                              double* __restrict__ Q, double* __restrict__ kf) {
+    const int c = blockIdx.x * IB + threadIdx.x;
+    if (c >= p.N) return;
+    for (int v = 0; v < 4; ++v) Q[(size_t)v * p.N + c] = Qe[(size_t)v * p.N + c];
+    kf[c] = p.g * p.n2 * pow(hinv_d(Qe[(size_t)p.N + c], p.eps), 7.0 / 3.0);
 }
 
-__device__ __forceinline__ void block_sum(double v, double* lds, double* out) {
+__device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// *out = the sum of v over a block of B threads (wave sums, then the waves
+// in order)
+template <int B>
+__device__ __forceinline__ void block_sum(double v, double* lds, double* out) {
+    v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int i = 0; i < IB / 64; ++i) s += lds[i];
+        for (int i = 0; i < B / 64; ++i) s += lds[i];
         *out = s;
     }
     __syncthreads();
 }
 
-// R = Q - Qexp - dt S(Q, aux) in the momentum rows (R[d N + c], d < dim;
-// the b and h rows of Q equal Qexp's); part[block] = sum of R^2 over the
-// block
-__global__ __launch_bounds__(IB) void k_src_residual(SrcPrm p, const double* __restrict__ Q,
+// Newton update and residual, one cell per thread: Q_d += corr_d (corr =
+// -R_d, the residual of the last call, when neg_r; nothing when corr is
+// null and neg_r is 0), then R = Q - Qexp - dt S(Q, aux) in the momentum
+// rows (R[d N + c], d < dim; the b and h rows of Q equal Qexp's);
+// part[block] = sum of R^2 over the block
+__global__ __launch_bounds__(IB) void k_src_residual(SrcPrm p, double* __restrict__ Q,
                                                      const double* __restrict__ Qe,
                                                      const double* __restrict__ aux,
                                                      const double* __restrict__ kf,
+                                                     const double* __restrict__ corr, int neg_r,
                                                      double* __restrict__ R,
                                                      double* __restrict__ part) {
     __shared__ double lds[IB / 64];
     double acc = 0.0;
     const int N = p.N;
-    for (int c = blockIdx.x * IB + threadIdx.x; c < N; c += gridDim.x * IB) {
+    const int c = blockIdx.x * IB + threadIdx.x;
+    if (c < N) {
         double q[4] = {0.0, 0.0, 0.0, 0.0}, hx[2] = {0.0, 0.0};
         for (int v = 1; v < 4; ++v) q[v] = Q[(size_t)v * N + c];
+        if (corr || neg_r) {
+            for (int d = 0; d < p.dim; ++d) {
+                const size_t j = (size_t)d * N + c;
+                q[2 + d] += neg_r ? -R[j] : corr[j];
+                Q[(size_t)(2 + d) * N + c] = q[2 + d];
+            }
+        }
         for (int d = 0; d < p.dim; ++d) hx[d] = aux[(size_t)row_dh(d) * N + c];
         double S[2];
         source_at(p, q, kf[c], hx, S);
@@ -118,128 +143,189 @@ __global__ __launch_bounds__(IB) void k_src_residual(SrcPrm p, const double* __r
             acc += r * r;
         }
     }
-    block_sum(acc, lds, part + blockIdx.x);
+    block_sum<IB>(acc, lds, part + blockIdx.x);
 }
 
-// out = J v = v - dt dS/dq v for momentum vectors v (v[d N + c], d < dim):
-// with v_h = 0 the terms of dh and of aux(v) (whose dh/dx_d rows are the
-// gradients of v_h) vanish
-__global__ __launch_bounds__(IB) void k_src_jvp(SrcPrm p, const double* __restrict__ Q,
-                                                const double* __restrict__ kf,
-                                                const double* __restrict__ V,
-                                                double* __restrict__ out,
-                                                const int* __restrict__ stop) {
-    if (stop && *stop) return;
-    const int N = p.N;
-    for (int c = blockIdx.x * IB + threadIdx.x; c < N; c += gridDim.x * IB) {
-        double q[4] = {0.0, 0.0, 0.0, 0.0}, v[2] = {0.0, 0.0};
-        for (int k = 2; k < 4; ++k) q[k] = Q[(size_t)k * N + c];
-        for (int d = 0; d < p.dim; ++d) v[d] = V[(size_t)d * N + c];
-        const double qq = sqrt(q[2] * q[2] + q[3] * q[3]);
-        const double k = kf[c];
-        // d(|q| k) = k (q.dq / |q|) at fixed h
-        const double dqq = qq > 0.0 ? (q[2] * v[0] + (p.dim == 2 ? q[3] * v[1] : 0.0)) / qq : 0.0;
-        const double dm = k * dqq;
-        const double m = k * qq;
-        for (int d = 0; d < p.dim; ++d) {
-            const double dS = -(dm * q[2 + d] + m * v[d]);
-            out[(size_t)d * N + c] = v[d] - p.dt * dS;
-        }
-    }
-}
-
-struct Coefs {
-    double a[MAXK + 1];
-};
-
-// part[i * gridDim + block] = <w, v_i> over the block's entries, i <= k
-__global__ __launch_bounds__(IB) void k_mdot(int n, int k, const double* __restrict__ w,
-                                             const double* __restrict__ Vb,
-                                             double* __restrict__ part,
-                                             const int* __restrict__ stop) {
-    if (stop && *stop) return;
-    __shared__ double lds[IB / 64];
-    for (int i = 0; i <= k; ++i) {
-        const double* vi = Vb + (size_t)i * n;
-        double acc = 0.0;
-        for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB)
-            acc += w[j] * vi[j];
-        block_sum(acc, lds, part + (size_t)i * gridDim.x + blockIdx.x);
-    }
-}
-
-// out[i] = sum of the nb partials of dot i, in a fixed order
-__global__ __launch_bounds__(IB) void k_fold(int nb, const double* __restrict__ part,
-                                             double* __restrict__ out,
-                                             const int* __restrict__ stop) {
-    if (stop && *stop) return;
-    __shared__ double lds[IB / 64];
-    const double* p = part + (size_t)blockIdx.x * nb;
-    double acc = 0.0;
-    for (int j = threadIdx.x; j < nb; j += IB) acc += p[j];
-    block_sum(acc, lds, out + blockIdx.x);
-}
-
-// w += sum_i a_i v_i (i < m) (and w *= s first when scale)
-__global__ __launch_bounds__(IB) void k_maxpy(int n, int m, Coefs c, double s,
-                                              const double* __restrict__ Vb,
-                                              double* __restrict__ w) {
-    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) {
-        double x = w[j] * s;
-        for (int i = 0; i < m; ++i) x += c.a[i] * Vb[(size_t)i * n + j];
-        w[j] = x;
-    }
-}
-
-// y = a x
-__global__ __launch_bounds__(IB) void k_scale(int n, double a, const double* __restrict__ x,
-                                              double* __restrict__ y) {
-    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) y[j] = a * x[j];
+// (J v)_d at cell c = v_d - dt (dS/dq v)_d for momentum vectors v: with
+// v_h = 0 the terms of dh and of aux(v) (whose dh/dx_d rows are the
+// gradients of v_h) vanish; q2, q3 the momenta, k the cell's kf
+__device__ __forceinline__ void jvp_at(const SrcPrm& p, double q2, double q3, double k,
+                                       const double* v, double* out) {
+    const double qq = sqrt(q2 * q2 + q3 * q3);
+    // d(|q| k) = k (q.dq / |q|) at fixed h
+    const double dqq = qq > 0.0 ? (q2 * v[0] + (p.dim == 2 ? q3 * v[1] : 0.0)) / qq : 0.0;
+    const double dm = k * dqq;
+    const double m = k * qq;
+    const double qd[2] = {q2, q3};
+    for (int d = 0; d < p.dim; ++d) out[d] = v[d] - p.dt * (-(dm * qd[d] + m * v[d]));
 }
 
 // GMRES state on the device (one restart cycle): the Hessenberg columns
 // H[i * MAXK + k], the rotations, the rotated right-hand side e, the
-// solution coefficients y, the two Gram-Schmidt passes' dot products d1, d2
-// and the norm dot dn; stop ends the Arnoldi kernels of the cycle, k counts
-// its iterations, its every iteration of the solve, conv marks convergence.
+// Gram-Schmidt passes' dot products d1, d2 and the norm dot dn of the
+// current iteration, hn = |w| of the last one (v_{k+1} is stored unscaled
+// and scaled by 1 / hn when the next iteration reads it; v_0 by 1 / beta);
+// stop ends the Arnoldi kernels of the cycle, k counts its iterations, its
+// every iteration of the solve, conv marks convergence.
 struct GmresDev {
     double H[(MAXK + 1) * MAXK];
-    double cs[MAXK], sn[MAXK], e[MAXK + 1], y[MAXK];
+    double cs[MAXK], sn[MAXK], e[MAXK + 1];
     double d1[MAXK + 1], d2[MAXK + 1];
     double dn, hn, beta, target;
     int stop, k, its, conv;
 };
 
-// w += sign sum_i coef_i v_i over m columns (m from *mdev when given)
-__global__ __launch_bounds__(IB) void k_maxpy_dev(int n, int m, const int* __restrict__ mdev,
-                                                  const double* __restrict__ coef, double sign,
-                                                  const double* __restrict__ Vb,
-                                                  double* __restrict__ w,
-                                                  const int* __restrict__ stop) {
-    if (stop && *stop) return;
-    if (mdev) m = *mdev;
-    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) {
-        double x = w[j];
-        for (int i = 0; i < m; ++i) x += (sign * coef[i]) * Vb[(size_t)i * n + j];
-        w[j] = x;
+// Block partials of the dots of a one-cell-per-thread kernel: every thread
+// of the block calls emit(v, i) for the same i in turn (0 for cells past
+// the end), then finish(nd) stores part[i * gridDim + block], i < nd.
+struct BlockDots {
+    double (*lds)[IB / 64];
+    __device__ __forceinline__ void emit(double v, int i) const {
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) lds[i][threadIdx.x >> 6] = v;
+    }
+    __device__ __forceinline__ void finish(int nd, double* __restrict__ part) const {
+        __syncthreads();
+        if ((int)threadIdx.x < nd) {
+            double s = 0.0;
+            for (int j = 0; j < IB / 64; ++j) s += lds[threadIdx.x][j];
+            part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+        }
+    }
+};
+
+// The cell's momentum entries of basis vector i (zero past the end)
+__device__ __forceinline__ void load_v(const double* Vb, int i, int n, int N, int c, int dim,
+                                       double* v) {
+    v[0] = 0.0;
+    v[1] = 0.0;
+    if (c < N)
+        for (int d = 0; d < dim; ++d) v[d] = Vb[(size_t)i * n + (size_t)d * N + c];
+}
+
+// Start of a cycle, one cell per thread: v_0 = -R (first) or -R - J dx;
+// the partials of <v_0, v_0>
+__global__ __launch_bounds__(IB) void k_gm_start(SrcPrm p, const double* __restrict__ Q,
+                                                 const double* __restrict__ kf,
+                                                 const double* __restrict__ R,
+                                                 const double* __restrict__ dx, int first,
+                                                 double* __restrict__ V0,
+                                                 double* __restrict__ part) {
+    __shared__ double lds[1][IB / 64];
+    const BlockDots bd{lds};
+    const int N = p.N, c = blockIdx.x * IB + threadIdx.x;
+    double acc = 0.0;
+    if (c < N) {
+        double v[2] = {0.0, 0.0}, jv[2] = {0.0, 0.0};
+        for (int d = 0; d < p.dim; ++d) v[d] = -R[(size_t)d * N + c];
+        if (!first) {
+            double x[2] = {0.0, 0.0};
+            for (int d = 0; d < p.dim; ++d) x[d] = dx[(size_t)d * N + c];
+            jvp_at(p, Q[(size_t)2 * N + c], Q[(size_t)3 * N + c], kf[c], x, jv);
+        }
+        for (int d = 0; d < p.dim; ++d) {
+            const double r = v[d] - jv[d];
+            V0[(size_t)d * N + c] = r;
+            acc += r * r;
+        }
+    }
+    bd.emit(acc, 0);
+    bd.finish(1, part);
+}
+
+// Arnoldi iteration k, first kernel, one cell per thread: v_k *= 1 / beta
+// (k = 0) or 1 / hn (the deferred normalisation, written back), w = J v_k
+// into v_{k+1}; the partials of <w, v_i>, i <= k
+__global__ __launch_bounds__(IB) void k_arn_jvp(SrcPrm p, const double* __restrict__ Q,
+                                                const double* __restrict__ kf, double* Vb,
+                                                int n, int k, const GmresDev* __restrict__ G,
+                                                double* __restrict__ part) {
+    if (G->stop) return;
+    __shared__ double lds[MAXK + 1][IB / 64];
+    const BlockDots bd{lds};
+    const int N = p.N, c = blockIdx.x * IB + threadIdx.x;
+    const double sv = k == 0 ? G->beta : G->hn;
+    const double s = sv > 0.0 ? 1.0 / sv : 1.0;
+    double v[2], w[2] = {0.0, 0.0};
+    load_v(Vb, k, n, N, c, p.dim, v);
+    if (c < N) {
+        for (int d = 0; d < p.dim; ++d) {
+            v[d] *= s;
+            Vb[(size_t)k * n + (size_t)d * N + c] = v[d];
+        }
+        jvp_at(p, Q[(size_t)2 * N + c], Q[(size_t)3 * N + c], kf[c], v, w);
+        for (int d = 0; d < p.dim; ++d) Vb[(size_t)(k + 1) * n + (size_t)d * N + c] = w[d];
+    }
+    for (int i = 0; i < k; ++i) {
+        double u[2];
+        load_v(Vb, i, n, N, c, p.dim, u);
+        bd.emit(w[0] * u[0] + w[1] * u[1], i);
+    }
+    bd.emit(w[0] * v[0] + w[1] * v[1], k);
+    bd.finish(k + 1, part);
+}
+
+// Arnoldi iteration k, a Gram-Schmidt pass, one cell per thread: w -=
+// sum_i coef_i v_i (i <= k, coef = d1 or d2); then the partials of
+// <w, v_i>, i <= k (pass 1; the v_i read again, from cache) or of <w, w>
+// (pass 2)
+template <int PASS>
+__global__ __launch_bounds__(IB) void k_arn_gs(double* Vb, int n, int N, int dim, int k,
+                                               const GmresDev* __restrict__ G,
+                                               double* __restrict__ part) {
+    if (G->stop) return;
+    __shared__ double lds[PASS == 1 ? MAXK + 1 : 1][IB / 64];
+    const BlockDots bd{lds};
+    const int c = blockIdx.x * IB + threadIdx.x;
+    const double* coef = PASS == 1 ? G->d1 : G->d2;
+    double w[2];
+    load_v(Vb, k + 1, n, N, c, dim, w);
+    for (int i = 0; i <= k; ++i) {
+        double u[2];
+        load_v(Vb, i, n, N, c, dim, u);
+        const double a = coef[i];
+        w[0] -= a * u[0];
+        w[1] -= a * u[1];
+    }
+    if (c < N)
+        for (int d = 0; d < dim; ++d) Vb[(size_t)(k + 1) * n + (size_t)d * N + c] = w[d];
+    if (PASS == 1) {
+        for (int i = 0; i <= k; ++i) {
+            double u[2];
+            load_v(Vb, i, n, N, c, dim, u);
+            bd.emit(w[0] * u[0] + w[1] * u[1], i);
+        }
+        bd.finish(k + 1, part);
+    } else {
+        bd.emit(w[0] * w[0] + w[1] * w[1], 0);
+        bd.finish(1, part);
     }
 }
 
-// x *= 1 / s with s = beta (which == 0) or hn (which == 1), unless stopped
-// or s == 0
-__global__ __launch_bounds__(IB) void k_gm_unit(int n, const GmresDev* __restrict__ G, int which,
-                                                double* __restrict__ x) {
-    if (G->stop) return;
-    const double sv = which ? G->hn : G->beta;
-    if (!(sv > 0.0)) return;
-    const double a = 1.0 / sv;
-    for (int j = blockIdx.x * IB + threadIdx.x; j < n; j += gridDim.x * IB) x[j] = a * x[j];
+constexpr int FB = 1024;  // threads of a fold block
+
+// out[i] = sum of the nb partials of dot i (block i), in a fixed order
+__global__ __launch_bounds__(FB) void k_fold(int nb, const double* __restrict__ part,
+                                             double* __restrict__ out,
+                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    __shared__ double lds[FB / 64];
+    const double* pp = part + (size_t)blockIdx.x * nb;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nb; j += FB) acc += pp[j];
+    block_sum<FB>(acc, lds, out + blockIdx.x);
 }
 
-// Start of a cycle: beta = |r| from dn, the target of the solve on the first
-// cycle (tol |b|), e = (beta, 0, ...); converged (stop) when |r| <= target
-// (first cycle: when b = 0).
-__global__ void k_gm_cycle(GmresDev* G, double tol, int first) {
+// Start of a cycle (one block): dn = |v_0|^2 from the partials, beta, the
+// target of the solve on the first cycle (tol |b|), e = (beta, 0, ...);
+// converged (stop) when |r| <= target (first cycle: when b = 0).
+__global__ __launch_bounds__(FB) void k_gm_cycle(int nb, const double* __restrict__ part,
+                                                 GmresDev* G, double tol, int first) {
+    __shared__ double lds[FB / 64];
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nb; j += FB) acc += part[j];
+    block_sum<FB>(acc, lds, &G->dn);
+    if (threadIdx.x != 0) return;
     const double beta = sqrt(fmax(G->dn, 0.0));
     G->beta = beta;
     if (first) {
@@ -253,12 +339,18 @@ __global__ void k_gm_cycle(GmresDev* G, double tol, int first) {
     G->stop = G->conv;
 }
 
-// Arnoldi iteration k, after both Gram-Schmidt passes and the norm dot: the
-// Hessenberg column (h = d1 + d2, h_{k+1} = |v_{k+1}|), the previous
-// rotations applied, a new one, e rotated; converged when |e_{k+1}| <=
-// target, stop then or on breakdown (|v_{k+1}| = 0).
-__global__ void k_gm_givens(GmresDev* G, int k) {
+// End of Arnoldi iteration k (one block): dn = |w|^2 from the partials; the
+// Hessenberg column (h = d1 + d2, h_{k+1} = |w|), the previous rotations
+// applied, a new one, e rotated; converged when |e_{k+1}| <= target, stop
+// then or on breakdown (|w| = 0).
+__global__ __launch_bounds__(FB) void k_gm_givens(int nb, const double* __restrict__ part,
+                                                  GmresDev* G, int k) {
     if (G->stop) return;
+    __shared__ double lds[FB / 64];
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nb; j += FB) acc += part[j];
+    block_sum<FB>(acc, lds, &G->dn);
+    if (threadIdx.x != 0) return;
     double h[MAXK + 1];
     for (int i = 0; i <= k; ++i) h[i] = G->d1[i] + G->d2[i];
     const double hn = sqrt(fmax(G->dn, 0.0));
@@ -282,13 +374,29 @@ __global__ void k_gm_givens(GmresDev* G, int k) {
     if (G->conv || hn == 0.0) G->stop = 1;
 }
 
-// End of a cycle: y = H^-1 e over its k iterations (upper triangular).
-__global__ void k_gm_finish(GmresDev* G) {
+// End of a cycle, one cell per thread: y = H^-1 e over the cycle's k
+// iterations (upper triangular; every block solves it alike in LDS), then
+// dx = V y (first cycle) or dx += V y
+__global__ __launch_bounds__(IB) void k_gm_update(const GmresDev* __restrict__ G,
+                                                  const double* __restrict__ Vb, int n, int N,
+                                                  int dim, int first, double* __restrict__ dx) {
+    __shared__ double y[MAXK];
     const int k = G->k;
-    for (int i = k - 1; i >= 0; --i) {
-        double sv = G->e[i];
-        for (int j = i + 1; j < k; ++j) sv -= G->H[(size_t)i * MAXK + j] * G->y[j];
-        G->y[i] = sv / G->H[(size_t)i * MAXK + i];
+    if (threadIdx.x == 0) {
+        for (int i = k - 1; i >= 0; --i) {
+            double sv = G->e[i];
+            for (int j = i + 1; j < k; ++j) sv -= G->H[(size_t)i * MAXK + j] * y[j];
+            y[i] = sv / G->H[(size_t)i * MAXK + i];
+        }
+    }
+    __syncthreads();
+    const int c = blockIdx.x * IB + threadIdx.x;
+    if (c >= N) return;
+    for (int d = 0; d < dim; ++d) {
+        const size_t j = (size_t)d * N + c;
+        double x = first ? 0.0 : dx[j];
+        for (int i = 0; i < k; ++i) x += y[i] * Vb[(size_t)i * n + j];
+        dx[j] = x;
     }
 }
 
@@ -296,104 +404,82 @@ __global__ void k_gm_finish(GmresDev* G) {
 
 struct zj_imex_global {
     zj_solver* s = nullptr;
-    int N = 0, n = 0, dim = 2, m = 20;
+    int N = 0, n = 0, dim = 2, m = 20, nb = 0;
     double alpha = 0.0;
-    double *aux = nullptr, *kf = nullptr, *R = nullptr, *W = nullptr, *Vb = nullptr,
-           *dX = nullptr, *part = nullptr, *dots = nullptr, *host = nullptr;
-    GmresDev* gd = nullptr;       // the GMRES block on the device
-    GmresDev* gd_host = nullptr;  // its pinned copy (flags read per chunk)
-    int last_its = 4;             // the first chunk of the next solve
+    double *aux = nullptr, *kf = nullptr, *R = nullptr, *Vb = nullptr, *dX = nullptr,
+           *part = nullptr, *dots = nullptr, *host = nullptr;
+    GmresDev* gd = nullptr;  // the GMRES block on the device
+    int* flags = nullptr;    // pinned copy of its stop, k, its, conv
+    int last_its = 4;        // the first chunk of the next solve
     int newton_its = 0, gmres_its = 0, fallbacks = 0;
     double rnorm = 0.0;
 };
 
 namespace {
 
-// R(Q) into g->R and its norm (g->aux holds aux(Q), whose dh/dx_d rows do
-// not change during the stage)
-int residual(zj_imex_global* g, const SrcPrm& p, const double* Q, const double* Qe,
-             double* rnorm, hipStream_t st) {
-    k_src_residual<<<IGRID, IB, 0, st>>>(p, Q, Qe, g->aux, g->kf, g->R, g->part);
-    k_fold<<<1, IB, 0, st>>>(IGRID, g->part, g->dots, nullptr);
+// Q_mom += corr (or -R when neg_r; neither when both are off), then R(Q)
+// into g->R and its norm (g->aux holds aux(Q), whose dh/dx_d rows do not
+// change during the stage)
+int residual(zj_imex_global* g, const SrcPrm& p, double* Q, const double* Qe,
+             const double* corr, int neg_r, double* rnorm, hipStream_t st) {
+    k_src_residual<<<g->nb, IB, 0, st>>>(p, Q, Qe, g->aux, g->kf, corr, neg_r, g->R, g->part);
+    k_fold<<<1, FB, 0, st>>>(g->nb, g->part, g->dots, nullptr);
     IHIPCHK(hipMemcpyAsync(g->host, g->dots, sizeof(double), hipMemcpyDeviceToHost, st));
     IHIPCHK(hipStreamSynchronize(st));
     *rnorm = std::sqrt(std::max(g->host[0], 0.0));
     return ZJ_OK;
 }
 
-// GMRES(m) for J x = b (b = -R in g->R's negation), x accumulated into dx
-// (zero start), relative tolerance tol, at most maxres restart cycles; the
-// iteration state lives in g->gd (see GmresDev).  Sets *converged when the
-// tolerance was met.
+// GMRES(m) for J x = b (b = -R), x into dx (zero start), relative
+// tolerance tol, at most maxres restart cycles; the iteration state lives
+// in g->gd (see GmresDev).  Sets *converged when the tolerance was met.
 int gmres(zj_imex_global* g, const SrcPrm& p, const double* Q, double* dx, double tol,
           int maxres, int* its, int* converged, hipStream_t st) {
-    const int n = g->n, m = g->m;
-    double* V = g->Vb;                     // (m + 1) basis vectors
-    double* w = g->W;
+    const int n = g->n, m = g->m, nb = g->nb, N = g->N, dim = g->dim;
+    double* V = g->Vb;  // (m + 1) basis vectors
     GmresDev* G = g->gd;
     const int* stop = &G->stop;
-    IHIPCHK(hipMemsetAsync(dx, 0, sizeof(double) * n, st));
+    int* fl = g->flags;  // stop, k, its, conv
     *converged = 0;
     *its = 0;
-    auto read = [&]() -> int {
-        IHIPCHK(hipMemcpyAsync(g->gd_host, G, sizeof(GmresDev), hipMemcpyDeviceToHost, st));
-        IHIPCHK(hipStreamSynchronize(st));
-        return ZJ_OK;
-    };
-    // one Arnoldi iteration k (the kernels return at once after a stop)
-    auto arnoldi = [&](int k) {
-        double* vk1 = V + (size_t)(k + 1) * n;
-        k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, V + (size_t)k * n, vk1, stop);
-        // classical Gram-Schmidt, twice
-        k_mdot<<<IGRID, IB, 0, st>>>(n, k, vk1, V, g->part, stop);
-        k_fold<<<k + 1, IB, 0, st>>>(IGRID, g->part, G->d1, stop);
-        k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, k + 1, nullptr, G->d1, -1.0, V, vk1, stop);
-        k_mdot<<<IGRID, IB, 0, st>>>(n, k, vk1, V, g->part, stop);
-        k_fold<<<k + 1, IB, 0, st>>>(IGRID, g->part, G->d2, stop);
-        k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, k + 1, nullptr, G->d2, -1.0, V, vk1, stop);
-        k_mdot<<<IGRID, IB, 0, st>>>(n, 0, vk1, vk1, g->part, stop);
-        k_fold<<<1, IB, 0, st>>>(IGRID, g->part, &G->dn, stop);
-        k_gm_givens<<<1, 1, 0, st>>>(G, k);
-        k_gm_unit<<<IGRID, IB, 0, st>>>(n, G, 1, vk1);
-    };
+    bool have_dx = false;
     for (int cyc = 0; cyc < maxres; ++cyc) {
-        // V0 = r = b - J dx
-        k_scale<<<IGRID, IB, 0, st>>>(n, -1.0, g->R, V);
-        if (cyc > 0) {
-            k_src_jvp<<<IGRID, IB, 0, st>>>(p, Q, g->kf, dx, w, nullptr);
-            Coefs c1{};
-            c1.a[0] = -1.0;
-            k_maxpy<<<IGRID, IB, 0, st>>>(n, 1, c1, 1.0, w, V);
-        }
-        k_mdot<<<IGRID, IB, 0, st>>>(n, 0, V, V, g->part, nullptr);
-        k_fold<<<1, IB, 0, st>>>(IGRID, g->part, &G->dn, nullptr);
-        k_gm_cycle<<<1, 1, 0, st>>>(G, tol, cyc == 0);
-        k_gm_unit<<<IGRID, IB, 0, st>>>(n, G, 0, V);
+        k_gm_start<<<nb, IB, 0, st>>>(p, Q, g->kf, g->R, dx, !have_dx, V, g->part);
+        k_gm_cycle<<<1, FB, 0, st>>>(nb, g->part, G, tol, cyc == 0);
         int launched = 0;
         int chunk = std::max(1, std::min(m, g->last_its));
         while (true) {
-            for (int k = launched; k < launched + chunk; ++k) arnoldi(k);
+            for (int k = launched; k < launched + chunk; ++k) {
+                k_arn_jvp<<<nb, IB, 0, st>>>(p, Q, g->kf, V, n, k, G, g->part);
+                k_fold<<<k + 1, FB, 0, st>>>(nb, g->part, G->d1, stop);
+                k_arn_gs<1><<<nb, IB, 0, st>>>(V, n, N, dim, k, G, g->part);
+                k_fold<<<k + 1, FB, 0, st>>>(nb, g->part, G->d2, stop);
+                k_arn_gs<2><<<nb, IB, 0, st>>>(V, n, N, dim, k, G, g->part);
+                k_gm_givens<<<1, FB, 0, st>>>(nb, g->part, G, k);
+            }
             IHIPCHK(hipGetLastError());
             launched += chunk;
-            if (int rc = read()) return rc;
-            if (g->gd_host->stop || launched >= m) break;
+            IHIPCHK(hipMemcpyAsync(fl, &G->stop, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+            IHIPCHK(hipStreamSynchronize(st));
+            if (fl[0] || launched >= m) break;
             chunk = std::min(2, m - launched);
         }
-        // the cycle converged on its start (stop with no iteration) or after
-        // its iterations: y and dx += V y
-        if (g->gd_host->k > 0) {
-            k_gm_finish<<<1, 1, 0, st>>>(G);
-            k_maxpy_dev<<<IGRID, IB, 0, st>>>(n, 0, &G->k, G->y, 1.0, V, dx, nullptr);
+        // y and dx (+)= V y over the cycle's iterations (none when the
+        // cycle converged on its start)
+        if (fl[1] > 0) {
+            k_gm_update<<<nb, IB, 0, st>>>(G, V, n, N, dim, !have_dx, dx);
             IHIPCHK(hipGetLastError());
+            have_dx = true;
         }
-        *its = g->gd_host->its;
+        *its = fl[2];
         // otherwise (the cycle's m iterations, or a breakdown short of the
         // target) the next cycle restarts from the true residual
-        if (g->gd_host->conv) {
+        if (fl[3]) {
             *converged = 1;
             break;
         }
     }
+    if (!have_dx) IHIPCHK(hipMemsetAsync(dx, 0, sizeof(double) * n, st));
     g->last_its = std::max(1, *its);
     return ZJ_OK;
 }
@@ -412,19 +498,19 @@ int zj_imex_global_create(zj_solver* s, double alpha, int32_t restart, zj_imex_g
     g->dim = zj::solver_dim(s);
     g->n = g->dim * g->N;  // the momentum rows
     g->m = restart;
+    g->nb = (g->N + IB - 1) / IB;
     g->alpha = alpha;
     const size_t naux = (size_t)(3 * g->dim + 1) * g->N;
     if (hipMalloc((void**)&g->aux, sizeof(double) * naux) != hipSuccess ||
         hipMalloc((void**)&g->kf, sizeof(double) * g->N) != hipSuccess ||
         hipMalloc((void**)&g->R, sizeof(double) * g->n) != hipSuccess ||
-        hipMalloc((void**)&g->W, sizeof(double) * g->n) != hipSuccess ||
         hipMalloc((void**)&g->dX, sizeof(double) * g->n) != hipSuccess ||
         hipMalloc((void**)&g->Vb, sizeof(double) * g->n * (size_t)(g->m + 1)) != hipSuccess ||
-        hipMalloc((void**)&g->part, sizeof(double) * IGRID * (MAXK + 1)) != hipSuccess ||
+        hipMalloc((void**)&g->part, sizeof(double) * (size_t)g->nb * (MAXK + 1)) != hipSuccess ||
         hipMalloc((void**)&g->dots, sizeof(double) * (MAXK + 1)) != hipSuccess ||
         hipHostMalloc((void**)&g->host, sizeof(double) * (MAXK + 1)) != hipSuccess ||
         hipMalloc((void**)&g->gd, sizeof(GmresDev)) != hipSuccess ||
-        hipHostMalloc((void**)&g->gd_host, sizeof(GmresDev)) != hipSuccess) {
+        hipHostMalloc((void**)&g->flags, 4 * sizeof(int)) != hipSuccess) {
         zj_imex_global_destroy(g);
         return ZJ_ENOMEM;
     }
@@ -434,11 +520,11 @@ int zj_imex_global_create(zj_solver* s, double alpha, int32_t restart, zj_imex_g
 
 int zj_imex_global_destroy(zj_imex_global* g) {
     if (!g) return ZJ_OK;
-    for (double* p : {g->aux, g->kf, g->R, g->W, g->Vb, g->dX, g->part, g->dots})
+    for (double* p : {g->aux, g->kf, g->R, g->Vb, g->dX, g->part, g->dots})
         if (p) (void)hipFree(p);
     if (g->host) (void)hipHostFree(g->host);
     if (g->gd) (void)hipFree(g->gd);
-    if (g->gd_host) (void)hipHostFree(g->gd_host);
+    if (g->flags) (void)hipHostFree(g->flags);
     delete g;
     return ZJ_OK;
 }
@@ -453,12 +539,11 @@ int zj_imex_global_solve(zj_imex_global* g, double dt, const double* Qexp, doubl
     const zj_params prm = zj::solver_params(g->s);
     SrcPrm p{prm.g, prm.manning_n * prm.manning_n, prm.eps_wet, g->alpha, dt, g->N, g->dim};
     double* corr = g->dX;  // the Newton correction (momentum rows)
-    double* Qmom = Qout + (size_t)2 * g->N;
-    IHIPCHK(hipMemcpyAsync(Qout, Qexp, sizeof(double) * 4 * g->N, hipMemcpyDeviceToDevice, st));
+    k_start<<<g->nb, IB, 0, st>>>(p, Qexp, Qout, g->kf);
+    IHIPCHK(hipGetLastError());
     if (int rc = zj_update_aux(g->s, Qout, g->aux, st)) return rc;
-    k_friction<<<IGRID, IB, 0, st>>>(p, Qout, g->kf);
     double rn = 0.0;
-    if (int rc = residual(g, p, Qout, Qexp, &rn, st)) return rc;
+    if (int rc = residual(g, p, Qout, Qexp, nullptr, 0, &rn, st)) return rc;
     g->newton_its = 0;
     g->gmres_its = 0;
     g->fallbacks = 0;
@@ -468,16 +553,10 @@ int zj_imex_global_solve(zj_imex_global* g, double dt, const double* Qexp, doubl
         rc = gmres(g, p, Qout, corr, gmres_tol, gmres_maxiter, &its, &ok, st);
         if (rc) break;
         g->gmres_its += its;
-        if (!ok) {
-            // GMRES missed its tolerance: the step falls back to -R, as the
-            // reference's jnp.where(info == 0, delta, b)
-            k_scale<<<IGRID, IB, 0, st>>>(g->n, -1.0, g->R, corr);
-            g->fallbacks += 1;
-        }
-        Coefs c{};
-        c.a[0] = 1.0;
-        k_maxpy<<<IGRID, IB, 0, st>>>(g->n, 1, c, 1.0, corr, Qmom);
-        if ((rc = residual(g, p, Qout, Qexp, &rn, st))) break;
+        // GMRES missed its tolerance: the step falls back to -R, as the
+        // reference's jnp.where(info == 0, delta, b)
+        if (!ok) g->fallbacks += 1;
+        if ((rc = residual(g, p, Qout, Qexp, ok ? corr : nullptr, !ok, &rn, st))) break;
         g->newton_its += 1;
     }
     g->rnorm = rn;
