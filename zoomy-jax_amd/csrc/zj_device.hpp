@@ -34,6 +34,15 @@ constexpr int SBLOCK = 256;  // stage kernel: 4 wave64 (3 blocks per CU in FP64,
 
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+// max / min of the limiter bounds and of the face-increment extremes: one
+// v_max / v_min (IEEE maxNum / minNum) where tmax / tmin take a compare and
+// a select per word.  They differ from tmax / tmin only for NaN operands
+// (dropped here) and in the sign of a zero result; a NaN state still
+// reaches the slopes through the least-squares sums.
+__device__ __forceinline__ double bmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double bmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float bmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float bmin(float a, float b) { return __builtin_fminf(a, b); }
 template <typename T> __device__ __forceinline__ T clip01(T x) {
     return x < T(0) ? T(0) : (x > T(1) ? T(1) : x);
 }

@@ -808,8 +808,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         T dv = wn[vv] - w[vv];
                         bx[vv] += T(rx) * dv;
                         by[vv] += T(ry) * dv;
-                        umax[vv] = tmax(umax[vv], wn[vv]);
-                        umin[vv] = tmin(umin[vv], wn[vv]);
+                        umax[vv] = bmax(umax[vv], wn[vv]);
+                        umin[vv] = bmin(umin[vv], wn[vv]);
                     }
                 } else {
                     T nx, ny, len;
@@ -823,8 +823,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
                         T bv = (kind == BC_PERIODIC) ? wg[vv] : T(2) * wg[vv] - w[vv];
-                        umax[vv] = tmax(umax[vv], bv);
-                        umin[vv] = tmin(umin[vv], bv);
+                        umax[vv] = bmax(umax[vv], bv);
+                        umin[vv] = bmin(umin[vv], bv);
                     }
                     const T an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
@@ -919,8 +919,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
                         const T d = face_increment(gx[vv], gy[vv], rx, ry);
-                        dpos[vv] = tmax(dpos[vv], d);
-                        dneg[vv] = tmin(dneg[vv], d);
+                        dpos[vv] = bmax(dpos[vv], d);
+                        dneg[vv] = bmin(dneg[vv], d);
                     }
                 }
 #pragma unroll
@@ -1055,8 +1055,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         const T dv = wn - w;
                         bx += T(rx) * dv;
                         by += T(ry) * dv;
-                        umax = tmax(umax, wn);
-                        umin = tmin(umin, wn);
+                        umax = bmax(umax, wn);
+                        umin = bmin(umin, wn);
                     } else {
                         T nx, ny, len;
                         face_normal<T, LEAN>(m, v, k, nx, ny, len);
@@ -1068,8 +1068,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         const T wg = vs == 0 ? qg[0] : vs == 1 ? qg[1] : vs == 2 ? qg[2] : qg[3];
                         const T rx = T(2) * (fx[k] - cc.x), ry = T(2) * (fy[k] - cc.y);
                         const T bv = (kind == BC_PERIODIC) ? wg : T(2) * wg - w;
-                        umax = tmax(umax, bv);
-                        umin = tmin(umin, bv);
+                        umax = bmax(umax, bv);
+                        umin = bmin(umin, bv);
                         const T an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
                         for (int d = 0; d < 2; ++d) {
@@ -1110,8 +1110,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 ZJ_FOR_FACES(k) {
                     const T rx = T(fx[k] - cc.x), ry = T(fy[k] - cc.y);
                     const T d = face_increment(gx, gy, rx, ry);
-                    dpos = tmax(dpos, d);
-                    dneg = tmin(dneg, d);
+                    dpos = bmax(dpos, d);
+                    dneg = bmin(dneg, d);
                 }
                 T phi = bj_phi<T, LEAN>(dpos, dneg, umax - w, umin - w);
                 if (o1_mask && o1_mask[gi]) phi = T(0);
