@@ -34,17 +34,34 @@ constexpr int SBLOCK = 256;  // stage kernel: 4 wave64 (3 blocks per CU in FP64,
 
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-// max / min of the limiter bounds and of the face-increment extremes: one
-// v_max / v_min (IEEE maxNum / minNum) where tmax / tmin take a compare and
-// a select per word.  They differ from tmax / tmin only for NaN operands
-// (dropped here) and in the sign of a zero result; a NaN state still
-// reaches the slopes through the least-squares sums.
+// max / min of the limiter bounds, the face-increment extremes and the HLL
+// hydrostatic states and wave speeds: one v_max / v_min (IEEE maxNum /
+// minNum) where tmax / tmin take a compare and a select per word.  They
+// differ from tmax / tmin only for NaN operands (dropped here) and in the
+// sign of a zero result; a NaN state still reaches the slopes through the
+// least-squares sums and the fluxes through the face velocities.
 __device__ __forceinline__ double bmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ double bmin(double a, double b) { return __builtin_fmin(a, b); }
 __device__ __forceinline__ float bmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ float bmin(float a, float b) { return __builtin_fminf(a, b); }
+// base[i] through a 32-bit unsigned byte offset, which the compiler turns
+// into the SGPR-base + VGPR-offset form of a global load or store (no
+// 64-bit address arithmetic per lane) when base is uniform; needs
+// i * sizeof(T) < 2^32.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* base, unsigned i) {
+    return *(const T*)((const char*)base + (size_t)(i * (unsigned)sizeof(T)));
+}
+
+template <typename T>
+__device__ __forceinline__ void sto(T* base, unsigned i, T v) {
+    *(T*)((char*)base + (size_t)(i * (unsigned)sizeof(T))) = v;
+}
+
+// x clipped to [0, 1] (a limiter value; a NaN gives 0, and the slope it
+// scales is then NaN itself)
 template <typename T> __device__ __forceinline__ T clip01(T x) {
-    return x < T(0) ? T(0) : (x > T(1) ? T(1) : x);
+    return bmin(bmax(x, T(0)), T(1));
 }
 
 // FP64 square root, reciprocal and quotient by Newton-Raphson from the
@@ -208,15 +225,15 @@ __device__ __forceinline__ void face_flux_left(int rs, const T* qL, const T* qR,
                                                T hiL, T hiR, T nx, T ny, T g,
                                                T eps, T* out) {
     if (rs == RS_HLL) {
-        T bs = tmax(qL[0], qR[0]);
-        T hLs = tmax(T(0), qL[1] + qL[0] - bs);
-        T hRs = tmax(T(0), qR[1] + qR[0] - bs);
+        T bs = bmax(qL[0], qR[0]);
+        T hLs = bmax(T(0), qL[1] + qL[0] - bs);
+        T hRs = bmax(T(0), qR[1] + qR[0] - bs);
         T uL = qL[2] * hiL, vL = qL[3] * hiL;
         T uR = qR[2] * hiR, vR = qR[3] * hiR;
         T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
         T cL = fsqrt_nn<LEAN>(g * hLs), cR = fsqrt_nn<LEAN>(g * hRs);
-        T sL = tmin(unL - cL, unR - cR);
-        T sR = tmax(unL + cL, unR + cR);
+        T sL = bmin(unL - cL, unR - cR);
+        T sR = bmax(unL + cL, unR + cR);
         T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
         T FL0 = hLs * unL, FL1 = hLs * uL * unL + pL * nx, FL2 = hLs * vL * unL + pL * ny;
         T FR0 = hRs * unR, FR1 = hRs * uR * unR + pR * nx, FR2 = hRs * vR * unR + pR * ny;
@@ -269,15 +286,15 @@ template <typename T>
 __device__ __forceinline__ void face_split(int rs, const T* qL, const T* qR, T hiL, T hiR,
                                            T nx, T ny, T g, T eps, T* F, T* Dp, T* Dm) {
     if (rs == RS_HLL) {
-        T bs = tmax(qL[0], qR[0]);
-        T hLs = tmax(T(0), qL[1] + qL[0] - bs);
-        T hRs = tmax(T(0), qR[1] + qR[0] - bs);
+        T bs = bmax(qL[0], qR[0]);
+        T hLs = bmax(T(0), qL[1] + qL[0] - bs);
+        T hRs = bmax(T(0), qR[1] + qR[0] - bs);
         T uL = qL[2] * hiL, vL = qL[3] * hiL;
         T uR = qR[2] * hiR, vR = qR[3] * hiR;
         T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
         T cL = fsqrt(g * hLs), cR = fsqrt(g * hRs);
-        T sL = tmin(unL - cL, unR - cR);
-        T sR = tmax(unL + cL, unR + cR);
+        T sL = bmin(unL - cL, unR - cR);
+        T sR = bmax(unL + cL, unR + cR);
         T pL = T(0.5) * g * hLs * hLs, pR = T(0.5) * g * hRs * hRs;
         T FL[3] = {hLs * unL, hLs * uL * unL + pL * nx, hLs * vL * unL + pL * ny};
         T FR[3] = {hRs * unR, hRs * uR * unR + pR * nx, hRs * vR * unR + pR * ny};

@@ -133,10 +133,8 @@ __device__ __forceinline__ void load_geo(const DevMesh& m, int c, Geo& G) {
 
 template <typename T>
 __device__ __forceinline__ void load_q(const T* __restrict__ Q, int N, int c, T* q) {
-    q[0] = Q[c];
-    q[1] = Q[N + c];
-    q[2] = Q[2 * N + c];
-    q[3] = Q[3 * N + c];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q[r] = ldo(Q + (size_t)r * N, c);
 }
 
 // Reconstruction variables: conservative (b, h, hu, hv), (b, eta, u, v),
@@ -452,11 +450,11 @@ struct TplRow {
                                          const uint32_t* te, int mf, int mxh, int MS, int ls) {
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
-            nbr[k] = k < mf ? tn[k * MS + ls] : LNONE;
-            vert[k] = k < mf ? tv[k * MS + ls] : 0;
+            nbr[k] = k < mf ? ldo(tn, k * MS + ls) : LNONE;
+            vert[k] = k < mf ? ldo(tv, k * MS + ls) : 0;
         }
 #pragma unroll
-        for (int u = 0; u < NB; ++u) ex[u] = u < mxh ? te[u * MS + ls] : ~0u;
+        for (int u = 0; u < NB; ++u) ex[u] = u < mxh ? ldo(te, u * MS + ls) : ~0u;
     }
 };
 
@@ -705,14 +703,18 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     {
         constexpr int SL = SPEC == 1 ? 2 : 3;
         const int* vids = pt.vert_ids + (size_t)p * MV;
+        // the state rows (uniform bases: the loads take 32-bit offsets)
+        const T* Qin1 = Qin + N;
+        const T* Qin2 = Qin + 2 * (size_t)N;
+        const T* Qin3 = Qin + 3 * (size_t)N;
         const int nit = nload > nvert ? nload : nvert;
         for (int i0 = tid; i0 < nit; i0 += SL * SBLOCK) {
             int gc[SL], gv[SL];
 #pragma unroll
             for (int u = 0; u < SL; ++u) {
                 const int i = i0 + u * SBLOCK;
-                gc[u] = (i < nown && !ex) ? c0 + i : (i < nload ? lids[i] : ex ? lids[0] : c0);
-                gv[u] = i < nvert ? vids[i] : 0;
+                gc[u] = (i < nown && !ex) ? c0 + i : (i < nload ? ldo(lids, i) : ex ? lids[0] : c0);
+                gv[u] = i < nvert ? ldo(vids, i) : 0;
             }
             T2 a[SL], b[SL];
             double2 cc[SL], vx[SL];
@@ -720,14 +722,14 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
             for (int u = 0; u < SL; ++u) {
                 const int gi = gc[u];
-                a[u].x = Qin[gi];
-                a[u].y = Qin[N + gi];
-                b[u].x = Qin[2 * N + gi];
-                b[u].y = Qin[3 * N + gi];
-                cc[u] = pt.cen2[gi];
+                a[u].x = ldo(Qin, gi);
+                a[u].y = ldo(Qin1, gi);
+                b[u].x = ldo(Qin2, gi);
+                b[u].y = ldo(Qin3, gi);
+                cc[u] = ldo(pt.cen2, gi);
                 // inradii: the slope cells' (own + face ring) enter the dt law
-                rr[u] = (final_stage && i0 + u * SBLOCK < nslope) ? m.rin[gi] : 0.0;
-                vx[u] = m.verts[gv[u]];
+                rr[u] = (final_stage && i0 + u * SBLOCK < nslope) ? ldo(m.rin, gi) : 0.0;
+                vx[u] = ldo(m.verts, gv[u]);
             }
 #pragma unroll
             for (int u = 0; u < SL; ++u) {
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         const int nsl4 = SPLIT ? (nsl < nown ? nsl : nown) : nsl;
         for (int ls = tid; ls < nsl4; ls += SBLOCK) {
             if (!PRE_TPL || ls != tid) row.load(tn, tv, te, m.mf, mxh, MS, ls);
-            const int gi = (ls < nown && !ex) ? c0 + ls : lids[ls];
+            const int gi = (ls < nown && !ex) ? c0 + ls : ldo(lids, ls);
             const uint16_t* nls = row.nbr;
             const uint32_t* pre = row.ex;
             T q[4];
@@ -862,7 +864,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     uint32_t els[MXB > 0 ? MXB / 2 : 1];
 #pragma unroll
                     for (int u = 0; u < MXB / 2; ++u)
-                        els[u] = b2 == 0 ? pre[u] : b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
+                        els[u] = b2 == 0 ? pre[u] : b2 + u < mxh ? ldo(te, (b2 + u) * MS + ls) : ~0u;
 #pragma unroll
                     for (int j = 0; j < MXB; ++j) {
                         const uint16_t el = (uint16_t)(els[j >> 1] >> (16 * (j & 1)));
@@ -1035,7 +1037,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 if (ls >= nsl) break;
                 TplRow<MXB / 2> rw;
                 rw.load(tn, tv, te, m.mf, mxh, MS, ls);
-                const int gi = lids[ls];
+                const int gi = ldo(lids, ls);
                 const T w = pick(ls);
                 double2 v[MAXF];
                 T fx[MAXF], fy[MAXF];
@@ -1095,7 +1097,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     uint32_t els[MXB > 0 ? MXB / 2 : 1];
 #pragma unroll
                     for (int u = 0; u < MXB / 2; ++u)
-                        els[u] = b2 == 0 ? rw.ex[u] : b2 + u < mxh ? te[(b2 + u) * MS + ls] : ~0u;
+                        els[u] = b2 == 0 ? rw.ex[u] : b2 + u < mxh ? ldo(te, (b2 + u) * MS + ls) : ~0u;
 #pragma unroll
                     for (int j = 0; j < MXB; ++j) {
                         const uint16_t el = (uint16_t)(els[j >> 1] >> (16 * (j & 1)));
@@ -1131,7 +1133,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     T dt3 = T(0);
     TplRow<0> row3;
     if (AUXONLY != 1 && tid < nown) {
-        gi3 = ex ? lids[tid] : c0 + tid;
+        gi3 = ex ? ldo(lids, tid) : c0 + tid;
         if (PRE_TPL) row3.load(tn, tv, te, m.mf, mxh, MS, tid);
         if (AUXONLY == 0) {
             if (mode == 1) load_q(Q0, N, gi3, q0);
@@ -1296,11 +1298,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 // (in the stage precision; the products of the comparison
                 // are exact in FP64 for FP32 operands)
                 const T hn = hinv_of<T, LEAN>(out[1], eps);
-                const T c = fsqrt_nn<LEAN>(g * tmax(out[1], T(0)));
+                const T c = fsqrt_nn<LEAN>(g * bmax(out[1], T(0)));
                 const T rc = Rs[ls];
                 ZJ_FOR_FACES(k) {
                     const uint16_t nl = nls[k];
-                    const T rf = nl != LNONE ? tmin(rc, Rs[nl]) : rc;
+                    const T rf = nl != LNONE ? bmin(rc, Rs[nl]) : rc;
                     const T a = fabs((out[2] * nxs[k] + out[3] * nys[k]) * hn) + c;
                     if (!(a == a)) { best_r = 0.0; best_a = 1.0; }
                     else if ((double)rf * best_a < best_r * (double)a) { best_r = rf; best_a = a; }
@@ -1347,10 +1349,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
     }
     if (gi_keep >= 0) {
-        if (write_b) Qout[gi_keep] = out_keep[0];
-        Qout[N + gi_keep] = out_keep[1];
-        Qout[2 * N + gi_keep] = out_keep[2];
-        Qout[3 * N + gi_keep] = out_keep[3];
+        if (write_b) sto(Qout, gi_keep, out_keep[0]);
+        sto(Qout + N, gi_keep, out_keep[1]);
+        sto(Qout + 2 * (size_t)N, gi_keep, out_keep[2]);
+        sto(Qout + 3 * (size_t)N, gi_keep, out_keep[3]);
     }
 }
 
