@@ -40,10 +40,30 @@ template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b
 // differ from tmax / tmin only for NaN operands (dropped here) and in the
 // sign of a zero result; a NaN state still reaches the slopes through the
 // least-squares sums and the fluxes through the face velocities.
-__device__ __forceinline__ double bmax(double a, double b) { return __builtin_fmax(a, b); }
-__device__ __forceinline__ double bmin(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ float bmax(float a, float b) { return __builtin_fmaxf(a, b); }
-__device__ __forceinline__ float bmin(float a, float b) { return __builtin_fminf(a, b); }
+// (Written as the instructions themselves: through fmax / fmin the
+// compiler quiets each operand it cannot prove canonical -- every value
+// read from LDS -- with one more v_max first.  The operands here are never
+// signalling NaNs.)
+__device__ __forceinline__ double bmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double bmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float bmax(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float bmin(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 // base[i] through a 32-bit unsigned byte offset, which the compiler turns
 // into the SGPR-base + VGPR-offset form of a global load or store (no
 // 64-bit address arithmetic per lane) when base is uniform; needs

@@ -810,8 +810,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         T dv = wn[vv] - w[vv];
                         bx[vv] += T(rx) * dv;
                         by[vv] += T(ry) * dv;
-                        umax[vv] = bmax(umax[vv], wn[vv]);
-                        umin[vv] = bmin(umin[vv], wn[vv]);
+                        if (AUXONLY != 1) {  // the aux rows need no bounds
+                            umax[vv] = bmax(umax[vv], wn[vv]);
+                            umin[vv] = bmin(umin[vv], wn[vv]);
+                        }
                     }
                 } else {
                     T nx, ny, len;
@@ -825,8 +827,10 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
 #pragma unroll
                     for (int vv = 0; vv < 4; ++vv) {
                         T bv = (kind == BC_PERIODIC) ? wg[vv] : T(2) * wg[vv] - w[vv];
-                        umax[vv] = bmax(umax[vv], bv);
-                        umin[vv] = bmin(umin[vv], bv);
+                        if (AUXONLY != 1) {
+                            umax[vv] = bmax(umax[vv], bv);
+                            umin[vv] = bmin(umin[vv], bv);
+                        }
                     }
                     const T an[2] = {fabs(nx), fabs(ny)};
 #pragma unroll
