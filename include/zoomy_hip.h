@@ -313,7 +313,9 @@ int zj_imex_local(zj_solver *s, const double *dt, int32_t maxiter,
  * tolerance gmres_tol, at most gmres_maxiter restart cycles; a GMRES that
  * misses its tolerance takes delta = -R (the reference's fallback).  The
  * Newton loop stops at maxiter steps or ||R||_2 <= tol.  FP64 solvers only;
- * host-synchronous (the Krylov scalars are read by the host). */
+ * host-synchronous: the Krylov scalars stay on the device, the host reads
+ * ||R|| once per Newton step and the GMRES convergence flags once per chunk
+ * of Arnoldi iterations. */
 typedef struct zj_imex_global zj_imex_global;
 int zj_imex_global_create(zj_solver *s, double alpha, int32_t restart,
                           zj_imex_global **out);
