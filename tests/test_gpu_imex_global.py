@@ -100,3 +100,69 @@ def test_imex_step_with_global_source_against_oracle():
     with pytest.raises(NotImplementedError):
         from zoomy_hip.fvm import HyperbolicSolver
         HyperbolicSolver().setup_simulation(mesh, model)
+
+
+def _global_solver(mesh, restart, alpha=0.5, n=0.05):
+    from zoomy_hip.fvm import IMEXSourceSolver, ShallowWater
+    model = ShallowWater(dimension=mesh.dim, order=2, riemann="hll",
+                         manning_n=n, depth_gradient_source=alpha)
+    s = IMEXSourceSolver(time_end=np.inf, cfl=0.9)
+    s.gmres_restart = restart
+    s.setup_simulation(mesh, model)
+    assert s.resolved_source_mode() == "global"
+    return s
+
+
+def _solve(s, Q, dt, maxiter, tol, gmres_tol, gmres_maxiter):
+    import ctypes
+    from zoomy_hip import _lib as L
+    Qe = torch.as_tensor(Q, device="cuda")
+    out = torch.empty_like(Qe)
+    rc = L.lib().zj_imex_global_solve(
+        s._global, dt, ctypes.c_void_p(Qe.data_ptr()),
+        ctypes.c_void_p(out.data_ptr()), maxiter, tol, gmres_tol,
+        gmres_maxiter, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    return out.cpu().numpy(), s.implicit_stats()
+
+
+@pytest.mark.parametrize("restart", [1, 2])
+def test_restarted_gmres_matches_oracle_newton(restart):
+    """GMRES(1) and GMRES(2) (a Newton step here takes 3 Krylov iterations
+    unrestarted): every Newton step needs several restart
+    cycles (the device restarts from the true residual -R - J dx and adds
+    each cycle's correction to dx), and the root still equals the oracle's
+    dense Newton root."""
+    from oracle import imex_global as IG
+    from oracle.oracle import Oracle
+    mesh, Q = _case("quad")
+    dt, alpha, n = 0.02, 0.5, 0.05
+    s = _global_solver(mesh, restart, alpha, n)
+    Qg, st = _solve(s, Q, dt, 10, 1e-11, 1e-10, 60)
+    print("restart", restart, st, flush=True)
+    assert st["fallbacks"] == 0 and st["rnorm"] <= 1e-11
+    # more Krylov iterations than one cycle holds, in some Newton step
+    assert st["gmres"] > restart * st["newton"]
+    o = Oracle(mesh, order=2, riemann="hll", manning_n=n)
+    Qo, _ = IG.newton(o, Q, dt, alpha, n=n)
+    assert max_rel_rows(Qg, Qo, (1, 2, 3)) < 1e-10
+
+
+def test_gmres_miss_falls_back_to_minus_residual():
+    """One GMRES(1) cycle against a tolerance it cannot reach: every Newton
+    step takes delta = -R instead (the reference's jnp.where(info == 0,
+    delta, b)); the iteration Q <- Q - R(Q) still converges here (dt dS/dQ
+    is small), to the oracle's root."""
+    from oracle import imex_global as IG
+    from oracle.oracle import Oracle
+    mesh, Q = _case("tri")
+    dt, alpha, n = 0.02, 0.5, 0.05
+    s = _global_solver(mesh, 1, alpha, n)
+    Qg, st = _solve(s, Q, dt, 40, 1e-11, 1e-15, 1)
+    print(st, flush=True)
+    assert st["fallbacks"] == st["newton"] >= 1
+    assert st["gmres"] == st["newton"]
+    assert st["rnorm"] <= 1e-11
+    o = Oracle(mesh, order=2, riemann="hll", manning_n=n)
+    Qo, _ = IG.newton(o, Q, dt, alpha, n=n)
+    assert max_rel_rows(Qg, Qo, (1, 2, 3)) < 1e-9
