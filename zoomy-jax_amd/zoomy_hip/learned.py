@@ -19,9 +19,11 @@ V-cycle measured in the operator's energy norm,
 
     loss = (z - p)^T S (z - p) / (p^T rhs),     z = M rhs,
 
-on (rhs, p) pairs generated by the plain solver (``solver_pairs``) and error
-pairs (S e, e) (``error_pairs``).  The network's input r - S V r does not
-depend on its parameters, so the gradient is the network's own backward.
+on (rhs, p) pairs that the plain solver generates: the pressure equations
+of Chorin runs and noise right-hand sides (``solver_pairs``), and the
+right-hand sides S e of random errors e (``error_pairs``), each solved to
+1e-8.  The network's input r - S V r does not depend on its parameters, so
+the gradient is the network's own backward.
 """
 from __future__ import annotations
 
@@ -246,11 +248,13 @@ def solver_pairs(n_pairs=5000, n=256, dx=0.0625, seeds=range(100, 110),
 
 
 def error_pairs(s, pairs, n_pairs, seed=5):
-    """(b, h, S e, e) pairs with random errors e (white, or smoothed by 1
-    or 3 passes) on the operators of ``pairs`` (every other one, in turn):
-    where the solver pairs weight the smooth error the coarse correction
-    already removes, these weight the oscillatory error the network acts on.
-    No solve is needed: the right-hand side is S e."""
+    """(b, h, rhs, p) pairs whose right-hand sides are rhs = S e for random
+    errors e (white, or smoothed by 1 or 3 passes) on the operators of
+    ``pairs`` (every other one, in turn), p the plain solver's solution of
+    S p = rhs (1e-8 relative residual; p equals e to that level): where the
+    other solver pairs weight the smooth error the coarse correction
+    already removes, these weight the oscillatory error the network acts
+    on."""
     gen = torch.Generator().manual_seed(seed)
     ops = [(b, h) for (b, h, _, _) in pairs[::2]]
     n = int(round(ops[0][0].numel() ** 0.5))
@@ -259,7 +263,8 @@ def error_pairs(s, pairs, n_pairs, seed=5):
         b, h = ops[k % len(ops)]
         s.setup_operator(b, h)
         e = _smooth_noise(n, gen, (0, 0, 1, 3)[k % 4]).to(s.device)
-        out.append((b, h, s.apply(e), e))
+        rhs = s.apply(e)
+        out.append((b, h, rhs, s.solve(rhs)))
     return out
 
 
@@ -327,8 +332,10 @@ def main(argv=None):
                                  "preconditioner on solver-generated pairs")
     ap.add_argument("--pairs", type=int, default=5000)
     ap.add_argument("--solver-pairs", type=int, default=3000,
-                    help="of --pairs, how many come from plain solves; the "
-                         "rest are error pairs")
+                    help="of --pairs, how many come from Chorin runs and "
+                         "noise right-hand sides; the rest from error "
+                         "right-hand sides S e (all solved by the plain "
+                         "solver)")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch", type=int, default=16)
