@@ -1,7 +1,9 @@
 """Meshes far below and just off the patch size (256 cells) and the wave
 size (64): a few cells where every cell is a boundary cell, a single
 partial patch, and full patches plus a ragged last one.  The device march
-matches the FP64 oracle march (relative 1e-12) and the simulated time."""
+matches the FP64 oracle march (FP64: relative 1e-12 and the simulated time;
+FP32: relative 1e-5 on the state after the same steps, each with its own
+dt, which agree to 1e-5)."""
 import numpy as np
 import pytest
 
@@ -38,9 +40,10 @@ def _state(mesh):
     return Q
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["line3", "line65", "quad3x2", "quad17x15",
                                   "tri_small", "tri_ragged"])
-def test_small_and_ragged_meshes_march_matches_oracle(kind):
+def test_small_and_ragged_meshes_march_matches_oracle(kind, dtype):
     from oracle.oracle import Oracle
     from zoomy_hip.fvm import HyperbolicSolver, ShallowWater
     mesh = _mesh(kind)
@@ -49,12 +52,14 @@ def test_small_and_ragged_meshes_march_matches_oracle(kind):
     Q0 = _state(mesh)
     o = Oracle(mesh, order=2, limiter="minmod", riemann="hll", cfl=0.9)
     Qo, to, _ = o.march(Q0, n_steps=5)
-    s = HyperbolicSolver(time_end=np.inf, cfl=0.9)
+    td = torch.float64 if dtype == "f64" else torch.float32
+    tol = 1e-12 if dtype == "f64" else 1e-5
+    s = HyperbolicSolver(time_end=np.inf, cfl=0.9, dtype=td)
     s.setup_simulation(mesh, ShallowWater(dimension=mesh.dim, order=2,
                                           limiter="minmod", riemann="hll"))
-    Q = torch.as_tensor(Q0, dtype=torch.float64, device="cuda").contiguous()
+    Q = torch.as_tensor(Q0, dtype=td, device="cuda").contiguous()
     Qg, t_dev, _ = s.march_device(Q, 5)
     rows = (1, 2, 3) if mesh.dim == 2 else (1, 2)
-    assert abs(float(t_dev) - to) <= 1e-12 * to
-    assert max_rel_rows(Qg.cpu().numpy(), Qo, rows) < 1e-12
-    assert abs(float(s.compute_mass(Qg)) - o.mass(Q0)) <= 1e-12 * o.mass(Q0)
+    assert abs(float(t_dev) - to) <= tol * to
+    assert max_rel_rows(Qg.double().cpu().numpy(), Qo, rows) < tol
+    assert abs(float(s.compute_mass(Qg)) - o.mass(Q0)) <= tol * o.mass(Q0)
