@@ -180,6 +180,27 @@ __device__ __forceinline__ void recon_face(int recon, const T* q, T hi, const T*
     }
 }
 
+// The eta reconstruction from a staged state whose momentum rows already
+// hold the velocities (u, v) = (hu, hv) / h -- the products to_w and
+// recon_face form -- so that a stencil row costs no conversion.
+template <typename T>
+__device__ __forceinline__ void to_w_prim(const T* p, T* w) {
+    w[0] = p[0];
+    w[1] = p[1] + p[0];
+    w[2] = p[2];
+    w[3] = p[3];
+}
+template <typename T>
+__device__ __forceinline__ void recon_face_prim(const T* p, const T* s, T rx, T ry, T* out) {
+    T b = p[0] + s[0] * rx + s[1] * ry;
+    T eta = (p[1] + p[0]) + s[2] * rx + s[3] * ry;
+    T u = p[2] + s[4] * rx + s[5] * ry;
+    T v = p[3] + s[6] * rx + s[7] * ry;
+    eta = tmax(eta, b);  // hydrostatic face clip, reconstruction_jax.py:1269-1272
+    T h = eta - b;
+    out[0] = b; out[1] = h; out[2] = u * h; out[3] = v * h;
+}
+
 // Per (slot, cell): the outward unit normal and r_f = min(r_cell, r_nbr)
 // (r_cell on boundary slots) -- what the dt law reads, precomputed once per
 // mesh so that k_dt streams them instead of gathering vertices.
@@ -657,6 +678,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // the lean FP64 reciprocal and quotient (zj_device.hpp) in the
     // instantiations with the numerics fixed at compile time
     constexpr bool LEAN = SPEC == 1 || SPEC == 3;
+    // the wet/dry instantiation stages the velocities (u, v) in place of the
+    // momenta when it reconstructs (its eta variables convert every stencil
+    // row otherwise); the face pass reads the own cell's momenta from Qin
+    constexpr bool PRIM = SPEC == 4 && AUXONLY == 0;
+    const bool prim = PRIM && use_slopes;
     // extra-stencil ids fetched per batch on triangle meshes (a vertex
     // stencil of up to 12 cells); the batch of the run-time numerics
     // (SPEC = 2) is smaller, for its register budget
@@ -736,10 +762,17 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 const int i = i0 + u * SBLOCK;
                 if (i < nload) {
                     Qs[i] = a[u];
-                    Qs[ML + i] = b[u];
                     // with conservative variables only the fluxes read 1/h:
                     // own and face-ring (slope) cells
-                    if (!LEAN || i < nslope) Hs[i] = hinv_of<T, LEAN>(a[u].y, eps);
+                    if (!LEAN || i < nslope) {
+                        const T hv = hinv_of<T, LEAN>(a[u].y, eps);
+                        Hs[i] = hv;
+                        if (prim) {
+                            b[u].x = b[u].x * hv;
+                            b[u].y = b[u].y * hv;
+                        }
+                    }
+                    Qs[ML + i] = b[u];
                     T2 cl;
                     cl.x = T(cc[u].x - org.x);
                     cl.y = T(cc[u].y - org.y);
@@ -783,7 +816,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             lds_q4(Qs, ML, ls, q);
             const T hic = Hs[ls];
             T w[4];
-            to_w(o.recon, q, hic, w);
+            if (PRIM) to_w_prim(q, w);
+            else to_w(o.recon, q, hic, w);
             double2 v[MAXF];
             T fx[MAXF], fy[MAXF];
             face_centres(m, Vs, row.vert, org, v, fx, fy);
@@ -801,7 +835,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 if (nl != LNONE) {
                     T qn[4], wn[4];
                     lds_q4(Qs, ML, nl, qn);
-                    to_w(o.recon, qn, Hs[nl], wn);
+                    if (PRIM) to_w_prim(qn, wn);
+                    else to_w(o.recon, qn, Hs[nl], wn);
                     const T2 cn = Cs[nl];
                     const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                     Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
@@ -821,7 +856,13 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     const int bf = -1 - m.nbr[k * N + gi];
                     const int kind = m.bf_kind[bf];
                     T qg[4], wg[4];
-                    ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
+                    if (PRIM) {
+                        T qc[4];  // the cell's conservative state
+                        load_q(Qin, N, gi, qc);
+                        ghost_state(m, Qin, bf, qc, T(nx), T(ny), qg);
+                    } else {
+                        ghost_state(m, Qin, bf, q, T(nx), T(ny), qg);
+                    }
                     to_w(o.recon, qg, hinv_of<T, LEAN>(qg[1], eps), wg);
                     const T rx = T(2) * (fx[k] - cc.x), ry = T(2) * (fy[k] - cc.y);
 #pragma unroll
@@ -850,7 +891,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             auto extra_row = [&](uint16_t el) {
                 T qn[4], wn[4];
                 lds_q4(Qs, ML, el, qn);
-                to_w(o.recon, qn, Hs[el], wn);
+                if (PRIM) to_w_prim(qn, wn);
+                else to_w(o.recon, qn, Hs[el], wn);
                 const T2 cn = Cs[el];
                 const T rx = cn.x - cc.x, ry = cn.y - cc.y;
                 Sxx += rx * rx; Sxy += rx * ry; Syy += ry * ry;
@@ -1134,12 +1176,14 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // stage 2 reads only the own cell of Q0, which this thread writes last
     int gi3 = 0;
     T q0[4] = {0, 0, 0, 0};
+    T qc3[4] = {0, 0, 0, 0};  // staged velocities: the own cell's conservative state
     T dt3 = T(0);
     TplRow<0> row3;
     if (AUXONLY != 1 && tid < nown) {
         gi3 = ex ? ldo(lids, tid) : c0 + tid;
         if (PRE_TPL) row3.load(tn, tv, te, m.mf, mxh, MS, tid);
         if (AUXONLY == 0) {
+            if (prim) load_q(Qin, N, gi3, qc3);
             if (mode == 1) load_q(Q0, N, gi3, q0);
             dt3 = mode == 2 ? T(0) : T(ENS ? ea.dts[ea.inst[gi3]] : *dtp);
         }
@@ -1203,7 +1247,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
         const T dt = dt3;
         const uint16_t* nls = row3.nbr;
         T q[4];
-        lds_q4(Qs, ML, ls, q);
+        lds_q4(Qs, ML, ls, q);  // with prim: (b, h, u, v) until the update
         const T hic = Hs[ls];
         // the fluxes' 1/h: this stage's state, or in a frozen second stage
         // the step state's (the face reconstruction keeps this stage's)
@@ -1231,7 +1275,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             nys[k] = nyd;
             const T nx = T(nxd), ny = T(nyd);
             T qL[4], qR[4];
-            if (use_slopes) recon_face(o.recon, q, hic, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qL);
+            if (prim) recon_face_prim(q, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qL);
+            else if (use_slopes) recon_face(o.recon, q, hic, sc, T(fx[k] - cc.x), T(fy[k] - cc.y), qL);
             else { qL[0] = q[0]; qL[1] = q[1]; qL[2] = q[2]; qL[3] = q[3]; }
             const uint16_t nl = nls[k];
             T hiR = hic, hfR = hif;
@@ -1250,7 +1295,8 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
                         sn[2 * vv + 1] = sv.y;
                     }
                     const T2 cn = Cs[nl];
-                    recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
+                    if (prim) recon_face_prim(qn, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
+                    else recon_face(o.recon, qn, hiR, sn, T(fx[k] - cn.x), T(fy[k] - cn.y), qR);
                 } else {
                     qR[0] = qn[0]; qR[1] = qn[1]; qR[2] = qn[2]; qR[3] = qn[3];
                 }
@@ -1271,6 +1317,11 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T
             }
         }
         const T ivol = frcp<LEAN>(T(cell_volume(m, v)));
+        if (prim) {
+            // the update reads the conservative momenta
+            q[2] = qc3[2];
+            q[3] = qc3[3];
+        }
         T out[4];
         if (mode == 2) {
 #pragma unroll
